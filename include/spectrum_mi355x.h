@@ -1,0 +1,172 @@
+/*
+ * spectrum_mi355x.h -- C ABI of the MI355X (gfx950) batched FFT -> |X| ->
+ * spectrum-maxima library (libspectrum_mi355x.so).
+ *
+ * Plain pointers and sizes only; every device pointer is a HIP device
+ * allocation owned by the caller (the library never hands back memory it
+ * owns).  All compute entry points are asynchronous on the context's stream
+ * (spx_set_stream); call spx_synchronize() before reading results.
+ * Status: 0 = SPX_OK, otherwise one of SPX_E*; spx_last_error() gives text.
+ *
+ * Transform convention: X[k] = sum_n x[n] exp(-2 pi i n k / N), unnormalised,
+ * as hipfftExecC2C(HIPFFT_FORWARD) in the reference
+ * (src/fft_func/src/fft_processor_rocm.cpp:353) and pocketfft c2c forward with
+ * fct = 1 in its CPU path (include/spectrum/utils/cpu_fft.hpp:68-89).
+ * Layouts: input [batch][n_point] complex64 (beam-major), spectra
+ * [batch][nfft], records spx_max_value[batch][cap] (32 bytes each).
+ * nfft must be a power of two, 2 .. 262144, nfft >= n_point.
+ */
+#ifndef SPECTRUM_MI355X_H
+#define SPECTRUM_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPX_OK 0
+#define SPX_EINVAL 1   /* bad size, null pointer, unsupported length */
+#define SPX_ENOMEM 2   /* device allocation failed */
+#define SPX_EHIP 3     /* HIP runtime error */
+#define SPX_ECAP 4     /* a capacity limit was exceeded */
+#define SPX_ENOTSUP 5  /* feature not available for these arguments */
+
+/* window applied to the n_point input samples before zero padding:
+ * symmetric, denominator n_point - 1 (fft_processor_kernels_rocm.hpp:99-109);
+ * values generated once in FP64 and rounded to FP32 */
+#define SPX_WINDOW_NONE 0
+#define SPX_WINDOW_HANN 1
+#define SPX_WINDOW_HAMMING 2
+#define SPX_WINDOW_BLACKMAN 3
+
+#define SPX_DTYPE_C64 0      /* complex64: interleaved float re, im */
+#define SPX_DTYPE_C32F16 1   /* complex fp16: interleaved half re, im */
+#define SPX_DTYPE_C32BF16 2  /* complex bf16: interleaved bfloat16 re, im */
+
+/* peak record, identical in size, order and meaning to the reference's
+ * MaxValue (include/spectrum/types/spectrum_result_types.hpp:34-43):
+ * phase in degrees, refined_frequency = (index + freq_offset) * fs / nfft */
+typedef struct spx_max_value {
+  uint32_t index;
+  float real, imag;
+  float magnitude;
+  float phase;
+  float freq_offset;
+  float refined_frequency;
+  uint32_t pad;
+} spx_max_value;
+
+typedef struct spx_ctx spx_ctx;
+
+/* ---- context ----------------------------------------------------------
+ * Replaces the IBackend/GpuContext pair the reference facades are built on
+ * (include/spectrum/fft_processor_rocm.hpp:59, spectrum_processor_rocm.hpp:98). */
+int spx_create(int device, spx_ctx** out);
+int spx_destroy(spx_ctx* ctx);
+/* borrow a caller stream (hipStream_t); NULL selects the context's own stream */
+int spx_set_stream(spx_ctx* ctx, void* hip_stream);
+void* spx_get_stream(spx_ctx* ctx);
+int spx_synchronize(spx_ctx* ctx);
+const char* spx_last_error(const spx_ctx* ctx);
+int spx_device_info(spx_ctx* ctx, char* name, size_t name_len, int* cu_count, uint64_t* hbm_bytes);
+/* caller-owned device memory helpers (hipMalloc/hipFree/hipMemcpy on the context device) */
+int spx_alloc(spx_ctx* ctx, uint64_t bytes, void** out);
+int spx_free(spx_ctx* ctx, void* ptr);
+int spx_memcpy_h2d(spx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int spx_memcpy_d2h(spx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- FFT front half ----------------------------------------------------
+ * Zero-pad n_point -> nfft, optional window, forward C2C, unnormalised.
+ * Replaces PadDataOp + hipfftExecC2C inside FFTProcessorROCm::ProcessComplex
+ * (fft_processor_rocm.cpp:306-429, pad_data_op.hpp:82-133). in != out. */
+int spx_fft_c2c(spx_ctx* ctx, const void* in, void* out, uint32_t batch, uint32_t n_point, uint32_t nfft,
+                uint32_t window);
+/* ... and interleaved {|X|, atan2(im,re) radians} float2 output
+ * (FFTProcessorROCm::ProcessMagPhase, fft_processor_rocm.cpp:435-570, MagPhaseOp) */
+int spx_fft_mag_phase(spx_ctx* ctx, const void* in, void* out_float2, uint32_t batch, uint32_t n_point,
+                      uint32_t nfft, uint32_t window);
+/* ... and |X|*scale or |X|^2*scale float output
+ * (FFTProcessorROCm::ProcessMagnitudesToGPU, fft_processor_rocm.cpp:579-659) */
+int spx_fft_magnitude(spx_ctx* ctx, const void* in, float* out, uint32_t batch, uint32_t n_point, uint32_t nfft,
+                      uint32_t window, int squared, float scale);
+
+/* ---- elementwise (no FFT) ------------------------------------------------
+ * {|z|, arg z} interleaved (complex_to_mag_phase, fft_processor_kernels_rocm.hpp:123-137;
+ * ComplexToMagPhaseROCm::Process*, complex_to_mag_phase_rocm.cpp:182-300) */
+int spx_mag_phase(spx_ctx* ctx, const void* in, void* out_float2, uint64_t total);
+/* |z|*inv_n or |z|^2*inv_n (complex_to_magnitude(_squared), fft_processor_kernels_rocm.hpp:143-174;
+ * ComplexToMagPhaseROCm::ProcessMagnitude*, complex_to_mag_phase_rocm.cpp:322-410) */
+int spx_magnitude(spx_ctx* ctx, const void* in, float* out, uint64_t total, float inv_n, int squared);
+
+/* ---- reference peak searches on a spectrum ---------------------------------
+ * ONE_PEAK (4 records per beam) / TWO_PEAKS (8): argmax over [0, sr/2) and
+ * [nfft - sr/2, nfft) with parabolic refinement and right-side mirroring
+ * (post_kernel_one_peak / _two_peaks, fft_kernel_sources_rocm.hpp:122-519;
+ * SpectrumPostOp::Execute, spectrum_post_op.hpp:77-106). */
+int spx_peaks_top(spx_ctx* ctx, const void* spectrum, void* maxval_out, uint32_t batch, uint32_t nfft,
+                  uint32_t search_range, float sample_rate, int two_peaks);
+/* AllMaxima: strict local maxima of |X| in [s0, s1) (s0 >= 1, s1 <= nfft - 1),
+ * ascending, at most cap records per beam, counts[b] = number found (uncapped)
+ * (AllMaximaPipelineROCm::Execute, all_maxima_pipeline_rocm.cpp:258-472;
+ * detect/scan/compact kernels, all_maxima_kernel_sources_rocm.hpp:57-264). */
+int spx_all_maxima(spx_ctx* ctx, const void* spectrum, void* maxval_out, uint32_t* counts, uint32_t batch,
+                   uint32_t nfft, float sample_rate, uint32_t s0, uint32_t s1, uint32_t cap);
+
+/* ---- fused window -> FFT -> |X| -> peaks (one launch) ---------------------
+ * peak_mode SPX_PEAKS_THRESHOLD: the north-star search.  mean and population
+ * std of |X| over [s0, s1) (default the whole band), threshold
+ * T = mean + nsigma*std, peaks = strict local maxima (circular neighbours)
+ * with |X| > T, the first max_peaks in ascending bin order, parabolic (or
+ * learned) sub-bin refinement, plus the global-maximum record.
+ * peak_mode SPX_PEAKS_ALL_MAXIMA: the AllMaxima rule set on a raw signal
+ * (SpectrumProcessorROCm::FindAllMaximaFromCPU / FromGPUPipeline,
+ * spectrum_processor_rocm.cpp:444-608): no threshold, no refinement,
+ * counts[2b+1] uncapped. */
+#define SPX_PEAKS_THRESHOLD 1
+#define SPX_PEAKS_ALL_MAXIMA 2
+#define SPX_REFINE_NONE 0
+#define SPX_REFINE_PARABOLIC 1
+#define SPX_REFINE_LEARNED 2
+
+typedef struct spx_fused_desc {
+  const void* input;        /* [batch][n_point], dtype below */
+  const float* in_scale;    /* optional per-channel scale applied on load (FP16/BF16 pre-scaling) */
+  uint32_t batch, n_point, nfft;
+  uint32_t window;          /* SPX_WINDOW_* */
+  uint32_t dtype;           /* SPX_DTYPE_* */
+  float sample_rate;
+  uint32_t peak_mode;       /* SPX_PEAKS_* */
+  float nsigma;             /* threshold multiplier (SPX_PEAKS_THRESHOLD) */
+  uint32_t max_peaks;       /* records kept per channel */
+  uint32_t search_start, search_end; /* [s0, s1); 0, 0 = whole band */
+  uint32_t refine;          /* SPX_REFINE_* */
+  spx_max_value* peaks_out; /* [batch][max_peaks] */
+  uint32_t* counts_out;     /* [batch][2] = {stored, found} */
+  spx_max_value* gmax_out;  /* optional [batch] global maximum record */
+  float* stats_out;         /* optional [batch][4] = {mean, std, threshold, mean |X|^2} */
+} spx_fused_desc;
+
+int spx_fused_peaks(spx_ctx* ctx, const spx_fused_desc* desc);
+
+/* ---- synthetic signals (host side, deterministic) --------------------------
+ * Sum of complex tones plus complex white Gaussian noise, generated in FP64
+ * with counter-based SplitMix64 streams keyed by (seed, channel) and rounded
+ * to complex64.  kind selects the workload's tone rules (SPX_GEN_*). Tones
+ * drawn per channel are returned in tone_freq/tone_amp ([n_chan][max_tones],
+ * cycles/sample; unused slots 0) when non-NULL. */
+#define SPX_GEN_THREE_TONES 1   /* 3 tones, f~U[-.5,.5), A~U[.5,2], noise 10 dB below the weakest */
+#define SPX_GEN_MULTI_TONE 2    /* 1-8 tones, per-tone SNR ~U[0,20] dB vs unit noise power */
+#define SPX_GEN_DYNAMIC 3       /* tone A=1, tone -60 dB, noise -80 dB, >= 64 bins apart */
+#define SPX_GEN_FRACTIONAL 4    /* 1-16 tones at (k+u)/N, A~U[.5,2], >= 3 bins apart, noise 20 dB below weakest */
+int spx_siggen(uint32_t kind, uint64_t seed, uint32_t n_chan, uint32_t n_point, uint32_t chan_offset,
+               void* out_c64, uint32_t max_tones, double* tone_freq, double* tone_amp, int threads);
+
+/* ---- version ---- */
+const char* spx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPECTRUM_MI355X_H */

@@ -1,0 +1,265 @@
+"""ctypes binding of the C ABI in include/spectrum_mi355x.h (libspectrum_mi355x.so).
+
+This is the thinnest host layer: it loads the HIP library, declares every
+exported entry point, maps status codes to the reference's exception types
+(std::invalid_argument -> ValueError, std::runtime_error -> RuntimeError,
+src/fft_func/src/fft_processor_rocm.cpp:311-314,359-361) and offers a small
+device-buffer helper so callers need neither torch nor hip-python.
+
+There is no CPU fallback: if the library or a GPU is missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libspectrum_mi355x.so")
+
+SPX_OK, SPX_EINVAL, SPX_ENOMEM, SPX_EHIP, SPX_ECAP, SPX_ENOTSUP = 0, 1, 2, 3, 4, 5
+WINDOW = {"none": 0, "hann": 1, "hamming": 2, "blackman": 3}
+DTYPE_C64, DTYPE_C32F16, DTYPE_C32BF16 = 0, 1, 2
+PEAKS_THRESHOLD, PEAKS_ALL_MAXIMA = 1, 2
+REFINE_NONE, REFINE_PARABOLIC, REFINE_LEARNED = 0, 1, 2
+GEN_THREE_TONES, GEN_MULTI_TONE, GEN_DYNAMIC, GEN_FRACTIONAL = 1, 2, 3, 4
+
+# 32-byte peak record, same layout as the reference MaxValue
+MAXVALUE_DTYPE = np.dtype([
+    ("index", "<u4"), ("real", "<f4"), ("imag", "<f4"), ("magnitude", "<f4"),
+    ("phase", "<f4"), ("freq_offset", "<f4"), ("refined_frequency", "<f4"), ("pad", "<u4"),
+])
+assert MAXVALUE_DTYPE.itemsize == 32
+
+
+class SpxFusedDesc(C.Structure):
+    _fields_ = [
+        ("input", C.c_void_p), ("in_scale", C.c_void_p),
+        ("batch", C.c_uint32), ("n_point", C.c_uint32), ("nfft", C.c_uint32),
+        ("window", C.c_uint32), ("dtype", C.c_uint32), ("sample_rate", C.c_float),
+        ("peak_mode", C.c_uint32), ("nsigma", C.c_float), ("max_peaks", C.c_uint32),
+        ("search_start", C.c_uint32), ("search_end", C.c_uint32), ("refine", C.c_uint32),
+        ("peaks_out", C.c_void_p), ("counts_out", C.c_void_p), ("gmax_out", C.c_void_p),
+        ("stats_out", C.c_void_p),
+    ]
+
+
+# (name, restype, argtypes) for every symbol declared in include/spectrum_mi355x.h
+_P, _U32, _U64, _I, _F = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float
+SIGNATURES = [
+    ("spx_create", _I, [_I, C.POINTER(C.c_void_p)]),
+    ("spx_destroy", _I, [_P]),
+    ("spx_set_stream", _I, [_P, _P]),
+    ("spx_get_stream", _P, [_P]),
+    ("spx_synchronize", _I, [_P]),
+    ("spx_last_error", C.c_char_p, [_P]),
+    ("spx_device_info", _I, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+    ("spx_alloc", _I, [_P, _U64, C.POINTER(C.c_void_p)]),
+    ("spx_free", _I, [_P, _P]),
+    ("spx_memcpy_h2d", _I, [_P, _P, _P, _U64]),
+    ("spx_memcpy_d2h", _I, [_P, _P, _P, _U64]),
+    ("spx_fft_c2c", _I, [_P, _P, _P, _U32, _U32, _U32, _U32]),
+    ("spx_fft_mag_phase", _I, [_P, _P, _P, _U32, _U32, _U32, _U32]),
+    ("spx_fft_magnitude", _I, [_P, _P, _P, _U32, _U32, _U32, _U32, _I, _F]),
+    ("spx_mag_phase", _I, [_P, _P, _P, _U64]),
+    ("spx_magnitude", _I, [_P, _P, _P, _U64, _F, _I]),
+    ("spx_peaks_top", _I, [_P, _P, _P, _U32, _U32, _U32, _F, _I]),
+    ("spx_all_maxima", _I, [_P, _P, _P, _P, _U32, _U32, _F, _U32, _U32, _U32]),
+    ("spx_fused_peaks", _I, [_P, C.POINTER(SpxFusedDesc)]),
+    ("spx_siggen", _I, [_U32, _U64, _U32, _U32, _U32, _P, _U32, _P, _P, _I]),
+    ("spx_version", C.c_char_p, []),
+]
+
+_lib = None
+
+
+def load(path: Optional[str] = None) -> C.CDLL:
+    """Load libspectrum_mi355x.so and declare its signatures (raises if absent)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(p)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return [s[0] for s in SIGNATURES]
+
+
+def _ptr(x) -> Optional[int]:
+    """Device address of a DeviceBuffer, a torch tensor, an int or None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if isinstance(x, DeviceBuffer):
+        return x.ptr
+    if hasattr(x, "data_ptr"):
+        return int(x.data_ptr())
+    raise TypeError(f"cannot take a device address of {type(x)!r}")
+
+
+class SpxError(RuntimeError):
+    pass
+
+
+class Context:
+    """One spx_ctx on one device (replaces the reference's IBackend + GpuContext)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.spx_create(device, C.byref(h))
+        if rc != SPX_OK:
+            raise RuntimeError(f"spx_create(device={device}) failed with status {rc} (no usable HIP device?)")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.spx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- status handling
+    def check(self, rc: int, what: str):
+        if rc == SPX_OK:
+            return
+        msg = (self.lib.spx_last_error(self.h) or b"").decode(errors="replace")
+        text = f"{what}: {msg} (status {rc})"
+        if rc == SPX_EINVAL:
+            raise ValueError(text)
+        raise SpxError(text)
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        cu = C.c_int()
+        hbm = C.c_uint64()
+        self.check(self.lib.spx_device_info(self.h, name, 256, C.byref(cu), C.byref(hbm)), "spx_device_info")
+        return {"name": name.value.decode(), "cu_count": cu.value, "hbm_bytes": hbm.value}
+
+    def set_stream(self, stream: Optional[int]):
+        self.check(self.lib.spx_set_stream(self.h, stream), "spx_set_stream")
+
+    def stream(self) -> int:
+        return int(self.lib.spx_get_stream(self.h) or 0)
+
+    def sync(self):
+        self.check(self.lib.spx_synchronize(self.h), "spx_synchronize")
+
+    # -- memory
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, arr.nbytes)
+        b.upload(arr)
+        return b
+
+    # -- entry points (asynchronous; call sync() before reading results)
+    def fft_c2c(self, inp, out, batch, n_point, nfft, window=0):
+        self.check(self.lib.spx_fft_c2c(self.h, _ptr(inp), _ptr(out), batch, n_point, nfft, window), "spx_fft_c2c")
+
+    def fft_mag_phase(self, inp, out, batch, n_point, nfft, window=0):
+        self.check(self.lib.spx_fft_mag_phase(self.h, _ptr(inp), _ptr(out), batch, n_point, nfft, window),
+                   "spx_fft_mag_phase")
+
+    def fft_magnitude(self, inp, out, batch, n_point, nfft, window=0, squared=False, scale=1.0):
+        self.check(self.lib.spx_fft_magnitude(self.h, _ptr(inp), _ptr(out), batch, n_point, nfft, window,
+                                              int(bool(squared)), float(scale)), "spx_fft_magnitude")
+
+    def mag_phase(self, inp, out, total):
+        self.check(self.lib.spx_mag_phase(self.h, _ptr(inp), _ptr(out), total), "spx_mag_phase")
+
+    def magnitude(self, inp, out, total, inv_n=1.0, squared=False):
+        self.check(self.lib.spx_magnitude(self.h, _ptr(inp), _ptr(out), total, float(inv_n), int(bool(squared))),
+                   "spx_magnitude")
+
+    def peaks_top(self, spec, out, batch, nfft, search_range, fs, two_peaks=False):
+        self.check(self.lib.spx_peaks_top(self.h, _ptr(spec), _ptr(out), batch, nfft, search_range, float(fs),
+                                          int(bool(two_peaks))), "spx_peaks_top")
+
+    def all_maxima(self, spec, out, counts, batch, nfft, fs, s0, s1, cap):
+        self.check(self.lib.spx_all_maxima(self.h, _ptr(spec), _ptr(out), _ptr(counts), batch, nfft, float(fs),
+                                           s0, s1, cap), "spx_all_maxima")
+
+    def fused_peaks(self, inp, batch, n_point, nfft, peaks, counts, *, window=1, dtype=DTYPE_C64, fs=1.0,
+                    peak_mode=PEAKS_THRESHOLD, nsigma=6.0, max_peaks=32, s0=0, s1=0, refine=REFINE_PARABOLIC,
+                    gmax=None, stats=None, in_scale=None):
+        d = SpxFusedDesc()
+        d.input = _ptr(inp)
+        d.in_scale = _ptr(in_scale)
+        d.batch, d.n_point, d.nfft = batch, n_point, nfft
+        d.window, d.dtype, d.sample_rate = window, dtype, float(fs)
+        d.peak_mode, d.nsigma, d.max_peaks = peak_mode, float(nsigma), max_peaks
+        d.search_start, d.search_end, d.refine = s0, s1, refine
+        d.peaks_out, d.counts_out = _ptr(peaks), _ptr(counts)
+        d.gmax_out, d.stats_out = _ptr(gmax), _ptr(stats)
+        self.check(self.lib.spx_fused_peaks(self.h, C.byref(d)), "spx_fused_peaks")
+
+
+class DeviceBuffer:
+    """Caller-owned device allocation made through spx_alloc (freed on close/GC)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx.check(ctx.lib.spx_alloc(ctx.h, self.nbytes, C.byref(p)), "spx_alloc")
+        self.ptr = int(p.value or 0)
+
+    def upload(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        self.ctx.check(self.ctx.lib.spx_memcpy_h2d(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes), "h2d")
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("requested more bytes than the buffer holds")
+        self.ctx.check(self.ctx.lib.spx_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            self.ctx.lib.spx_free(self.ctx.h, self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def siggen(kind: int, seed: int, n_chan: int, n_point: int, chan_offset: int = 0, max_tones: int = 16,
+           threads: int = 0, with_tones: bool = False):
+    """Host-side deterministic generator (spx_siggen); returns complex64 [n_chan, n_point]."""
+    lib = load()
+    out = np.empty((n_chan, n_point), dtype=np.complex64)
+    tf = np.zeros((n_chan, max_tones), dtype=np.float64) if with_tones else None
+    ta = np.zeros((n_chan, max_tones), dtype=np.float64) if with_tones else None
+    rc = lib.spx_siggen(kind, seed, n_chan, n_point, chan_offset, out.ctypes.data, max_tones,
+                        tf.ctypes.data if tf is not None else None, ta.ctypes.data if ta is not None else None,
+                        threads)
+    if rc != SPX_OK:
+        raise ValueError(f"spx_siggen failed with status {rc}")
+    return (out, tf, ta) if with_tones else out

@@ -1,0 +1,418 @@
+// Host side of the C ABI: contexts, FP64-generated tables, argument checks,
+// scratch management and kernel dispatch.  No compute happens on the host:
+// every entry point launches HIP kernels on the context stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "spx_kernels.h"
+
+struct spx_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  std::string name;
+  uint64_t hbm = 0;
+  std::string err;
+  std::map<std::tuple<int, uint32_t, uint32_t>, void*> tables;  // (kind, a, b) -> device table
+  void* scratch = nullptr;
+  uint64_t scratch_bytes = 0;
+};
+
+namespace {
+
+enum TableKind { TW = 0, TW4 = 1, WIN = 2 };
+
+int fail(spx_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(spx_ctx* c, hipError_t e, const char* what) {
+  return fail(c, SPX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define SPX_HIP(ctx, expr)                              \
+  do {                                                  \
+    hipError_t e_ = (expr);                             \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+bool is_pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
+
+constexpr uint32_t kMaxN = 1u << 18;
+
+// W_N^i = exp(-2 pi i / N), i in [0, N): computed in FP64, rounded once.
+std::vector<float> make_twiddles(uint32_t N) {
+  std::vector<float> t(2 * (size_t)N);
+  for (uint32_t i = 0; i < N; ++i) {
+    const double a = 2.0 * M_PI * ((double)i / (double)N);
+    t[2 * (size_t)i] = (float)std::cos(a);
+    t[2 * (size_t)i + 1] = (float)-std::sin(a);
+  }
+  return t;
+}
+
+// four-step table: entry [k2 * N1 + n1] = W_N^(n1 * k2), N = N1 * 256.
+std::vector<float> make_fourstep(uint32_t N) {
+  const uint32_t N2 = 256, N1 = N / N2;
+  std::vector<float> t(2 * (size_t)N);
+  for (uint32_t k2 = 0; k2 < N2; ++k2)
+    for (uint32_t n1 = 0; n1 < N1; ++n1) {
+      const uint64_t e = ((uint64_t)n1 * k2) % N;
+      const double a = 2.0 * M_PI * ((double)e / (double)N);
+      const size_t i = (size_t)k2 * N1 + n1;
+      t[2 * i] = (float)std::cos(a);
+      t[2 * i + 1] = (float)-std::sin(a);
+    }
+  return t;
+}
+
+// Symmetric windows over n_point samples, denominator n_point - 1
+// (include/spectrum/kernels/fft_processor_kernels_rocm.hpp:99-109);
+// n_point == 1 gives w = 1 instead of the reference's division by zero.
+std::vector<float> make_window(uint32_t type, uint32_t n) {
+  std::vector<float> w(n, 1.0f);
+  if (n < 2) return w;
+  for (uint32_t p = 0; p < n; ++p) {
+    const double th = 2.0 * M_PI * (double)p / (double)(n - 1);
+    double v = 1.0;
+    if (type == SPX_WINDOW_HANN) v = 0.5 * (1.0 - std::cos(th));
+    else if (type == SPX_WINDOW_HAMMING) v = 0.54 - 0.46 * std::cos(th);
+    else if (type == SPX_WINDOW_BLACKMAN) v = 0.42 - 0.5 * std::cos(th) + 0.08 * std::cos(2.0 * th);
+    w[p] = (float)v;
+  }
+  return w;
+}
+
+int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out) {
+  auto key = std::make_tuple(kind, a, b);
+  auto it = c->tables.find(key);
+  if (it != c->tables.end()) {
+    *out = it->second;
+    return SPX_OK;
+  }
+  std::vector<float> host;
+  if (kind == TW) host = make_twiddles(a);
+  else if (kind == TW4) host = make_fourstep(a);
+  else host = make_window(a, b);
+  void* d = nullptr;
+  SPX_HIP(c, hipSetDevice(c->device));
+  hipError_t e = hipMalloc(&d, host.size() * sizeof(float));
+  if (e != hipSuccess) return fail(c, SPX_ENOMEM, "table allocation failed");
+  SPX_HIP(c, hipMemcpy(d, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice));
+  c->tables[key] = d;
+  *out = d;
+  return SPX_OK;
+}
+
+int ensure_scratch(spx_ctx* c, uint64_t bytes) {
+  if (c->scratch_bytes >= bytes) return SPX_OK;
+  SPX_HIP(c, hipSetDevice(c->device));
+  if (c->scratch) {
+    SPX_HIP(c, hipStreamSynchronize(c->stream));
+    SPX_HIP(c, hipFree(c->scratch));
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+  }
+  hipError_t e = hipMalloc(&c->scratch, bytes);
+  if (e != hipSuccess) {
+    c->scratch = nullptr;
+    return fail(c, SPX_ENOMEM, "scratch allocation failed");
+  }
+  c->scratch_bytes = bytes;
+  return SPX_OK;
+}
+
+int check_fft_shape(spx_ctx* c, const void* in, const void* out, uint32_t batch, uint32_t n_point, uint32_t nfft,
+                    uint32_t window) {
+  if (!c) return SPX_EINVAL;
+  if (!in || !out) return fail(c, SPX_EINVAL, "null device pointer");
+  if (!is_pow2(nfft) || nfft > kMaxN) return fail(c, SPX_EINVAL, "nfft must be a power of two in [2, 262144]");
+  if (n_point == 0 || n_point > nfft) return fail(c, SPX_EINVAL, "n_point must be in [1, nfft]");
+  if (window > SPX_WINDOW_BLACKMAN) return fail(c, SPX_EINVAL, "unknown window type");
+  (void)batch;
+  return SPX_OK;
+}
+
+// Fill table pointers and scratch for a fused launch; returns grid size in *grid.
+int prepare(spx_ctx* c, FusedArgs& a, int* grid) {
+  int rc;
+  if (spx_fused_is_large(a.nfft)) {
+    const uint32_t N1 = a.nfft / 256;
+    const void* t;
+    if ((rc = get_table(c, TW, N1, 0, &t))) return rc;
+    a.tw_a = (const float2*)t;
+    if ((rc = get_table(c, TW, 256, 0, &t))) return rc;
+    a.tw_b = (const float2*)t;
+    if ((rc = get_table(c, TW4, a.nfft, 0, &t))) return rc;
+    a.tw_4 = (const float2*)t;
+    int g = c->cu_count > 0 ? c->cu_count : 256;
+    if ((uint32_t)g > a.batch) g = (int)a.batch;
+    a.slot_bytes = spx_large_slot_bytes(a.nfft);
+    if ((rc = ensure_scratch(c, a.slot_bytes * (uint64_t)g))) return rc;
+    a.scratch = c->scratch;
+    *grid = g;
+  } else {
+    const void* t;
+    if ((rc = get_table(c, TW, a.nfft, 0, &t))) return rc;
+    a.tw_a = (const float2*)t;
+    *grid = (int)a.batch;
+  }
+  return SPX_OK;
+}
+
+int run_fused(spx_ctx* c, FusedArgs& a) {
+  if (a.batch == 0) return SPX_OK;
+  int grid = 0, rc;
+  if ((rc = prepare(c, a, &grid))) return rc;
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_fused(&a, grid, c->stream));
+  return SPX_OK;
+}
+
+int window_ptr(spx_ctx* c, uint32_t window, uint32_t n_point, const float** w) {
+  *w = nullptr;
+  if (window == SPX_WINDOW_NONE) return SPX_OK;
+  const void* t;
+  int rc = get_table(c, WIN, window, n_point, &t);
+  if (rc) return rc;
+  *w = (const float*)t;
+  return SPX_OK;
+}
+
+int dense_fft(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_t n_point, uint32_t nfft, uint32_t window,
+              int kind, float scale) {
+  int rc = check_fft_shape(c, in, out, batch, n_point, nfft, window);
+  if (rc) return rc;
+  if (in == out) return fail(c, SPX_EINVAL, "in-place transform is not supported (in == out)");
+  FusedArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in;
+  a.batch = batch;
+  a.n_point = n_point;
+  a.nfft = nfft;
+  a.dtype = SPX_DTYPE_C64;
+  if ((rc = window_ptr(c, window, n_point, &a.window))) return rc;
+  if (kind == 0) a.spec_out = (float2*)out;
+  else {
+    a.mag_out = (float*)out;
+    a.mag_kind = (uint32_t)kind;
+    a.mag_scale = scale;
+  }
+  a.peak_mode = 0;
+  a.s0 = 0;
+  a.s1 = nfft;
+  return run_fused(c, a);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spx_version(void) { return "spectrum_mi355x 0.1.0 (gfx950)"; }
+
+int spx_create(int device, spx_ctx** out) {
+  if (!out) return SPX_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return SPX_EHIP;
+  if (device < 0 || device >= n) return SPX_EINVAL;
+  spx_ctx* c = new spx_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return SPX_EHIP; }
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) { delete c; return SPX_EHIP; }
+  c->cu_count = p.multiProcessorCount;
+  c->name = p.name;
+  c->hbm = p.totalGlobalMem;
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SPX_EHIP; }
+  c->stream = c->own_stream;
+  *out = c;
+  return SPX_OK;
+}
+
+int spx_destroy(spx_ctx* c) {
+  if (!c) return SPX_EINVAL;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->tables) (void)hipFree(kv.second);
+  if (c->scratch) (void)hipFree(c->scratch);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return SPX_OK;
+}
+
+int spx_set_stream(spx_ctx* c, void* s) {
+  if (!c) return SPX_EINVAL;
+  c->stream = s ? (hipStream_t)s : c->own_stream;
+  return SPX_OK;
+}
+
+void* spx_get_stream(spx_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int spx_synchronize(spx_ctx* c) {
+  if (!c) return SPX_EINVAL;
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, hipStreamSynchronize(c->stream));
+  return SPX_OK;
+}
+
+const char* spx_last_error(const spx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int spx_device_info(spx_ctx* c, char* name, size_t name_len, int* cu_count, uint64_t* hbm_bytes) {
+  if (!c) return SPX_EINVAL;
+  if (name && name_len) {
+    std::strncpy(name, c->name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cu_count) *cu_count = c->cu_count;
+  if (hbm_bytes) *hbm_bytes = c->hbm;
+  return SPX_OK;
+}
+
+int spx_alloc(spx_ctx* c, uint64_t bytes, void** out) {
+  if (!c || !out) return SPX_EINVAL;
+  *out = nullptr;
+  SPX_HIP(c, hipSetDevice(c->device));
+  if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return fail(c, SPX_ENOMEM, "hipMalloc failed");
+  return SPX_OK;
+}
+
+int spx_free(spx_ctx* c, void* p) {
+  if (!c) return SPX_EINVAL;
+  SPX_HIP(c, hipSetDevice(c->device));
+  if (p) SPX_HIP(c, hipFree(p));
+  return SPX_OK;
+}
+
+int spx_memcpy_h2d(spx_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return SPX_EINVAL;
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  SPX_HIP(c, hipStreamSynchronize(c->stream));
+  return SPX_OK;
+}
+
+int spx_memcpy_d2h(spx_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return SPX_EINVAL;
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  SPX_HIP(c, hipStreamSynchronize(c->stream));
+  return SPX_OK;
+}
+
+int spx_fft_c2c(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_t n_point, uint32_t nfft,
+                uint32_t window) {
+  return dense_fft(c, in, out, batch, n_point, nfft, window, 0, 1.0f);
+}
+
+int spx_fft_mag_phase(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_t n_point, uint32_t nfft,
+                      uint32_t window) {
+  return dense_fft(c, in, out, batch, n_point, nfft, window, SPX_MAG_PHASE, 1.0f);
+}
+
+int spx_fft_magnitude(spx_ctx* c, const void* in, float* out, uint32_t batch, uint32_t n_point, uint32_t nfft,
+                      uint32_t window, int squared, float scale) {
+  return dense_fft(c, in, out, batch, n_point, nfft, window, squared ? SPX_MAG_SQUARED : SPX_MAG_ABS, scale);
+}
+
+int spx_mag_phase(spx_ctx* c, const void* in, void* out, uint64_t total) {
+  if (!c) return SPX_EINVAL;
+  if (total == 0) return SPX_OK;
+  if (!in || !out) return fail(c, SPX_EINVAL, "null device pointer");
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_mag_phase(in, out, total, c->stream));
+  return SPX_OK;
+}
+
+int spx_magnitude(spx_ctx* c, const void* in, float* out, uint64_t total, float inv_n, int squared) {
+  if (!c) return SPX_EINVAL;
+  if (total == 0) return SPX_OK;
+  if (!in || !out) return fail(c, SPX_EINVAL, "null device pointer");
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_magnitude(in, out, total, inv_n, squared, c->stream));
+  return SPX_OK;
+}
+
+int spx_peaks_top(spx_ctx* c, const void* spec, void* out, uint32_t batch, uint32_t nfft, uint32_t search_range,
+                  float fs, int two_peaks) {
+  if (!c) return SPX_EINVAL;
+  if (batch == 0) return SPX_OK;
+  if (!spec || !out) return fail(c, SPX_EINVAL, "null device pointer");
+  if (nfft < 2) return fail(c, SPX_EINVAL, "nfft must be >= 2");
+  if (search_range > nfft) search_range = nfft;
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_peaks_top(spec, out, batch, nfft, search_range, fs, two_peaks, c->stream));
+  return SPX_OK;
+}
+
+int spx_all_maxima(spx_ctx* c, const void* spec, void* out, uint32_t* counts, uint32_t batch, uint32_t nfft,
+                   float fs, uint32_t s0, uint32_t s1, uint32_t cap) {
+  if (!c) return SPX_EINVAL;
+  if (batch == 0) return SPX_OK;
+  if (!spec || !counts || (!out && cap)) return fail(c, SPX_EINVAL, "null device pointer");
+  if (nfft < 3) return fail(c, SPX_EINVAL, "nfft must be >= 3");
+  if (s0 < 1 || s1 > nfft - 1 || s0 > s1) return fail(c, SPX_EINVAL, "need 1 <= search_start <= search_end <= nfft-1");
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_all_maxima(spec, out, counts, batch, nfft, fs, s0, s1, cap, c->stream));
+  return SPX_OK;
+}
+
+int spx_fused_peaks(spx_ctx* c, const spx_fused_desc* d) {
+  if (!c) return SPX_EINVAL;
+  if (!d) return fail(c, SPX_EINVAL, "null descriptor");
+  if (d->batch == 0) return SPX_OK;
+  int rc = check_fft_shape(c, d->input, d->peaks_out ? (const void*)d->peaks_out : (const void*)d->counts_out,
+                           d->batch, d->n_point, d->nfft, d->window);
+  if (rc) return rc;
+  if (!d->counts_out || (!d->peaks_out && d->max_peaks)) return fail(c, SPX_EINVAL, "null output pointer");
+  if (d->dtype > SPX_DTYPE_C32BF16) return fail(c, SPX_EINVAL, "unknown dtype");
+  if (d->peak_mode != SPX_PEAKS_THRESHOLD && d->peak_mode != SPX_PEAKS_ALL_MAXIMA)
+    return fail(c, SPX_EINVAL, "unknown peak_mode");
+  if (d->refine > SPX_REFINE_LEARNED) return fail(c, SPX_EINVAL, "unknown refine mode");
+  if (d->refine == SPX_REFINE_LEARNED) return fail(c, SPX_ENOTSUP, "learned refinement needs spx_fused_peaks_mlp");
+  uint32_t s0 = d->search_start, s1 = d->search_end;
+  if (d->peak_mode == SPX_PEAKS_ALL_MAXIMA) {
+    // reference range normalisation, all_maxima_pipeline_rocm.cpp:274-276
+    if (s0 == 0) s0 = 1;
+    if (s1 == 0) s1 = d->nfft / 2;
+    if (s1 >= d->nfft) s1 = d->nfft - 1;
+  } else if (s0 == 0 && s1 == 0) {
+    s1 = d->nfft;
+  }
+  if (s0 >= s1 || s1 > d->nfft) return fail(c, SPX_EINVAL, "empty or out-of-range search window");
+  if (spx_fused_is_large(d->nfft) && d->max_peaks > 2048)
+    return fail(c, SPX_ECAP, "max_peaks above 2048 is not supported for nfft >= 32768");
+  FusedArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = d->input;
+  a.in_scale = d->in_scale;
+  a.batch = d->batch;
+  a.n_point = d->n_point;
+  a.nfft = d->nfft;
+  a.dtype = d->dtype;
+  a.fs = d->sample_rate;
+  if ((rc = window_ptr(c, d->window, d->n_point, &a.window))) return rc;
+  a.peak_mode = d->peak_mode;
+  a.nsigma = d->nsigma;
+  a.kcap = d->max_peaks;
+  a.s0 = s0;
+  a.s1 = s1;
+  a.refine = d->peak_mode == SPX_PEAKS_ALL_MAXIMA ? 0u : d->refine;
+  a.peaks = d->peaks_out;
+  a.counts = d->counts_out;
+  a.gmax = d->gmax_out;
+  a.stats = d->stats_out;
+  return run_fused(c, a);
+}
+
+}  // extern "C"

@@ -1,0 +1,296 @@
+// Device building blocks for the gfx950 FFT -> |X| -> peak-search kernels.
+//
+// Everything here is written for CDNA4 wave64: compile-time radix-2/4/8/16
+// DFT butterflies held in VGPRs, LDS-staged Stockham passes (in place, one
+// butterfly set per thread per pass), and wave/block reductions and scans
+// built on 64-lane shuffles.
+//
+// Transform convention (matches the reference's hipFFT/pocketfft forward C2C,
+// src/fft_func/src/fft_processor_rocm.cpp:353, include/spectrum/utils/cpu_fft.hpp:82-86):
+//   X[k] = sum_n x[n] * exp(-2*pi*i*n*k/N), unnormalised.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spx {
+
+// ---------------------------------------------------------------------------
+// complex helpers (float2 = {re, im})
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 w) {
+  return make_float2(fmaf(a.x, w.x, -a.y * w.y), fmaf(a.x, w.y, a.y * w.x));
+}
+__device__ __forceinline__ float2 c_mul_mj(float2 a) { return make_float2(a.y, -a.x); }  // * (-j)
+__device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+constexpr int ipow16(int p) { return p <= 0 ? 1 : 16 * ipow16(p - 1); }
+
+// cos(pi/8), sin(pi/8), sqrt(1/2): the only irrational constants of a radix-16 butterfly.
+#define SPX_C8 0.92387953251128674f
+#define SPX_S8 0.38268343236508978f
+#define SPX_R2 0.70710678118654752f
+
+// multiply by W16^E = exp(-2*pi*i*E/16), E known at compile time.
+template <int E>
+__device__ __forceinline__ float2 tw16(float2 a) {
+  constexpr int e = E & 15;
+  if constexpr (e == 0) return a;
+  else if constexpr (e == 4) return make_float2(a.y, -a.x);
+  else if constexpr (e == 8) return make_float2(-a.x, -a.y);
+  else if constexpr (e == 12) return make_float2(-a.y, a.x);
+  else if constexpr (e == 2) return make_float2(SPX_R2 * (a.x + a.y), SPX_R2 * (a.y - a.x));
+  else if constexpr (e == 6) return make_float2(SPX_R2 * (a.y - a.x), -SPX_R2 * (a.x + a.y));
+  else if constexpr (e == 10) return make_float2(-SPX_R2 * (a.x + a.y), SPX_R2 * (a.x - a.y));
+  else if constexpr (e == 14) return make_float2(SPX_R2 * (a.x - a.y), SPX_R2 * (a.x + a.y));
+  else {
+    // W16^e = (cos t, -sin t), t = 2*pi*e/16
+    constexpr float c = (e == 1 || e == 15) ? SPX_C8 : (e == 3 || e == 13) ? SPX_S8
+                       : (e == 5 || e == 11) ? -SPX_S8 : -SPX_C8;
+    constexpr float s = (e == 1 || e == 7) ? SPX_S8 : (e == 3 || e == 5) ? SPX_C8
+                       : (e == 9 || e == 15) ? -SPX_S8 : -SPX_C8;
+    // (x + iy)(c - is) = (xc + ys) + i(yc - xs)
+    return make_float2(fmaf(a.x, c, a.y * s), fmaf(a.y, c, -a.x * s));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DFT butterflies, natural-order in and out, in registers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+  float2 t = a;
+  a = c_add(t, b);
+  b = c_sub(t, b);
+}
+
+__device__ __forceinline__ void dft4(float2& a, float2& b, float2& c, float2& d) {
+  float2 t0 = c_add(a, c), t1 = c_sub(a, c), t2 = c_add(b, d), t3 = c_mul_mj(c_sub(b, d));
+  a = c_add(t0, t2);
+  c = c_sub(t0, t2);
+  b = c_add(t1, t3);
+  d = c_sub(t1, t3);
+}
+
+template <int R> struct Dft;
+template <> struct Dft<1> { __device__ __forceinline__ static void run(float2*) {} };
+template <> struct Dft<2> { __device__ __forceinline__ static void run(float2* v) { dft2(v[0], v[1]); } };
+template <> struct Dft<4> {
+  __device__ __forceinline__ static void run(float2* v) { dft4(v[0], v[1], v[2], v[3]); }
+};
+// 8 = 2 x 4: n = 2*n2 + n1, k = k1 + 4*k2
+template <> struct Dft<8> {
+  __device__ __forceinline__ static void run(float2* v) {
+    dft4(v[0], v[2], v[4], v[6]);
+    dft4(v[1], v[3], v[5], v[7]);
+    // A[n1][k1] lives at v[n1 + 2*k1]; twiddle W8^(n1*k1) = W16^(2*n1*k1)
+    v[3] = tw16<2>(v[3]);
+    v[5] = tw16<4>(v[5]);
+    v[7] = tw16<6>(v[7]);
+    float2 o[8];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      float2 a = v[2 * k1], b = v[2 * k1 + 1];
+      o[k1] = c_add(a, b);
+      o[k1 + 4] = c_sub(a, b);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = o[i];
+  }
+};
+// 16 = 4 x 4: n = 4*n2 + n1, k = k1 + 4*k2
+template <> struct Dft<16> {
+  __device__ __forceinline__ static void run(float2* v) {
+    dft4(v[0], v[4], v[8], v[12]);
+    dft4(v[1], v[5], v[9], v[13]);
+    dft4(v[2], v[6], v[10], v[14]);
+    dft4(v[3], v[7], v[11], v[15]);
+    // A[n1][k1] at v[n1 + 4*k1]; twiddle W16^(n1*k1)
+    v[5] = tw16<1>(v[5]);
+    v[9] = tw16<2>(v[9]);
+    v[13] = tw16<3>(v[13]);
+    v[6] = tw16<2>(v[6]);
+    v[10] = tw16<4>(v[10]);
+    v[14] = tw16<6>(v[14]);
+    v[7] = tw16<3>(v[7]);
+    v[11] = tw16<6>(v[11]);
+    v[15] = tw16<9>(v[15]);
+    dft4(v[0], v[1], v[2], v[3]);
+    dft4(v[4], v[5], v[6], v[7]);
+    dft4(v[8], v[9], v[10], v[11]);
+    dft4(v[12], v[13], v[14], v[15]);
+    // X[k1 + 4*k2] sits at v[4*k1 + k2]: transpose (register renaming only)
+    float2 o[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) o[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = o[i];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Radix plan for a power-of-two length L: the odd radix first (no twiddles
+// there), then radix-16 passes.  L = 2^a: passes = ceil(a/4).
+// ---------------------------------------------------------------------------
+template <int L>
+struct Plan {
+  static constexpr int LOG = ilog2(L);
+  static constexpr int NP = (LOG + 3) / 4;
+  static constexpr int R0 = (LOG % 4) ? (1 << (LOG % 4)) : 16;
+  static constexpr int radix(int p) { return p == 0 ? R0 : 16; }
+  static constexpr int ns(int p) { return p == 0 ? 1 : R0 * ipow16(p - 1); }
+  static constexpr int RL = radix(NP - 1);  // radix of the last pass
+};
+
+// ---------------------------------------------------------------------------
+// LDS layouts for a tile of NSEQ sequences of length L.
+//  Contig: element i of sequence s at pad(s*L + i), pad(a) = a + a/16
+//          (one spare slot per 16 complex breaks the 128-byte stride of the
+//          radix-16 writes across the 64 LDS banks).
+//  Inter:  element i of sequence s at i*NSEQ + s (sequences interleaved; the
+//          lanes of a wave walk s, so every access is unit-stride).
+// Thread mapping of butterfly j (radix R): Contig walks jj fastest, Inter walks s.
+// ---------------------------------------------------------------------------
+template <int L, int NSEQ>
+struct LayContig {
+  static constexpr int SIZE = L * NSEQ + (L * NSEQ) / 16;
+  __device__ __forceinline__ static int idx(int s, int i) {
+    int a = s * L + i;
+    return a + (a >> 4);
+  }
+  template <int R>
+  __device__ __forceinline__ static void map(int j, int& s, int& jj) {
+    s = j / (L / R);
+    jj = j % (L / R);
+  }
+};
+
+template <int L, int NSEQ>
+struct LayInter {
+  static constexpr int SIZE = L * NSEQ;
+  __device__ __forceinline__ static int idx(int s, int i) { return i * NSEQ + s; }
+  template <int R>
+  __device__ __forceinline__ static void map(int j, int& s, int& jj) {
+    s = j % NSEQ;
+    jj = j / NSEQ;
+  }
+};
+
+// One Stockham pass (radix R, current sub-transform length NS) over a tile of
+// NSEQ sequences of length L held by T threads with PPT points each.
+//  in : v[b*R + r] = x[s][jj + r*L/R]   (loaded from LDS unless FIRST)
+//  out: LDS y[s][(jj/NS)*NS*R + jj%NS + r*NS]           (unless LAST)
+//       or, for LAST, v[b*R + r] = X[s][jj + r*L/R] in registers.
+// Twiddles W_(NS*R)^(r*(jj%NS)) = tw[r*(jj%NS)*L/(NS*R)], tw[i] = W_L^i.
+template <class LAY, int L, int NSEQ, int R, int NS, int T, int PPT, bool FIRST, bool LAST>
+__device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[PPT],
+                                         const float2* __restrict__ tw, int tid) {
+  constexpr int BPT = PPT / R;
+  constexpr int NB = NSEQ * L / R;            // butterflies in the tile
+  constexpr bool FULL = (NB == T * BPT);      // every thread busy
+  if constexpr (!FIRST) {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = tid + b * T;
+      if (FULL || j < NB) {
+        int s, jj;
+        LAY::template map<R>(j, s, jj);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b * R + r] = lds[LAY::idx(s, jj + r * (L / R))];
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = tid + b * T;
+    if (FULL || j < NB) {
+      if constexpr (NS > 1) {
+        int s, jj;
+        LAY::template map<R>(j, s, jj);
+        const int k = jj % NS;
+        constexpr int STEP = L / (NS * R);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], tw[r * k * STEP]);
+      }
+      Dft<R>::run(&v[b * R]);
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = tid + b * T;
+      if (FULL || j < NB) {
+        int s, jj;
+        LAY::template map<R>(j, s, jj);
+        const int base = (jj / NS) * NS * R + (jj % NS);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[LAY::idx(s, base + r * NS)] = v[b * R + r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <class LAY, int L, int NSEQ, int T, int PPT, int P>
+struct FftPasses {
+  __device__ __forceinline__ static void run(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
+    using PL = Plan<L>;
+    fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1>(lds, v, tw, tid);
+    if constexpr (P + 1 < PL::NP) FftPasses<LAY, L, NSEQ, T, PPT, P + 1>::run(lds, v, tw, tid);
+  }
+};
+
+// Full length-L transform of the tile.  Before the call v holds the first
+// pass's inputs (x[s][jj + r*L/R0] for the radix-R0 mapping of j = tid + b*T);
+// after it v holds X[s][jj + r*L/RL] for the radix-RL mapping.
+template <class LAY, int L, int NSEQ, int T, int PPT>
+__device__ __forceinline__ void fft_tile(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
+  FftPasses<LAY, L, NSEQ, T, PPT, 0>::run(lds, v, tw, tid);
+}
+
+// ---------------------------------------------------------------------------
+// wave64 / block reductions and scans
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ float wave_sumf(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+// argmax with lowest index on ties; (m, k) pairs
+__device__ __forceinline__ void wave_argmax(float& m, uint32_t& k) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    float om = __shfl_xor(m, o, 64);
+    uint32_t ok = __shfl_xor(k, o, 64);
+    if (om > m || (om == m && ok < k)) { m = om; k = ok; }
+  }
+}
+// inclusive scan of ints across the wave
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+struct BlockRed {
+  double s1[16], s2[16];
+  float m[16];
+  uint32_t k[16];
+  int c[16];
+  int total;
+};
+
+}  // namespace spx

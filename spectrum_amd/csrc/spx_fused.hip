@@ -1,0 +1,771 @@
+// Fused window -> FFT -> |X| -> statistics -> local maxima -> threshold ->
+// ascending compaction -> sub-bin refinement, for gfx950.
+//
+// Two kernel families, both one workgroup per channel (no cross-workgroup
+// synchronisation, so no agent-scope L2 write-back/invalidate is ever needed):
+//
+//  * small  (N <= 16384): the whole channel lives in LDS.  The first Stockham
+//    pass reads the input straight from HBM in butterfly order (window applied
+//    in registers), later passes go through LDS, the last pass stays in VGPRs.
+//    |X|, mean/std, local maxima, the block scan and the records are produced
+//    from the LDS copy of the spectrum; nothing but records leaves the chip.
+//
+//  * large  (N = N1 * 256, N >= 32768): persistent grid, one workgroup per CU,
+//    channels strided over workgroups.  Phase 1 runs 256-point column FFTs
+//    (64 columns per tile, interleaved LDS layout) and writes the twiddled
+//    intermediate Z[k2][n1] to a per-workgroup scratch slot (L2/MALL
+//    resident); phase 2 runs N1-point row FFTs; local maxima of interior rows
+//    are decided from the LDS magnitude tile, tile-boundary rows from a small
+//    edge buffer.  Only local maxima that can clear the threshold are kept as
+//    candidates: by Parseval mean|X|^2 = sum|x_w|^2 =: P, and mean + k*std
+//    >= min(1,k)*sqrt(P), so candidates below that bound are dropped before
+//    the mean and std are known.  Phase 3 filters candidates by the final
+//    threshold and selects the first K in ascending bin order (bucket
+//    histogram + bitonic sort of the boundary bucket).
+//
+// Peak rules (peak_mode 1, the fused north-star path; no reference kernel):
+//   m[k] = |X[k]| (FP32), range [s0,s1) (default the whole band, circular
+//   neighbours), mean/std over the range (population std), T = mean + nsigma*std,
+//   peak iff m[k] > m[k-1] && m[k] > m[k+1] && m[k] > T, first K in ascending k,
+//   parabolic delta = 0.5(yl-yr)/(yl-2yc+yr) (|den| > 1e-10, clamped to +-0.5)
+//   as in include/spectrum/kernels/fft_kernel_sources_rocm.hpp:228-239.
+// peak_mode 2 is the AllMaxima rule set (all_maxima_kernel_sources_rocm.hpp:57-87,
+// 205-264): no threshold, record freq_offset 0 and f = k*fs/N, count uncapped.
+#include "spx_device.hpp"
+#include "spx_types.h"
+#include "spx_kernels.h"
+
+namespace spx {
+
+using MaxValue = spx_max_value;
+
+// ---------------------------------------------------------------------------
+// input loaders (complex64, complex fp16, complex bf16)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__device__ __forceinline__ float2 load_in(const FusedArgs& a, uint64_t ch, uint32_t n) {
+  if (n >= a.n_point) return make_float2(0.f, 0.f);
+  const uint64_t i = ch * (uint64_t)a.n_point + n;
+  float2 x;
+  if (a.dtype == SPX_DTYPE_C64) {
+    x = reinterpret_cast<const float2*>(a.in)[i];
+  } else if (a.dtype == SPX_DTYPE_C32F16) {
+    const __half2 h = reinterpret_cast<const __half2*>(a.in)[i];
+    x = __half22float2(h);
+  } else {
+    const uint32_t u = reinterpret_cast<const uint32_t*>(a.in)[i];
+    x = make_float2(bf16_to_f((uint16_t)(u & 0xffffu)), bf16_to_f((uint16_t)(u >> 16)));
+  }
+  if (a.in_scale != nullptr) x = c_scale(x, a.in_scale[ch]);
+  if (a.window != nullptr) x = c_scale(x, a.window[n]);
+  return x;
+}
+
+// Hide a pointer's value from the optimiser so that loads through it are not
+// hoisted out of the persistent channel loop (they are channel-invariant, and
+// keeping them all live would spill).
+__device__ __forceinline__ int opaque_v(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+template <class P>
+__device__ __forceinline__ P* opaque(P* p) {
+  uint32_t z = 0;
+  asm volatile("" : "+s"(z));
+  return p + z;
+}
+
+__device__ __forceinline__ float mag_of(float2 x) { return __builtin_sqrtf(fmaf(x.x, x.x, x.y * x.y)); }
+
+__device__ __forceinline__ float parabolic(float yl, float yc, float yr) {
+  float den = yl - 2.0f * yc + yr;
+  float d = 0.0f;
+  if (fabsf(den) > 1e-10f) {
+    d = 0.5f * (yl - yr) / den;
+    d = fminf(fmaxf(d, -0.5f), 0.5f);
+  }
+  return d;
+}
+
+__device__ __noinline__ void write_record(MaxValue* dst, uint32_t k, float2 x, float yl, float yc, float yr,
+                                             const FusedArgs& a) {
+  float d = (a.refine != 0) ? parabolic(yl, yc, yr) : 0.0f;
+  MaxValue mv;
+  mv.index = k;
+  mv.real = x.x;
+  mv.imag = x.y;
+  mv.magnitude = yc;
+  mv.phase = atan2f(x.y, x.x) * 57.29577951f;
+  mv.freq_offset = d;
+  mv.refined_frequency = ((float)k + d) * (a.fs / (float)a.nfft);
+  mv.pad = 0;
+  *dst = mv;
+}
+
+__device__ __forceinline__ bool in_range(uint32_t k, const FusedArgs& a) { return k >= a.s0 && k < a.s1; }
+
+// dense spectrum / magnitude outputs (phase needs atan2f: kept out of line)
+__device__ __noinline__ void store_magphase(float2* dst, float2 x) { *dst = make_float2(mag_of(x), atan2f(x.y, x.x)); }
+__device__ __forceinline__ void store_dense(const FusedArgs& a, uint64_t i, float2 x) {
+  if (a.spec_out) a.spec_out[i] = x;
+  if (a.mag_out) {
+    if (a.mag_kind == SPX_MAG_ABS) a.mag_out[i] = mag_of(x) * a.mag_scale;
+    else if (a.mag_kind == SPX_MAG_SQUARED) a.mag_out[i] = fmaf(x.x, x.x, x.y * x.y) * a.mag_scale;
+    else store_magphase(reinterpret_cast<float2*>(a.mag_out) + i, x);
+  }
+}
+
+// threshold from the block-reduced sums; -inf when no threshold applies
+__device__ __forceinline__ float threshold_of(const FusedArgs& a, double s1, double s2, double cnt, float& mu_out,
+                                              float& sd_out) {
+  double mu = cnt > 0 ? s1 / cnt : 0.0;
+  double var = cnt > 0 ? s2 / cnt - mu * mu : 0.0;
+  if (var < 0) var = 0;
+  double sd = sqrt(var);
+  mu_out = (float)mu;
+  sd_out = (float)sd;
+  if (a.peak_mode == 1) return (float)(mu + (double)a.nsigma * sd);
+  return -__builtin_inff();
+}
+
+// Block-wide sum of two doubles and argmax (lowest index wins ties).
+// Result valid in all threads on return.
+template <int T>
+__device__ __forceinline__ void block_stats(double& s1, double& s2, float& bm, uint32_t& bk, BlockRed& red) {
+  constexpr int NW = (T + 63) / 64;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  wave_argmax(bm, bk);
+  if (lane == 0) {
+    red.s1[w] = s1;
+    red.s2[w] = s2;
+    red.m[w] = bm;
+    red.k[w] = bk;
+  }
+  __syncthreads();
+  s1 = 0;
+  s2 = 0;
+  bm = -1.0f;
+  bk = 0xffffffffu;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    s1 += red.s1[i];
+    s2 += red.s2[i];
+    if (red.m[i] > bm || (red.m[i] == bm && red.k[i] < bk)) {
+      bm = red.m[i];
+      bk = red.k[i];
+    }
+  }
+  __syncthreads();
+}
+
+// Block exclusive scan of per-thread counts; returns the offset, total in `total`.
+template <int T>
+__device__ __forceinline__ int block_excl_scan(int c, int& total, BlockRed& red) {
+  constexpr int NW = (T + 63) / 64;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int inc = wave_incl_scan(c);
+  if (lane == 63) red.c[w] = inc;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    if (i < w) off += red.c[i];
+    tot += red.c[i];
+  }
+  __syncthreads();
+  total = tot;
+  return off + inc - c;
+}
+
+// ===========================================================================
+// small path
+// ===========================================================================
+template <int L>
+struct SmallCfg {
+  static constexpr int T = (L / 16 > 64) ? L / 16 : 64;
+  static constexpr int PPT = (L >= 16) ? 16 : L;
+  static constexpr int E = (L >= T) ? L / T : 1;   // bins per owner thread in the peak stage
+  static constexpr int OWN = L / E;                 // owner threads
+  static constexpr int WPE = 4;
+  using LAY = LayContig<L, 1>;
+};
+
+template <int L, int MODE>
+__global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_per_eu(SmallCfg<L>::WPE))) k_fused_small(FusedArgs a) {
+  using C = SmallCfg<L>;
+  using PL = Plan<L>;
+  using LAY = typename C::LAY;
+  constexpr int T = C::T, PPT = C::PPT;
+  constexpr int R0 = PL::R0, RL = PL::RL;
+  __shared__ float2 lds[LAY::SIZE];
+  __shared__ BlockRed red;
+  const int tid = threadIdx.x;
+
+  // one channel per workgroup: a grid-stride loop here would let the compiler
+  // hoist every (channel-invariant) twiddle and window load out of the loop
+  // and keep them all live in VGPRs.
+  {
+    const uint32_t ch = blockIdx.x;
+    if (ch >= a.batch) return;
+    float2 v[PPT];
+    // ---- load in first-pass butterfly order: x[jj + r*L/R0]
+#pragma unroll
+    for (int b = 0; b < PPT / R0; ++b) {
+      const int j = tid + b * T;
+      if (j < L / R0) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[b * R0 + r] = load_in(a, ch, (uint32_t)(j + r * (L / R0)));
+      }
+    }
+    fft_tile<LAY, L, 1, T, PPT>(lds, v, a.tw_a, tid);
+    // v[b*RL + r] = X[j + r*L/RL], j = tid + b*T < L/RL
+
+    const uint64_t cbase = (uint64_t)ch * L;
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int b = 0; b < PPT / RL; ++b) {
+        const int j = tid + b * T;
+        if (j < L / RL) {
+#pragma unroll
+          for (int r = 0; r < RL; ++r) store_dense(a, cbase + j + r * (L / RL), v[b * RL + r]);
+        }
+      }
+    }
+
+    if constexpr (MODE == 1) {
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < PPT / RL; ++b) {
+        const int j = tid + b * T;
+        if (j < L / RL) {
+#pragma unroll
+          for (int r = 0; r < RL; ++r) lds[LAY::idx(0, j + r * (L / RL))] = v[b * RL + r];
+        }
+      }
+      __syncthreads();
+
+      constexpr int E = C::E;
+      float m[E];
+      double s1 = 0.0, s2 = 0.0;
+      float fs1 = 0.f, fs2 = 0.f;
+      float bm = -1.0f;
+      uint32_t bk = 0xffffffffu;
+      const bool owner = tid < C::OWN;
+      if (owner) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const uint32_t k = tid * E + e;
+          m[e] = mag_of(lds[LAY::idx(0, k)]);
+          if (in_range(k, a)) {
+            fs1 += m[e];
+            fs2 = fmaf(m[e], m[e], fs2);
+            if (m[e] > bm) { bm = m[e]; bk = k; }
+          }
+        }
+      }
+      s1 = fs1;
+      s2 = fs2;
+      block_stats<T>(s1, s2, bm, bk, red);
+      const double cnt = (double)(a.s1 - a.s0);
+      float mu, sd;
+      const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
+
+      // neighbours of the owned run (circular)
+      float ml = 0.f, mr = 0.f;
+      uint32_t flags = 0;
+      int c = 0;
+      if (owner) {
+        ml = mag_of(lds[LAY::idx(0, (tid * E + L - 1) % L)]);
+        mr = mag_of(lds[LAY::idx(0, (tid * E + E) % L)]);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const uint32_t k = tid * E + e;
+          const float yl = (e == 0) ? ml : m[e - 1];
+          const float yr = (e == E - 1) ? mr : m[e + 1];
+          if (in_range(k, a) && m[e] > yl && m[e] > yr && m[e] > thr) {
+            flags |= 1u << e;
+            ++c;
+          }
+        }
+      }
+      int total;
+      int off = block_excl_scan<T>(c, total, red);
+      if (owner && flags && off < (int)a.kcap) {
+        MaxValue* out = a.peaks + (uint64_t)ch * a.kcap;
+        int pos = off;
+        while (flags && pos < (int)a.kcap) {
+          const int e = __builtin_ctz(flags);
+          flags &= flags - 1;
+          const uint32_t k = tid * E + e;
+          const float yc = mag_of(lds[LAY::idx(0, k)]);
+          const float yl = mag_of(lds[LAY::idx(0, (k + L - 1) % L)]);
+          const float yr = mag_of(lds[LAY::idx(0, (k + 1) % L)]);
+          write_record(out + pos, k, lds[LAY::idx(0, k)], yl, yc, yr, a);
+          ++pos;
+        }
+      }
+      if (tid == 0) {
+        if (a.counts) {
+          a.counts[2 * ch + 0] = (uint32_t)min(total, (int)a.kcap);
+          a.counts[2 * ch + 1] = (uint32_t)total;
+        }
+        if (a.stats) {
+          double pe = cnt > 0 ? s2 / cnt : 0.0;
+          a.stats[4 * ch + 0] = mu;
+          a.stats[4 * ch + 1] = sd;
+          a.stats[4 * ch + 2] = thr;
+          a.stats[4 * ch + 3] = (float)pe;
+        }
+        if (a.gmax && bk != 0xffffffffu) {
+          const float yl = mag_of(lds[LAY::idx(0, (bk + L - 1) % L)]);
+          const float yr = mag_of(lds[LAY::idx(0, (bk + 1) % L)]);
+          write_record(a.gmax + ch, bk, lds[LAY::idx(0, bk)], yl, bm, yr, a);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ===========================================================================
+// large path: N = N1 * 256
+// ===========================================================================
+struct Cand {
+  uint32_t k;
+  float re, im, ml, mc, mr;
+};
+
+__device__ __noinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t cap, uint32_t k, float2 x, float yl,
+                                      float yc, float yr) {
+  const uint32_t p = atomicAdd(ncand, 1u);
+  if (p < cap) {
+    Cand cd;
+    cd.k = k; cd.re = x.x; cd.im = x.y; cd.ml = yl; cd.mc = yc; cd.mr = yr;
+    cand[p] = cd;
+  }
+}
+
+template <int N1>
+struct LargeCfg {
+  static constexpr int N2 = 256;
+  static constexpr int N = N1 * N2;
+  static constexpr int T = 1024;
+  static constexpr int PPT = 16;
+  static constexpr int TILE = T * PPT;          // 16384 points
+  static constexpr int CB = TILE / N2;          // 64 columns per phase-1 tile
+  static constexpr int RB = TILE / N1;          // rows per phase-2 tile
+  static constexpr int NT1 = N1 / CB;           // phase-1 tiles
+  static constexpr int NT2 = N2 / RB;           // phase-2 tiles
+  using LAY1 = LayInter<N2, CB>;
+  using LAY2 = LayContig<N1, RB>;
+  static constexpr int LDS = LAY2::SIZE > LAY1::SIZE ? LAY2::SIZE : LAY1::SIZE;
+  // scratch slot layout (bytes)
+  static constexpr uint64_t Z_BYTES = (uint64_t)N * 8;
+  static constexpr uint64_t CAND_CAP = N / 2 + 2;
+  static constexpr uint64_t CAND_BYTES = CAND_CAP * sizeof(Cand);
+  static constexpr uint64_t EDGE_M_BYTES = (uint64_t)NT2 * 4 * N1 * 4;
+  static constexpr uint64_t EDGE_X_BYTES = (uint64_t)NT2 * 2 * N1 * 8;
+  static constexpr uint64_t SLOT_BYTES = Z_BYTES + CAND_BYTES + EDGE_M_BYTES + EDGE_X_BYTES;
+  static constexpr int NBK = 1024;              // selection buckets
+  static constexpr int BSH = ilog2(N) - 10;     // bucket = k >> BSH
+  static_assert(RB >= 4, "phase-2 tile needs at least four rows");
+  static_assert(NT1 >= 1 && NT2 >= 1, "tile shape");
+};
+
+template <int N1>
+struct LargeShared {
+  using C = LargeCfg<N1>;
+  float2 tile[C::LDS];
+  BlockRed red;
+  int hist[C::NBK];
+  uint32_t ncand;
+  uint32_t ngather;
+  int bstar;
+  float thr;
+};
+
+// MODE 0: dense outputs only (spectrum / |X| / |X|^2 / mag+phase); MODE 1: peaks only.
+template <int N1, int MODE>
+__global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
+  using C = LargeCfg<N1>;
+  constexpr int N = C::N, N2 = C::N2, T = C::T, PPT = C::PPT, CB = C::CB, RB = C::RB;
+  constexpr int NT1 = C::NT1, NT2 = C::NT2;
+  using LAY1 = typename C::LAY1;
+  using LAY2 = typename C::LAY2;
+  constexpr int R0 = Plan<N1>::R0;
+  static_assert(Plan<N1>::RL == 16 && Plan<N2>::RL == 16, "last passes are radix 16");
+
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  LargeShared<N1>& sh = *reinterpret_cast<LargeShared<N1>*>(smem_raw);
+  float2* lds = sh.tile;
+  float* ldsf = reinterpret_cast<float*>(sh.tile);
+  const int tid0 = threadIdx.x;
+
+  unsigned char* slot = reinterpret_cast<unsigned char*>(a.scratch) + (uint64_t)blockIdx.x * a.slot_bytes;
+  float2* Z = reinterpret_cast<float2*>(slot);
+  Cand* cand = reinterpret_cast<Cand*>(slot + C::Z_BYTES);
+  float* edgeM = reinterpret_cast<float*>(slot + C::Z_BYTES + C::CAND_BYTES);
+  float2* edgeX = reinterpret_cast<float2*>(slot + C::Z_BYTES + C::CAND_BYTES + C::EDGE_M_BYTES);
+
+  constexpr bool peaks = (MODE == 1);
+  const bool full_band = (a.s0 == 0 && a.s1 == (uint32_t)N);
+
+  for (uint32_t ch = blockIdx.x; ch < a.batch; ch += gridDim.x) {
+    int tid = opaque_v(tid0);
+    FusedArgs ai = a;
+    ai.tw_a = opaque(a.tw_a);
+    ai.tw_b = opaque(a.tw_b);
+    ai.tw_4 = opaque(a.tw_4);
+    ai.window = opaque(a.window);
+    // ------------------------------------------------------------------
+    // phase 1: columns n1, 256-point DFT over n2, twiddle W_N^(n1*k2),
+    //          store Z[k2][n1]
+    // ------------------------------------------------------------------
+    float energy = 0.f;
+    {
+#pragma unroll 1
+      for (int t = 0; t < NT1; ++t) {
+        tid = opaque_v(tid0);
+        const int c = tid % CB, jj = tid / CB;  // radix-16 first pass: jj in [0,16)
+        ai.tw_b = opaque(a.tw_b);
+        ai.tw_4 = opaque(a.tw_4);
+        ai.window = opaque(a.window);
+        const int n1 = t * CB + c;
+        float2 v[PPT];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          v[r] = load_in(ai, ch, (uint32_t)(n1 + N1 * (jj + 16 * r)));
+          energy = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, energy));
+        }
+        fft_tile<LAY1, N2, CB, T, PPT>(lds, v, ai.tw_b, tid);
+        // v[r] = Y[n1][k2 = jj + 16 r]
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int k2 = jj + 16 * r;
+          const float2 w = ai.tw_4[k2 * N1 + n1];
+          Z[k2 * N1 + n1] = c_mul(v[r], w);
+        }
+      }
+    }
+    __syncthreads();
+
+    // block reduce the input energy -> P = mean |X|^2 (Parseval)
+    double P;
+    {
+      double e = energy, dummy = 0.0;
+      float bm = -1.f;
+      uint32_t bk = 0;
+      block_stats<T>(e, dummy, bm, bk, sh.red);
+      P = e;  // sum_n |x_w[n]|^2 = (1/N) sum_k |X[k]|^2
+    }
+    float lb = 0.f;
+    if (a.peak_mode == 1 && full_band && a.nsigma >= 0.f) {
+      lb = (float)(fmin(1.0, (double)a.nsigma) * sqrt(P) * 0.999);
+    }
+    if (tid == 0) sh.ncand = 0;
+
+    // ------------------------------------------------------------------
+    // phase 2: rows k2, N1-point DFT over n1 -> X[k2 + 256*k1]
+    // ------------------------------------------------------------------
+    float fs1 = 0.f, fs2 = 0.f;
+    float bm = -1.f;
+    uint32_t bk = 0xffffffffu;
+    float2 bx = make_float2(0.f, 0.f);
+    float bl = 0.f, br = 0.f;
+
+    auto decide = [&](uint32_t k, float2 x, float yl, float yc, float yr) {
+      if (!in_range(k, a)) return;
+      if (yc > bm || (yc == bm && k < bk)) {
+        bm = yc; bk = k; bx = x; bl = yl; br = yr;
+      }
+      if (yc > yl && yc > yr && yc >= lb) push_cand(cand, &sh.ncand, (uint32_t)C::CAND_CAP, k, x, yl, yc, yr);
+    };
+
+#pragma unroll 1
+    for (int t = 0; t < NT2; ++t) {
+      tid = opaque_v(tid0);
+      const int s_row = tid / (N1 / 16), jj_row = tid % (N1 / 16);  // last-pass mapping
+      ai.tw_a = opaque(a.tw_a);
+      float2 v[PPT];
+#pragma unroll
+      for (int b = 0; b < PPT / R0; ++b) {
+        const int j = tid + b * T;
+        int s, jj;
+        LAY2::template map<R0>(j, s, jj);
+        const float2* zr = Z + (uint64_t)(t * RB + s) * N1;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[b * R0 + r] = zr[jj + r * (N1 / R0)];
+      }
+      fft_tile<LAY2, N1, RB, T, PPT>(lds, v, ai.tw_a, tid);
+      // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + r*N1/16]
+      const int k2 = t * RB + s_row;
+      float m[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        m[r] = mag_of(v[r]);
+        const uint32_t k = (uint32_t)k2 + 256u * (uint32_t)(jj_row + r * (N1 / 16));
+        if (in_range(k, a)) {
+          fs1 += m[r];
+          fs2 = fmaf(m[r], m[r], fs2);
+        }
+      }
+
+      constexpr bool want_out = (MODE == 0);
+      __syncthreads();  // LDS free again
+      if constexpr (want_out) {
+        // transpose through LDS so that natural-order runs of RB bins are written
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int k1 = jj_row + r * (N1 / 16);
+          const int e = k1 * RB + s_row;
+          lds[e + (e >> 4)] = v[r];
+        }
+        __syncthreads();
+        const uint64_t cbase = (uint64_t)ch * N;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int e = tid + i * T;
+          const int k1 = e / RB, s = e % RB;
+          const uint64_t k = (uint64_t)(t * RB + s) + 256ull * k1;
+          store_dense(a, cbase + k, lds[e + (e >> 4)]);
+        }
+      }
+      if constexpr (peaks) {
+        // magnitude tile [s][k1] and edge rows
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int k1 = jj_row + r * (N1 / 16);
+          ldsf[s_row * N1 + k1] = m[r];
+        }
+        {
+          int slotm = -1, slotx = -1;
+          if (s_row == 0) { slotm = 0; slotx = 0; }
+          else if (s_row == 1) slotm = 1;
+          if (s_row == RB - 2) slotm = 2;
+          if (s_row == RB - 1) { slotm = 3; slotx = 1; }
+          if (slotm >= 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              const int k1 = jj_row + r * (N1 / 16);
+              edgeM[(t * 4 + slotm) * N1 + k1] = m[r];
+              if (slotx >= 0) edgeX[(t * 2 + slotx) * N1 + k1] = v[r];
+            }
+          }
+        }
+        __syncthreads();
+        if (s_row >= 1 && s_row <= RB - 2) {
+          // cheap screen in registers, then the rare interesting bins one by one
+          uint32_t mask = 0;
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int k1 = jj_row + r * (N1 / 16);
+            const float yl = ldsf[(s_row - 1) * N1 + k1];
+            const float yr = ldsf[(s_row + 1) * N1 + k1];
+            const bool lmax = m[r] > yl && m[r] > yr && m[r] >= lb;
+            if (lmax || m[r] >= bm) mask |= 1u << r;
+          }
+          while (mask) {
+            const int r = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const int k1 = jj_row + r * (N1 / 16);
+            float2 x = v[0];
+#pragma unroll
+            for (int q = 1; q < 16; ++q) if (q == r) x = v[q];
+            decide((uint32_t)k2 + 256u * (uint32_t)k1, x, ldsf[(s_row - 1) * N1 + k1], mag_of(x),
+                   ldsf[(s_row + 1) * N1 + k1]);
+          }
+        }
+        __syncthreads();
+      }
+    }
+
+    if constexpr (peaks) {
+      tid = opaque_v(tid0);
+      // phase 2b: tile-boundary rows from the edge buffers
+      constexpr int NB = 2 * NT2 * N1;
+      for (int i = opaque_v(tid0); i < NB; i += T) {
+        const int k1 = i % N1;
+        const int q = i / N1;          // 0 .. 2*NT2-1
+        const int t = q >> 1;
+        const bool last = (q & 1) != 0;  // row RB-1 of tile t, else row 0
+        float yl, yc, yr;
+        float2 x;
+        uint32_t k2;
+        if (!last) {
+          k2 = (uint32_t)(t * RB);
+          yc = edgeM[(t * 4 + 0) * N1 + k1];
+          x = edgeX[(t * 2 + 0) * N1 + k1];
+          yr = edgeM[(t * 4 + 1) * N1 + k1];
+          if (t > 0) yl = edgeM[((t - 1) * 4 + 3) * N1 + k1];
+          else yl = edgeM[((NT2 - 1) * 4 + 3) * N1 + (k1 + N1 - 1) % N1];  // k-1 wraps to row 255, column k1-1
+        } else {
+          k2 = (uint32_t)(t * RB + RB - 1);
+          yc = edgeM[(t * 4 + 3) * N1 + k1];
+          x = edgeX[(t * 2 + 1) * N1 + k1];
+          yl = edgeM[(t * 4 + 2) * N1 + k1];
+          if (t < NT2 - 1) yr = edgeM[((t + 1) * 4 + 0) * N1 + k1];
+          else yr = edgeM[(0 * 4 + 0) * N1 + (k1 + 1) % N1];  // k+1 wraps to row 0, column k1+1
+        }
+        decide(k2 + 256u * (uint32_t)k1, x, yl, yc, yr);
+      }
+      __syncthreads();
+
+      // ------------------------------------------------------------------
+      // phase 3: statistics, threshold, selection of the first K survivors
+      // ------------------------------------------------------------------
+      double s1 = fs1, s2 = fs2;
+      float gm = bm;
+      uint32_t gk = bk;
+      block_stats<T>(s1, s2, gm, gk, sh.red);
+      const double cnt = (double)(a.s1 - a.s0);
+      float mu, sd;
+      const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
+      if (a.gmax && gk != 0xffffffffu && bk == gk && bm == gm) write_record(a.gmax + ch, bk, bx, bl, bm, br, a);
+
+      const uint32_t ncand = min(sh.ncand, (uint32_t)C::CAND_CAP);
+      for (int i = opaque_v(tid0); i < C::NBK; i += T) sh.hist[i] = 0;
+      __syncthreads();
+      for (uint32_t i = opaque_v(tid0); i < ncand; i += T) {
+        const Cand cd = cand[i];
+        if (cd.mc > thr) atomicAdd(&sh.hist[cd.k >> C::BSH], 1);
+      }
+      __syncthreads();
+      // inclusive scan over buckets (one bucket per thread)
+      int hv = sh.hist[tid];
+      int total;
+      int excl = block_excl_scan<T>(hv, total, sh.red);
+      const int kcap = (int)a.kcap;
+      if (tid == 0) { sh.bstar = C::NBK - 1; sh.ngather = 0; }
+      __syncthreads();
+      if (total > kcap && excl < kcap && excl + hv >= kcap) sh.bstar = tid;
+      __syncthreads();
+      const int bstar = sh.bstar;
+      // gather (key = bin, value = candidate index) of survivors in buckets <= bstar
+      uint2* gat = reinterpret_cast<uint2*>(lds);
+      constexpr int GMAX = C::LDS / 2 > 4096 ? 4096 : C::LDS / 2;
+      for (uint32_t i = opaque_v(tid0); i < ncand; i += T) {
+        const Cand cd = cand[i];
+        if (cd.mc > thr && (int)(cd.k >> C::BSH) <= bstar) {
+          const uint32_t p = atomicAdd(&sh.ngather, 1u);
+          if (p < (uint32_t)GMAX) gat[p] = make_uint2(cd.k, i);
+        }
+      }
+      __syncthreads();
+      const int ng = min((int)sh.ngather, GMAX);
+      int np2 = 1;
+      while (np2 < ng) np2 <<= 1;
+      for (int i = ng + opaque_v(tid0); i < np2; i += T) gat[i] = make_uint2(0xffffffffu, 0u);
+      __syncthreads();
+      // bitonic sort by key
+      for (int size = 2; size <= np2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+          for (int i = opaque_v(tid0); i < np2 / 2; i += T) {
+            const int lo = 2 * i - (i & (stride - 1));
+            const int hi = lo + stride;
+            const bool up = ((lo & size) == 0);
+            uint2 x = gat[lo], y = gat[hi];
+            if ((x.x > y.x) == up) { gat[lo] = y; gat[hi] = x; }
+          }
+          __syncthreads();
+        }
+      }
+      const int nout = min(total, kcap);
+      MaxValue* out = a.peaks + (uint64_t)ch * a.kcap;
+      for (int i = opaque_v(tid0); i < nout && i < ng; i += T) {
+        const Cand cd = cand[gat[i].y];
+        write_record(out + i, cd.k, make_float2(cd.re, cd.im), cd.ml, cd.mc, cd.mr, a);
+      }
+      if (tid == 0) {
+        if (a.counts) {
+          a.counts[2 * ch + 0] = (uint32_t)nout;
+          a.counts[2 * ch + 1] = (uint32_t)total;
+        }
+        if (a.stats) {
+          a.stats[4 * ch + 0] = mu;
+          a.stats[4 * ch + 1] = sd;
+          a.stats[4 * ch + 2] = thr;
+          a.stats[4 * ch + 3] = (float)P;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ===========================================================================
+// dispatch
+// ===========================================================================
+template <int L>
+static hipError_t launch_small(const FusedArgs& a, int grid, hipStream_t st) {
+  if (a.peak_mode != 0)
+    hipLaunchKernelGGL((k_fused_small<L, 1>), dim3(grid), dim3(SmallCfg<L>::T), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_fused_small<L, 0>), dim3(grid), dim3(SmallCfg<L>::T), 0, st, a);
+  return hipGetLastError();
+}
+
+template <int N1, int MODE>
+static hipError_t launch_large_mode(const FusedArgs& a, int grid, hipStream_t st) {
+  const size_t smem = sizeof(LargeShared<N1>);
+  static bool attr_done = false;  // the attribute is per kernel function
+  if (!attr_done) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_fused_large<N1, MODE>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    attr_done = true;
+  }
+  hipLaunchKernelGGL((k_fused_large<N1, MODE>), dim3(grid), dim3(1024), smem, st, a);
+  return hipGetLastError();
+}
+
+template <int N1>
+static hipError_t launch_large(const FusedArgs& a, int grid, hipStream_t st) {
+  return a.peak_mode != 0 ? launch_large_mode<N1, 1>(a, grid, st) : launch_large_mode<N1, 0>(a, grid, st);
+}
+
+}  // namespace spx
+
+extern "C" {
+
+uint64_t spx_large_slot_bytes(uint32_t nfft) {
+  using namespace spx;
+  switch (nfft) {
+    case 1u << 15: return LargeCfg<128>::SLOT_BYTES;
+    case 1u << 16: return LargeCfg<256>::SLOT_BYTES;
+    case 1u << 17: return LargeCfg<512>::SLOT_BYTES;
+    case 1u << 18: return LargeCfg<1024>::SLOT_BYTES;
+    default: return 0;
+  }
+}
+
+int spx_fused_is_large(uint32_t nfft) { return nfft >= (1u << 15) ? 1 : 0; }
+
+hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st) {
+  using namespace spx;
+  switch (a->nfft) {
+    case 1u << 1: return launch_small<2>(*a, grid, st);
+    case 1u << 2: return launch_small<4>(*a, grid, st);
+    case 1u << 3: return launch_small<8>(*a, grid, st);
+    case 1u << 4: return launch_small<16>(*a, grid, st);
+    case 1u << 5: return launch_small<32>(*a, grid, st);
+    case 1u << 6: return launch_small<64>(*a, grid, st);
+    case 1u << 7: return launch_small<128>(*a, grid, st);
+    case 1u << 8: return launch_small<256>(*a, grid, st);
+    case 1u << 9: return launch_small<512>(*a, grid, st);
+    case 1u << 10: return launch_small<1024>(*a, grid, st);
+    case 1u << 11: return launch_small<2048>(*a, grid, st);
+    case 1u << 12: return launch_small<4096>(*a, grid, st);
+    case 1u << 13: return launch_small<8192>(*a, grid, st);
+    case 1u << 14: return launch_small<16384>(*a, grid, st);
+    case 1u << 15: return launch_large<128>(*a, grid, st);
+    case 1u << 16: return launch_large<256>(*a, grid, st);
+    case 1u << 17: return launch_large<512>(*a, grid, st);
+    case 1u << 18: return launch_large<1024>(*a, grid, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// Internal interface between the C-ABI host code and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+#include "spx_types.h"
+
+#define SPX_MAG_ABS 1
+#define SPX_MAG_SQUARED 2
+#define SPX_MAG_PHASE 3
+
+// Arguments of the fused window -> FFT -> |X| -> peaks kernels.
+struct FusedArgs {
+  const void* in;          // [batch][n_point] complex (dtype)
+  const float* in_scale;   // optional per-channel scale applied on load (FP16/BF16 pre-scaling)
+  uint32_t batch, n_point, nfft, dtype;
+  const float* window;     // optional [n_point] real window
+  float fs;
+  // optional dense outputs
+  float2* spec_out;        // [batch][nfft] complex64
+  float* mag_out;          // [batch][nfft] float (ABS/SQUARED) or float2 (PHASE: {|X|, arg X})
+  uint32_t mag_kind;
+  float mag_scale;
+  // peak search: 0 none, 1 threshold (mean + nsigma*std), 2 local maxima only
+  uint32_t peak_mode;
+  float nsigma;
+  uint32_t kcap;
+  uint32_t s0, s1;
+  uint32_t refine;         // 0 none, 1 parabolic, 2 learned (MLP) with parabolic fallback
+  spx_max_value* peaks;    // [batch][kcap]
+  uint32_t* counts;        // [batch][2] {stored, found}
+  spx_max_value* gmax;     // [batch]
+  float* stats;            // [batch][4] {mean, std, threshold, mean |X|^2}
+  // tables
+  const float2* tw_a;      // small: W_N^i ; large: W_N1^i
+  const float2* tw_b;      // large: W_256^i
+  const float2* tw_4;      // large: W_N^(n1*k2) at [k2*N1 + n1]
+  void* scratch;           // large: per-workgroup slots
+  uint64_t slot_bytes;
+  const void* mlp;         // learned refiner weights (refine == 2)
+};
+
+extern "C" {
+hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st);
+uint64_t spx_large_slot_bytes(uint32_t nfft);
+int spx_fused_is_large(uint32_t nfft);
+hipError_t spx_launch_mag_phase(const void* in, void* out, uint64_t total, hipStream_t st);
+hipError_t spx_launch_magnitude(const void* in, float* out, uint64_t total, float inv_n, int squared,
+                                hipStream_t st);
+hipError_t spx_launch_peaks_top(const void* spec, void* out, uint32_t batch, uint32_t nfft, uint32_t search_range,
+                                float fs, int two_peaks, hipStream_t st);
+hipError_t spx_launch_all_maxima(const void* spec, void* out, uint32_t* counts, uint32_t batch, uint32_t nfft,
+                                 float fs, uint32_t s0, uint32_t s1, uint32_t cap, hipStream_t st);
+}
