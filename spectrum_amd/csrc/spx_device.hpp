@@ -25,6 +25,21 @@ __device__ __forceinline__ float2 c_mul(float2 a, float2 w) {
 __device__ __forceinline__ float2 c_mul_mj(float2 a) { return make_float2(a.y, -a.x); }  // * (-j)
 __device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
+// Pointers that travel inside a by-value argument struct are generic to the
+// compiler; dereferencing them as such emits flat_* instructions (both
+// waitcounts, 64-bit addresses, serialised waits).  gp() asserts the global
+// address space so loads and stores become global_* with scalar bases.
+// (The host compilation pass parses device code too but has no address
+// spaces; there the pointer is returned unchanged.)
+template <class T>
+__device__ __forceinline__ auto gp(T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (__attribute__((address_space(1))) T*)p;
+#else
+  return p;
+#endif
+}
+
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 constexpr int ipow16(int p) { return p <= 0 ? 1 : 16 * ipow16(p - 1); }
 
@@ -213,7 +228,7 @@ __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[P
         const int k = jj % NS;
         constexpr int STEP = L / (NS * R);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], tw[r * k * STEP]);
+        for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], gp(tw)[r * k * STEP]);
       }
       Dft<R>::run(&v[b * R]);
     }

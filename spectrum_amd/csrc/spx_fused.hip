@@ -49,16 +49,16 @@ __device__ __forceinline__ float2 load_in(const FusedArgs& a, uint64_t ch, uint3
   const uint64_t i = ch * (uint64_t)a.n_point + n;
   float2 x;
   if (a.dtype == SPX_DTYPE_C64) {
-    x = reinterpret_cast<const float2*>(a.in)[i];
+    x = gp(reinterpret_cast<const float2*>(a.in))[i];
   } else if (a.dtype == SPX_DTYPE_C32F16) {
-    const __half2 h = reinterpret_cast<const __half2*>(a.in)[i];
+    const __half2 h = gp(reinterpret_cast<const __half2*>(a.in))[i];
     x = __half22float2(h);
   } else {
-    const uint32_t u = reinterpret_cast<const uint32_t*>(a.in)[i];
+    const uint32_t u = gp(reinterpret_cast<const uint32_t*>(a.in))[i];
     x = make_float2(bf16_to_f((uint16_t)(u & 0xffffu)), bf16_to_f((uint16_t)(u >> 16)));
   }
-  if (a.in_scale != nullptr) x = c_scale(x, a.in_scale[ch]);
-  if (a.window != nullptr) x = c_scale(x, a.window[n]);
+  if (a.in_scale != nullptr) x = c_scale(x, gp(a.in_scale)[ch]);
+  if (a.window != nullptr) x = c_scale(x, gp(a.window)[n]);
   return x;
 }
 
@@ -100,18 +100,18 @@ __device__ __noinline__ void write_record(MaxValue* dst, uint32_t k, float2 x, f
   mv.freq_offset = d;
   mv.refined_frequency = ((float)k + d) * (a.fs / (float)a.nfft);
   mv.pad = 0;
-  *dst = mv;
+  *gp(dst) = mv;
 }
 
 __device__ __forceinline__ bool in_range(uint32_t k, const FusedArgs& a) { return k >= a.s0 && k < a.s1; }
 
 // dense spectrum / magnitude outputs (phase needs atan2f: kept out of line)
-__device__ __noinline__ void store_magphase(float2* dst, float2 x) { *dst = make_float2(mag_of(x), atan2f(x.y, x.x)); }
+__device__ __noinline__ void store_magphase(float2* dst, float2 x) { *gp(dst) = make_float2(mag_of(x), atan2f(x.y, x.x)); }
 __device__ __forceinline__ void store_dense(const FusedArgs& a, uint64_t i, float2 x) {
-  if (a.spec_out) a.spec_out[i] = x;
+  if (a.spec_out) gp(a.spec_out)[i] = x;
   if (a.mag_out) {
-    if (a.mag_kind == SPX_MAG_ABS) a.mag_out[i] = mag_of(x) * a.mag_scale;
-    else if (a.mag_kind == SPX_MAG_SQUARED) a.mag_out[i] = fmaf(x.x, x.x, x.y * x.y) * a.mag_scale;
+    if (a.mag_kind == SPX_MAG_ABS) gp(a.mag_out)[i] = mag_of(x) * a.mag_scale;
+    else if (a.mag_kind == SPX_MAG_SQUARED) gp(a.mag_out)[i] = fmaf(x.x, x.x, x.y * x.y) * a.mag_scale;
     else store_magphase(reinterpret_cast<float2*>(a.mag_out) + i, x);
   }
 }
@@ -309,15 +309,15 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
       }
       if (tid == 0) {
         if (a.counts) {
-          a.counts[2 * ch + 0] = (uint32_t)min(total, (int)a.kcap);
-          a.counts[2 * ch + 1] = (uint32_t)total;
+          gp(a.counts)[2 * ch + 0] = (uint32_t)min(total, (int)a.kcap);
+          gp(a.counts)[2 * ch + 1] = (uint32_t)total;
         }
         if (a.stats) {
           double pe = cnt > 0 ? s2 / cnt : 0.0;
-          a.stats[4 * ch + 0] = mu;
-          a.stats[4 * ch + 1] = sd;
-          a.stats[4 * ch + 2] = thr;
-          a.stats[4 * ch + 3] = (float)pe;
+          gp(a.stats)[4 * ch + 0] = mu;
+          gp(a.stats)[4 * ch + 1] = sd;
+          gp(a.stats)[4 * ch + 2] = thr;
+          gp(a.stats)[4 * ch + 3] = (float)pe;
         }
         if (a.gmax && bk != 0xffffffffu) {
           const float yl = mag_of(lds[LAY::idx(0, (bk + L - 1) % L)]);
@@ -344,7 +344,7 @@ __device__ __noinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t cap
   if (p < cap) {
     Cand cd;
     cd.k = k; cd.re = x.x; cd.im = x.y; cd.ml = yl; cd.mc = yc; cd.mr = yr;
-    cand[p] = cd;
+    gp(cand)[p] = cd;
   }
 }
 
@@ -445,8 +445,8 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int k2 = jj + 16 * r;
-          const float2 w = ai.tw_4[k2 * N1 + n1];
-          Z[k2 * N1 + n1] = c_mul(v[r], w);
+          const float2 w = gp(ai.tw_4)[k2 * N1 + n1];
+          gp(Z)[k2 * N1 + n1] = c_mul(v[r], w);
         }
       }
     }
@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
         LAY2::template map<R0>(j, s, jj);
         const float2* zr = Z + (uint64_t)(t * RB + s) * N1;
 #pragma unroll
-        for (int r = 0; r < R0; ++r) v[b * R0 + r] = zr[jj + r * (N1 / R0)];
+        for (int r = 0; r < R0; ++r) v[b * R0 + r] = gp(zr)[jj + r * (N1 / R0)];
       }
       fft_tile<LAY2, N1, RB, T, PPT>(lds, v, ai.tw_a, tid);
       // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + r*N1/16]
@@ -550,8 +550,8 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
               const int k1 = jj_row + r * (N1 / 16);
-              edgeM[(t * 4 + slotm) * N1 + k1] = m[r];
-              if (slotx >= 0) edgeX[(t * 2 + slotx) * N1 + k1] = v[r];
+              gp(edgeM)[(t * 4 + slotm) * N1 + k1] = m[r];
+              if (slotx >= 0) gp(edgeX)[(t * 2 + slotx) * N1 + k1] = v[r];
             }
           }
         }
@@ -596,18 +596,18 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
         uint32_t k2;
         if (!last) {
           k2 = (uint32_t)(t * RB);
-          yc = edgeM[(t * 4 + 0) * N1 + k1];
-          x = edgeX[(t * 2 + 0) * N1 + k1];
-          yr = edgeM[(t * 4 + 1) * N1 + k1];
-          if (t > 0) yl = edgeM[((t - 1) * 4 + 3) * N1 + k1];
-          else yl = edgeM[((NT2 - 1) * 4 + 3) * N1 + (k1 + N1 - 1) % N1];  // k-1 wraps to row 255, column k1-1
+          yc = gp(edgeM)[(t * 4 + 0) * N1 + k1];
+          x = gp(edgeX)[(t * 2 + 0) * N1 + k1];
+          yr = gp(edgeM)[(t * 4 + 1) * N1 + k1];
+          if (t > 0) yl = gp(edgeM)[((t - 1) * 4 + 3) * N1 + k1];
+          else yl = gp(edgeM)[((NT2 - 1) * 4 + 3) * N1 + (k1 + N1 - 1) % N1];  // k-1 wraps to row 255, column k1-1
         } else {
           k2 = (uint32_t)(t * RB + RB - 1);
-          yc = edgeM[(t * 4 + 3) * N1 + k1];
-          x = edgeX[(t * 2 + 1) * N1 + k1];
-          yl = edgeM[(t * 4 + 2) * N1 + k1];
-          if (t < NT2 - 1) yr = edgeM[((t + 1) * 4 + 0) * N1 + k1];
-          else yr = edgeM[(0 * 4 + 0) * N1 + (k1 + 1) % N1];  // k+1 wraps to row 0, column k1+1
+          yc = gp(edgeM)[(t * 4 + 3) * N1 + k1];
+          x = gp(edgeX)[(t * 2 + 1) * N1 + k1];
+          yl = gp(edgeM)[(t * 4 + 2) * N1 + k1];
+          if (t < NT2 - 1) yr = gp(edgeM)[((t + 1) * 4 + 0) * N1 + k1];
+          else yr = gp(edgeM)[(0 * 4 + 0) * N1 + (k1 + 1) % N1];  // k+1 wraps to row 0, column k1+1
         }
         decide(k2 + 256u * (uint32_t)k1, x, yl, yc, yr);
       }
@@ -629,7 +629,7 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
       for (int i = opaque_v(tid0); i < C::NBK; i += T) sh.hist[i] = 0;
       __syncthreads();
       for (uint32_t i = opaque_v(tid0); i < ncand; i += T) {
-        const Cand cd = cand[i];
+        const Cand cd = gp(cand)[i];
         if (cd.mc > thr) atomicAdd(&sh.hist[cd.k >> C::BSH], 1);
       }
       __syncthreads();
@@ -647,7 +647,7 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
       uint2* gat = reinterpret_cast<uint2*>(lds);
       constexpr int GMAX = C::LDS / 2 > 4096 ? 4096 : C::LDS / 2;
       for (uint32_t i = opaque_v(tid0); i < ncand; i += T) {
-        const Cand cd = cand[i];
+        const Cand cd = gp(cand)[i];
         if (cd.mc > thr && (int)(cd.k >> C::BSH) <= bstar) {
           const uint32_t p = atomicAdd(&sh.ngather, 1u);
           if (p < (uint32_t)GMAX) gat[p] = make_uint2(cd.k, i);
@@ -675,19 +675,19 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
       const int nout = min(total, kcap);
       MaxValue* out = a.peaks + (uint64_t)ch * a.kcap;
       for (int i = opaque_v(tid0); i < nout && i < ng; i += T) {
-        const Cand cd = cand[gat[i].y];
+        const Cand cd = gp(cand)[gat[i].y];
         write_record(out + i, cd.k, make_float2(cd.re, cd.im), cd.ml, cd.mc, cd.mr, a);
       }
       if (tid == 0) {
         if (a.counts) {
-          a.counts[2 * ch + 0] = (uint32_t)nout;
-          a.counts[2 * ch + 1] = (uint32_t)total;
+          gp(a.counts)[2 * ch + 0] = (uint32_t)nout;
+          gp(a.counts)[2 * ch + 1] = (uint32_t)total;
         }
         if (a.stats) {
-          a.stats[4 * ch + 0] = mu;
-          a.stats[4 * ch + 1] = sd;
-          a.stats[4 * ch + 2] = thr;
-          a.stats[4 * ch + 3] = (float)P;
+          gp(a.stats)[4 * ch + 0] = mu;
+          gp(a.stats)[4 * ch + 1] = sd;
+          gp(a.stats)[4 * ch + 2] = thr;
+          gp(a.stats)[4 * ch + 3] = (float)P;
         }
       }
     }

@@ -115,11 +115,15 @@ def test_fused_cap_overflow_keeps_first_k(ctx, nfft):
 def test_fused_degenerate_inputs(ctx, nfft):
     print(f"[case] degenerate nfft={nfft}", flush=True)
     x = np.zeros((3, nfft), dtype=np.complex64)
-    x[1] = 1.0  # constant -> one bin
-    x[2, 0] = 1.0  # impulse -> flat spectrum
+    x[1] = 1.0  # constant -> all energy in bin 0
+    x[2, 0] = 1.0  # impulse -> exactly flat spectrum
     pk, ct, gm, st = run_fused(ctx, x, nfft, K=8, window=0)
-    assert np.all(ct[:, 1] == 0)
+    # zeros: nothing; flat: no strict maxima
+    assert ct[0, 1] == 0 and ct[2, 1] == 0
     assert st[0, 0] == 0 and st[2, 1] < 1e-5
+    # constant: bin 0 (neighbours N-1 and 1 are circular) clears mean + 6 std = 1 + 6 sqrt(N-1) < N
+    assert ct[1, 1] == 1 and pk[1, 0]["index"] == 0 and abs(pk[1, 0]["magnitude"] - nfft) < 1e-3 * nfft
+    assert gm[1]["index"] == 0
 
 
 @pytest.mark.parametrize("nfft", [1024, 8192, 65536])
