@@ -44,22 +44,47 @@ using MaxValue = spx_max_value;
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-__device__ __forceinline__ float2 load_in(const FusedArgs& a, uint64_t ch, uint32_t n) {
-  if (n >= a.n_point) return make_float2(0.f, 0.f);
-  const uint64_t i = ch * (uint64_t)a.n_point + n;
-  float2 x;
-  if (a.dtype == SPX_DTYPE_C64) {
-    x = gp(reinterpret_cast<const float2*>(a.in))[i];
-  } else if (a.dtype == SPX_DTYPE_C32F16) {
-    const __half2 h = gp(reinterpret_cast<const __half2*>(a.in))[i];
-    x = __half22float2(h);
+template <int DT>
+__device__ __forceinline__ float2 ld_raw(const void* in, uint64_t i) {
+  if constexpr (DT == SPX_DTYPE_C64) {
+    return gp(reinterpret_cast<const float2*>(in))[i];
+  } else if constexpr (DT == SPX_DTYPE_C32F16) {
+    const __half2 h = gp(reinterpret_cast<const __half2*>(in))[i];
+    return __half22float2(h);
   } else {
-    const uint32_t u = gp(reinterpret_cast<const uint32_t*>(a.in))[i];
-    x = make_float2(bf16_to_f((uint16_t)(u & 0xffffu)), bf16_to_f((uint16_t)(u >> 16)));
+    const uint32_t u = gp(reinterpret_cast<const uint32_t*>(in))[i];
+    return make_float2(bf16_to_f((uint16_t)(u & 0xffffu)), bf16_to_f((uint16_t)(u >> 16)));
   }
-  if (a.in_scale != nullptr) x = c_scale(x, gp(a.in_scale)[ch]);
-  if (a.window != nullptr) x = c_scale(x, gp(a.window)[n]);
-  return x;
+}
+
+// v[r] = scale * w[n] * x[n] for n = n0 + r*stride, r < R; zero for n >= n_point.
+// Indices are clamped instead of branched on, so all R loads (and R window
+// loads) are in flight together before the first use.
+template <int DT, int R>
+__device__ __forceinline__ void load_run_t(const FusedArgs& a, uint64_t ch, uint32_t n0, uint32_t stride, float2* v) {
+  const uint32_t last = a.n_point - 1;
+  const uint64_t base = ch * (uint64_t)a.n_point;
+  float2 raw[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) raw[r] = ld_raw<DT>(a.in, base + min(n0 + r * stride, last));
+  float wv[R];
+  if (a.window != nullptr) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) wv[r] = gp(a.window)[min(n0 + r * stride, last)];
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) wv[r] = 1.0f;
+  }
+  const float sc = (a.in_scale != nullptr) ? gp(a.in_scale)[ch] : 1.0f;
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = c_scale(raw[r], (n0 + r * stride < a.n_point) ? wv[r] * sc : 0.0f);
+}
+
+template <int R>
+__device__ __forceinline__ void load_run(const FusedArgs& a, uint64_t ch, uint32_t n0, uint32_t stride, float2* v) {
+  if (a.dtype == SPX_DTYPE_C64) load_run_t<SPX_DTYPE_C64, R>(a, ch, n0, stride, v);
+  else if (a.dtype == SPX_DTYPE_C32F16) load_run_t<SPX_DTYPE_C32F16, R>(a, ch, n0, stride, v);
+  else load_run_t<SPX_DTYPE_C32BF16, R>(a, ch, n0, stride, v);
 }
 
 // Hide a pointer's value from the optimiser so that loads through it are not
@@ -88,7 +113,7 @@ __device__ __forceinline__ float parabolic(float yl, float yc, float yr) {
   return d;
 }
 
-__device__ __noinline__ void write_record(MaxValue* dst, uint32_t k, float2 x, float yl, float yc, float yr,
+__device__ __forceinline__ void write_record(MaxValue* dst, uint32_t k, float2 x, float yl, float yc, float yr,
                                              const FusedArgs& a) {
   float d = (a.refine != 0) ? parabolic(yl, yc, yr) : 0.0f;
   MaxValue mv;
@@ -216,8 +241,7 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
     for (int b = 0; b < PPT / R0; ++b) {
       const int j = tid + b * T;
       if (j < L / R0) {
-#pragma unroll
-        for (int r = 0; r < R0; ++r) v[b * R0 + r] = load_in(a, ch, (uint32_t)(j + r * (L / R0)));
+        load_run<R0>(a, ch, (uint32_t)j, (uint32_t)(L / R0), &v[b * R0]);
       }
     }
     fft_tile<LAY, L, 1, T, PPT>(lds, v, a.tw_a, tid);
@@ -338,7 +362,7 @@ struct Cand {
   float re, im, ml, mc, mr;
 };
 
-__device__ __noinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t cap, uint32_t k, float2 x, float yl,
+__device__ __forceinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t cap, uint32_t k, float2 x, float yl,
                                       float yc, float yr) {
   const uint32_t p = atomicAdd(ncand, 1u);
   if (p < cap) {
@@ -435,11 +459,9 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
         ai.window = opaque(a.window);
         const int n1 = t * CB + c;
         float2 v[PPT];
+        load_run<16>(ai, ch, (uint32_t)(n1 + N1 * jj), (uint32_t)(16 * N1), v);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          v[r] = load_in(ai, ch, (uint32_t)(n1 + N1 * (jj + 16 * r)));
-          energy = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, energy));
-        }
+        for (int r = 0; r < 16; ++r) energy = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, energy));
         fft_tile<LAY1, N2, CB, T, PPT>(lds, v, ai.tw_b, tid);
         // v[r] = Y[n1][k2 = jj + 16 r]
 #pragma unroll
@@ -557,25 +579,38 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
         }
         __syncthreads();
         if (s_row >= 1 && s_row <= RB - 2) {
-          // cheap screen in registers, then the rare interesting bins one by one
+          // best-bin tracking with selects; local maxima that can clear the
+          // threshold are rare and handled one by one afterwards
           uint32_t mask = 0;
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
             const int k1 = jj_row + r * (N1 / 16);
+            const uint32_t k = (uint32_t)k2 + 256u * (uint32_t)k1;
             const float yl = ldsf[(s_row - 1) * N1 + k1];
             const float yr = ldsf[(s_row + 1) * N1 + k1];
-            const bool lmax = m[r] > yl && m[r] > yr && m[r] >= lb;
-            if (lmax || m[r] >= bm) mask |= 1u << r;
+            const bool ok = full_band || in_range(k, a);
+            const bool upd = ok && (m[r] > bm || (m[r] == bm && k < bk));
+            bm = upd ? m[r] : bm;
+            bk = upd ? k : bk;
+            bx.x = upd ? v[r].x : bx.x;
+            bx.y = upd ? v[r].y : bx.y;
+            bl = upd ? yl : bl;
+            br = upd ? yr : br;
+            if (ok && m[r] > yl && m[r] > yr && m[r] >= lb) mask |= 1u << r;
           }
           while (mask) {
             const int r = __builtin_ctz(mask);
             mask &= mask - 1;
             const int k1 = jj_row + r * (N1 / 16);
             float2 x = v[0];
+            float mc = m[0];
 #pragma unroll
-            for (int q = 1; q < 16; ++q) if (q == r) x = v[q];
-            decide((uint32_t)k2 + 256u * (uint32_t)k1, x, ldsf[(s_row - 1) * N1 + k1], mag_of(x),
-                   ldsf[(s_row + 1) * N1 + k1]);
+            for (int q = 1; q < 16; ++q) {
+              x = (q == r) ? v[q] : x;
+              mc = (q == r) ? m[q] : mc;
+            }
+            push_cand(cand, &sh.ncand, (uint32_t)C::CAND_CAP, (uint32_t)k2 + 256u * (uint32_t)k1, x,
+                      ldsf[(s_row - 1) * N1 + k1], mc, ldsf[(s_row + 1) * N1 + k1]);
           }
         }
         __syncthreads();
@@ -628,9 +663,13 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
       const uint32_t ncand = min(sh.ncand, (uint32_t)C::CAND_CAP);
       for (int i = opaque_v(tid0); i < C::NBK; i += T) sh.hist[i] = 0;
       __syncthreads();
-      for (uint32_t i = opaque_v(tid0); i < ncand; i += T) {
-        const Cand cd = gp(cand)[i];
-        if (cd.mc > thr) atomicAdd(&sh.hist[cd.k >> C::BSH], 1);
+      for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
+        Cand cd[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cd[q] = gp(cand)[min(i0 + q * T, ncand - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (i0 + q * T < ncand && cd[q].mc > thr) atomicAdd(&sh.hist[cd[q].k >> C::BSH], 1);
       }
       __syncthreads();
       // inclusive scan over buckets (one bucket per thread)
@@ -646,11 +685,17 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
       // gather (key = bin, value = candidate index) of survivors in buckets <= bstar
       uint2* gat = reinterpret_cast<uint2*>(lds);
       constexpr int GMAX = C::LDS / 2 > 4096 ? 4096 : C::LDS / 2;
-      for (uint32_t i = opaque_v(tid0); i < ncand; i += T) {
-        const Cand cd = gp(cand)[i];
-        if (cd.mc > thr && (int)(cd.k >> C::BSH) <= bstar) {
-          const uint32_t p = atomicAdd(&sh.ngather, 1u);
-          if (p < (uint32_t)GMAX) gat[p] = make_uint2(cd.k, i);
+      for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
+        Cand cd[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cd[q] = gp(cand)[min(i0 + q * T, ncand - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t i = i0 + q * T;
+          if (i < ncand && cd[q].mc > thr && (int)(cd[q].k >> C::BSH) <= bstar) {
+            const uint32_t p = atomicAdd(&sh.ngather, 1u);
+            if (p < (uint32_t)GMAX) gat[p] = make_uint2(cd[q].k, i);
+          }
         }
       }
       __syncthreads();
