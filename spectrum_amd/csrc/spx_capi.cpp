@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -50,20 +51,34 @@ bool is_pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
 
 constexpr uint32_t kMaxN = 1u << 18;
 
-// W_N^i = exp(-2 pi i / N), i in [0, N): computed in FP64, rounded once.
-std::vector<float> make_twiddles(uint32_t N) {
-  std::vector<float> t(2 * (size_t)N);
-  for (uint32_t i = 0; i < N; ++i) {
-    const double a = 2.0 * M_PI * ((double)i / (double)N);
-    t[2 * (size_t)i] = (float)std::cos(a);
-    t[2 * (size_t)i + 1] = (float)-std::sin(a);
+// Per-pass twiddle tables of a length-L transform (radix-16 passes first, the
+// leftover radix last; pass p has sub-length NS = 16^p): pass p >= 1 stores
+// [k][r] = W_(NS*R)^(r*k), k < NS, r < R, passes concatenated.  FP64, rounded once.
+std::vector<float> make_twiddles(uint32_t L) {
+  int log = 0;
+  while ((1u << log) < L) ++log;
+  const int np = (log + 3) / 4;
+  const int rl = (log % 4) ? (1 << (log % 4)) : 16;
+  std::vector<float> t;
+  uint64_t ns = 16;
+  for (int p = 1; p < np; ++p, ns *= 16) {
+    const int R = (p == np - 1) ? rl : 16;
+    for (uint64_t k = 0; k < ns; ++k)
+      for (int r = 0; r < R; ++r) {
+        const uint64_t m = ns * (uint64_t)R;
+        const uint64_t e = (k * (uint64_t)r) % m;
+        const double a = 2.0 * M_PI * ((double)e / (double)m);
+        t.push_back((float)std::cos(a));
+        t.push_back((float)-std::sin(a));
+      }
   }
+  if (t.empty()) t.assign(2, 0.0f);
   return t;
 }
 
-// four-step table: entry [k2 * N1 + n1] = W_N^(n1 * k2), N = N1 * 256.
+// four-step table: entry [k2 * 256 + n1] = W_N^(n1 * k2), N = 256 * N2.
 std::vector<float> make_fourstep(uint32_t N) {
-  const uint32_t N2 = 256, N1 = N / N2;
+  const uint32_t N1 = 256, N2 = N / N1;
   std::vector<float> t(2 * (size_t)N);
   for (uint32_t k2 = 0; k2 < N2; ++k2)
     for (uint32_t n1 = 0; n1 < N1; ++n1) {
@@ -101,7 +116,12 @@ int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out) {
     return SPX_OK;
   }
   std::vector<float> host;
-  if (kind == TW) host = make_twiddles(a);
+  if (kind == TW) {
+    host = make_twiddles(a);
+    const int want = spx_twiddle_table_len(a);
+    if (want < 0 || (size_t)std::max(want, 1) * 2 != host.size())
+      return fail(c, SPX_EINVAL, "twiddle table size mismatch");
+  }
   else if (kind == TW4) host = make_fourstep(a);
   else host = make_window(a, b);
   void* d = nullptr;
@@ -147,11 +167,11 @@ int check_fft_shape(spx_ctx* c, const void* in, const void* out, uint32_t batch,
 int prepare(spx_ctx* c, FusedArgs& a, int* grid) {
   int rc;
   if (spx_fused_is_large(a.nfft)) {
-    const uint32_t N1 = a.nfft / 256;
+    const uint32_t N2 = a.nfft / 256;
     const void* t;
-    if ((rc = get_table(c, TW, N1, 0, &t))) return rc;
-    a.tw_a = (const float2*)t;
     if ((rc = get_table(c, TW, 256, 0, &t))) return rc;
+    a.tw_a = (const float2*)t;
+    if ((rc = get_table(c, TW, N2, 0, &t))) return rc;
     a.tw_b = (const float2*)t;
     if ((rc = get_table(c, TW4, a.nfft, 0, &t))) return rc;
     a.tw_4 = (const float2*)t;

@@ -147,24 +147,32 @@ template <> struct Dft<16> {
 };
 
 // ---------------------------------------------------------------------------
-// Radix plan for a power-of-two length L: the odd radix first (no twiddles
-// there), then radix-16 passes.  L = 2^a: passes = ceil(a/4).
+// Radix plan for a power-of-two length L = 2^a: radix-16 passes first, the
+// leftover radix (2, 4 or 8) last; passes = ceil(a/4).  With this order every
+// LDS access of every pass is base + compile-time stride (see the layouts).
+// Twiddles: pass p >= 1 (sub-length NS = 16^p, radix R) reads a table laid
+// out [k][r] = W_(NS*R)^(r*k), k < NS, r < R, so one thread's R twiddles are
+// contiguous (R/2 16-byte loads from one address).  The tables of all passes
+// are concatenated; twoff(p) is the offset of pass p in float2 units.
 // ---------------------------------------------------------------------------
 template <int L>
 struct Plan {
   static constexpr int LOG = ilog2(L);
   static constexpr int NP = (LOG + 3) / 4;
-  static constexpr int R0 = (LOG % 4) ? (1 << (LOG % 4)) : 16;
-  static constexpr int radix(int p) { return p == 0 ? R0 : 16; }
-  static constexpr int ns(int p) { return p == 0 ? 1 : R0 * ipow16(p - 1); }
-  static constexpr int RL = radix(NP - 1);  // radix of the last pass
+  static constexpr int RL = (LOG % 4) ? (1 << (LOG % 4)) : 16;  // last-pass radix
+  static constexpr int radix(int p) { return p == NP - 1 ? RL : 16; }
+  static constexpr int ns(int p) { return ipow16(p); }
+  static constexpr int twoff(int p) { return p <= 1 ? 0 : twoff(p - 1) + ns(p - 1) * radix(p - 1); }
+  static constexpr int TWSIZE = NP <= 1 ? 1 : twoff(NP - 1) + ns(NP - 1) * radix(NP - 1);
 };
 
 // ---------------------------------------------------------------------------
 // LDS layouts for a tile of NSEQ sequences of length L.
 //  Contig: element i of sequence s at pad(s*L + i), pad(a) = a + a/16
 //          (one spare slot per 16 complex breaks the 128-byte stride of the
-//          radix-16 writes across the 64 LDS banks).
+//          radix-16 writes across the LDS banks).  Because reads step by
+//          L/R = 16^k and writes by NS = 1 or 16^k, pad() stays linear:
+//          element r of a butterfly is at base + r * (compile-time stride).
 //  Inter:  element i of sequence s at i*NSEQ + s (sequences interleaved; the
 //          lanes of a wave walk s, so every access is unit-stride).
 // Thread mapping of butterfly j (radix R): Contig walks jj fastest, Inter walks s.
@@ -176,6 +184,8 @@ struct LayContig {
     int a = s * L + i;
     return a + (a >> 4);
   }
+  static constexpr int rstride(int R) { return (L / R) % 16 == 0 ? (L / R) + (L / R) / 16 : -1; }
+  static constexpr int wstride(int NS) { return NS == 1 ? 1 : (NS % 16 == 0 ? NS + NS / 16 : -1); }
   template <int R>
   __device__ __forceinline__ static void map(int j, int& s, int& jj) {
     s = j / (L / R);
@@ -187,6 +197,8 @@ template <int L, int NSEQ>
 struct LayInter {
   static constexpr int SIZE = L * NSEQ;
   __device__ __forceinline__ static int idx(int s, int i) { return i * NSEQ + s; }
+  static constexpr int rstride(int R) { return (L / R) * NSEQ; }
+  static constexpr int wstride(int NS) { return NS * NSEQ; }
   template <int R>
   __device__ __forceinline__ static void map(int j, int& s, int& jj) {
     s = j % NSEQ;
@@ -194,18 +206,37 @@ struct LayInter {
   }
 };
 
+// R contiguous twiddles (a row of a [k][r] table) with 16-byte loads.
+template <int R>
+__device__ __forceinline__ void load_tw_row(const float2* row, float2* w) {
+  if constexpr (R == 2) {
+    w[0] = make_float2(1.f, 0.f);
+    w[1] = gp(row)[1];
+  } else {
+    const float4* q = reinterpret_cast<const float4*>(row);
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) {
+      const float4 t = gp(q)[i];
+      w[2 * i] = make_float2(t.x, t.y);
+      w[2 * i + 1] = make_float2(t.z, t.w);
+    }
+  }
+}
+
 // One Stockham pass (radix R, current sub-transform length NS) over a tile of
 // NSEQ sequences of length L held by T threads with PPT points each.
 //  in : v[b*R + r] = x[s][jj + r*L/R]   (loaded from LDS unless FIRST)
 //  out: LDS y[s][(jj/NS)*NS*R + jj%NS + r*NS]           (unless LAST)
 //       or, for LAST, v[b*R + r] = X[s][jj + r*L/R] in registers.
-// Twiddles W_(NS*R)^(r*(jj%NS)) = tw[r*(jj%NS)*L/(NS*R)], tw[i] = W_L^i.
+// Twiddles W_(NS*R)^(r*(jj%NS)) from the pass table twp[k*R + r].
 template <class LAY, int L, int NSEQ, int R, int NS, int T, int PPT, bool FIRST, bool LAST>
 __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[PPT],
-                                         const float2* __restrict__ tw, int tid) {
+                                         const float2* __restrict__ twp, int tid) {
   constexpr int BPT = PPT / R;
   constexpr int NB = NSEQ * L / R;            // butterflies in the tile
   constexpr bool FULL = (NB == T * BPT);      // every thread busy
+  constexpr int RS = LAY::rstride(R);
+  constexpr int WS = LAY::wstride(NS);
   if constexpr (!FIRST) {
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
@@ -213,8 +244,14 @@ __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[P
       if (FULL || j < NB) {
         int s, jj;
         LAY::template map<R>(j, s, jj);
+        if constexpr (RS > 0) {
+          const float2* base = lds + LAY::idx(s, jj);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b * R + r] = lds[LAY::idx(s, jj + r * (L / R))];
+          for (int r = 0; r < R; ++r) v[b * R + r] = base[r * RS];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[b * R + r] = lds[LAY::idx(s, jj + r * (L / R))];
+        }
       }
     }
   }
@@ -226,9 +263,10 @@ __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[P
         int s, jj;
         LAY::template map<R>(j, s, jj);
         const int k = jj % NS;
-        constexpr int STEP = L / (NS * R);
+        float2 w[R];
+        load_tw_row<R>(twp + (uint32_t)(k * R), w);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], gp(tw)[r * k * STEP]);
+        for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], w[r]);
       }
       Dft<R>::run(&v[b * R]);
     }
@@ -241,9 +279,15 @@ __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[P
       if (FULL || j < NB) {
         int s, jj;
         LAY::template map<R>(j, s, jj);
-        const int base = (jj / NS) * NS * R + (jj % NS);
+        const int d0 = (jj / NS) * NS * R + (jj % NS);
+        if constexpr (WS > 0) {
+          float2* base = lds + LAY::idx(s, d0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[LAY::idx(s, base + r * NS)] = v[b * R + r];
+          for (int r = 0; r < R; ++r) base[r * WS] = v[b * R + r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) lds[LAY::idx(s, d0 + r * NS)] = v[b * R + r];
+        }
       }
     }
     __syncthreads();
@@ -254,14 +298,16 @@ template <class LAY, int L, int NSEQ, int T, int PPT, int P>
 struct FftPasses {
   __device__ __forceinline__ static void run(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
     using PL = Plan<L>;
-    fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1>(lds, v, tw, tid);
+    fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1>(lds, v, tw + PL::twoff(P),
+                                                                                       tid);
     if constexpr (P + 1 < PL::NP) FftPasses<LAY, L, NSEQ, T, PPT, P + 1>::run(lds, v, tw, tid);
   }
 };
 
 // Full length-L transform of the tile.  Before the call v holds the first
-// pass's inputs (x[s][jj + r*L/R0] for the radix-R0 mapping of j = tid + b*T);
-// after it v holds X[s][jj + r*L/RL] for the radix-RL mapping.
+// pass's inputs (x[s][jj + r*L/R0], R0 = Plan<L>::radix(0), for the mapping of
+// j = tid + b*T); after it v holds X[s][jj + r*L/RL] for the radix-RL mapping.
+// tw: the concatenated per-pass twiddle tables of length L (Plan<L>::TWSIZE).
 template <class LAY, int L, int NSEQ, int T, int PPT>
 __device__ __forceinline__ void fft_tile(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
   FftPasses<LAY, L, NSEQ, T, PPT, 0>::run(lds, v, tw, tid);

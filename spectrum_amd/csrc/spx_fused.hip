@@ -10,16 +10,16 @@
 //    |X|, mean/std, local maxima, the block scan and the records are produced
 //    from the LDS copy of the spectrum; nothing but records leaves the chip.
 //
-//  * large  (N = N1 * 256, N >= 32768): persistent grid, one workgroup per CU,
-//    channels strided over workgroups.  Phase 1 runs 256-point column FFTs
-//    (64 columns per tile, interleaved LDS layout) and writes the twiddled
-//    intermediate Z[k2][n1] to a per-workgroup scratch slot (L2/MALL
-//    resident); phase 2 runs N1-point row FFTs; local maxima of interior rows
-//    are decided from the LDS magnitude tile, tile-boundary rows from a small
-//    edge buffer.  Only local maxima that can clear the threshold are kept as
-//    candidates: by Parseval mean|X|^2 = sum|x_w|^2 =: P, and mean + k*std
-//    >= min(1,k)*sqrt(P), so candidates below that bound are dropped before
-//    the mean and std are known.  Phase 3 filters candidates by the final
+//  * large  (N = 256 * N2, N >= 32768): persistent grid, one workgroup per
+//    CU, channels strided over workgroups.  Phase 1 runs N2-point column FFTs
+//    (16384/N2 columns per tile, interleaved LDS layout) and writes the
+//    twiddled intermediate Z[k2][n1] to a per-workgroup scratch slot; phase 2
+//    runs 256-point row FFTs; local maxima of interior rows are decided from
+//    the LDS magnitude tile, tile-boundary rows from a small edge buffer.
+//    Only local maxima that can clear the threshold are kept as candidates:
+//    by Parseval mean|X|^2 = sum|x_w|^2 =: P, and mean + k*std >=
+//    min(1,k)*sqrt(P), so candidates below that bound are dropped before the
+//    mean and std are known.  Phase 3 filters candidates by the final
 //    threshold and selects the first K in ascending bin order (bucket
 //    histogram + bitonic sort of the boundary bucket).
 //
@@ -45,46 +45,57 @@ using MaxValue = spx_max_value;
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
 template <int DT>
-__device__ __forceinline__ float2 ld_raw(const void* in, uint64_t i) {
+__device__ __forceinline__ float2 ld_raw(const void* chan_base, uint32_t i) {
   if constexpr (DT == SPX_DTYPE_C64) {
-    return gp(reinterpret_cast<const float2*>(in))[i];
+    return gp(reinterpret_cast<const float2*>(chan_base))[i];
   } else if constexpr (DT == SPX_DTYPE_C32F16) {
-    const __half2 h = gp(reinterpret_cast<const __half2*>(in))[i];
+    const __half2 h = gp(reinterpret_cast<const __half2*>(chan_base))[i];
     return __half22float2(h);
   } else {
-    const uint32_t u = gp(reinterpret_cast<const uint32_t*>(in))[i];
+    const uint32_t u = gp(reinterpret_cast<const uint32_t*>(chan_base))[i];
     return make_float2(bf16_to_f((uint16_t)(u & 0xffffu)), bf16_to_f((uint16_t)(u >> 16)));
   }
 }
 
-// v[r] = scale * w[n] * x[n] for n = n0 + r*stride, r < R; zero for n >= n_point.
-// Indices are clamped instead of branched on, so all R loads (and R window
-// loads) are in flight together before the first use.
-template <int DT, int R>
+// v[r] = scale * w[n] * x[n] for n = n0 + r*stride, r < R; zero for n >= n_point
+// (PAD only).  Offsets are 32-bit from a per-channel scalar base, and indices
+// are clamped instead of branched on, so all R loads (and the R window loads)
+// are in flight together before the first use.
+template <int DT, int R, bool PAD>
 __device__ __forceinline__ void load_run_t(const FusedArgs& a, uint64_t ch, uint32_t n0, uint32_t stride, float2* v) {
+  constexpr int ES = DT == SPX_DTYPE_C64 ? 8 : 4;
+  const void* cb = reinterpret_cast<const char*>(a.in) + ch * (uint64_t)a.n_point * ES;
   const uint32_t last = a.n_point - 1;
-  const uint64_t base = ch * (uint64_t)a.n_point;
   float2 raw[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) raw[r] = ld_raw<DT>(a.in, base + min(n0 + r * stride, last));
+  for (int r = 0; r < R; ++r) raw[r] = ld_raw<DT>(cb, PAD ? min(n0 + r * stride, last) : n0 + r * stride);
   float wv[R];
   if (a.window != nullptr) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) wv[r] = gp(a.window)[min(n0 + r * stride, last)];
+    for (int r = 0; r < R; ++r) wv[r] = gp(a.window)[PAD ? min(n0 + r * stride, last) : n0 + r * stride];
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) wv[r] = 1.0f;
   }
   const float sc = (a.in_scale != nullptr) ? gp(a.in_scale)[ch] : 1.0f;
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = c_scale(raw[r], (n0 + r * stride < a.n_point) ? wv[r] * sc : 0.0f);
+  for (int r = 0; r < R; ++r) {
+    const float f = PAD ? ((n0 + r * stride < a.n_point) ? wv[r] * sc : 0.0f) : wv[r] * sc;
+    v[r] = c_scale(raw[r], f);
+  }
 }
 
 template <int R>
 __device__ __forceinline__ void load_run(const FusedArgs& a, uint64_t ch, uint32_t n0, uint32_t stride, float2* v) {
-  if (a.dtype == SPX_DTYPE_C64) load_run_t<SPX_DTYPE_C64, R>(a, ch, n0, stride, v);
-  else if (a.dtype == SPX_DTYPE_C32F16) load_run_t<SPX_DTYPE_C32F16, R>(a, ch, n0, stride, v);
-  else load_run_t<SPX_DTYPE_C32BF16, R>(a, ch, n0, stride, v);
+  if (a.n_point == a.nfft) {
+    if (a.dtype == SPX_DTYPE_C64) load_run_t<SPX_DTYPE_C64, R, false>(a, ch, n0, stride, v);
+    else if (a.dtype == SPX_DTYPE_C32F16) load_run_t<SPX_DTYPE_C32F16, R, false>(a, ch, n0, stride, v);
+    else load_run_t<SPX_DTYPE_C32BF16, R, false>(a, ch, n0, stride, v);
+  } else {
+    if (a.dtype == SPX_DTYPE_C64) load_run_t<SPX_DTYPE_C64, R, true>(a, ch, n0, stride, v);
+    else if (a.dtype == SPX_DTYPE_C32F16) load_run_t<SPX_DTYPE_C32F16, R, true>(a, ch, n0, stride, v);
+    else load_run_t<SPX_DTYPE_C32BF16, R, true>(a, ch, n0, stride, v);
+  }
 }
 
 // Hide a pointer's value from the optimiser so that loads through it are not
@@ -101,7 +112,8 @@ __device__ __forceinline__ P* opaque(P* p) {
   return p + z;
 }
 
-__device__ __forceinline__ float mag_of(float2 x) { return __builtin_sqrtf(fmaf(x.x, x.x, x.y * x.y)); }
+// |X| with the hardware square root (v_sqrt_f32, ~1 ulp)
+__device__ __forceinline__ float mag_of(float2 x) { return __builtin_amdgcn_sqrtf(fmaf(x.x, x.x, x.y * x.y)); }
 
 __device__ __forceinline__ float parabolic(float yl, float yc, float yr) {
   float den = yl - 2.0f * yc + yr;
@@ -224,7 +236,7 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
   using PL = Plan<L>;
   using LAY = typename C::LAY;
   constexpr int T = C::T, PPT = C::PPT;
-  constexpr int R0 = PL::R0, RL = PL::RL;
+  constexpr int R0 = PL::radix(0), RL = PL::RL;
   __shared__ float2 lds[LAY::SIZE];
   __shared__ BlockRed red;
   const int tid = threadIdx.x;
@@ -355,7 +367,9 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
 }
 
 // ===========================================================================
-// large path: N = N1 * 256
+// large path: N = 256 * N2 (N2 = 128 .. 1024), x[n1 + 256*n2]
+//   phase 1: column n1, N2-point DFT over n2, times W_N^(n1*k2) -> Z[k2][n1]
+//   phase 2: row k2, 256-point DFT over n1               -> X[k2 + N2*k1]
 // ===========================================================================
 struct Cand {
   uint32_t k;
@@ -363,7 +377,7 @@ struct Cand {
 };
 
 __device__ __forceinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t cap, uint32_t k, float2 x, float yl,
-                                      float yc, float yr) {
+                                          float yc, float yr) {
   const uint32_t p = atomicAdd(ncand, 1u);
   if (p < cap) {
     Cand cd;
@@ -372,58 +386,57 @@ __device__ __forceinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t 
   }
 }
 
-template <int N1>
+template <int N2_>
 struct LargeCfg {
-  static constexpr int N2 = 256;
+  static constexpr int N1 = 256;                // phase-2 row length (radix 16 x 16)
+  static constexpr int N2 = N2_;                // phase-1 column length
   static constexpr int N = N1 * N2;
   static constexpr int T = 1024;
   static constexpr int PPT = 16;
-  static constexpr int TILE = T * PPT;          // 16384 points
-  static constexpr int CB = TILE / N2;          // 64 columns per phase-1 tile
-  static constexpr int RB = TILE / N1;          // rows per phase-2 tile
+  static constexpr int TILE = T * PPT;          // 16384 points per tile
+  static constexpr int CB = TILE / N2;          // phase-1 columns per tile
+  static constexpr int RB = TILE / N1;          // phase-2 rows per tile (64)
   static constexpr int NT1 = N1 / CB;           // phase-1 tiles
   static constexpr int NT2 = N2 / RB;           // phase-2 tiles
+  static constexpr int RL1 = Plan<N2>::RL;      // last radix of the column transform
   using LAY1 = LayInter<N2, CB>;
   using LAY2 = LayContig<N1, RB>;
   static constexpr int LDS = LAY2::SIZE > LAY1::SIZE ? LAY2::SIZE : LAY1::SIZE;
   // scratch slot layout (bytes)
   static constexpr uint64_t Z_BYTES = (uint64_t)N * 8;
   static constexpr uint64_t CAND_CAP = N / 2 + 2;
-  static constexpr uint64_t CAND_BYTES = CAND_CAP * sizeof(Cand);
+  static constexpr uint64_t CAND_BYTES = (CAND_CAP * sizeof(Cand) + 255) / 256 * 256;
   static constexpr uint64_t EDGE_M_BYTES = (uint64_t)NT2 * 4 * N1 * 4;
   static constexpr uint64_t EDGE_X_BYTES = (uint64_t)NT2 * 2 * N1 * 8;
   static constexpr uint64_t SLOT_BYTES = Z_BYTES + CAND_BYTES + EDGE_M_BYTES + EDGE_X_BYTES;
   static constexpr int NBK = 1024;              // selection buckets
   static constexpr int BSH = ilog2(N) - 10;     // bucket = k >> BSH
-  static_assert(RB >= 4, "phase-2 tile needs at least four rows");
-  static_assert(NT1 >= 1 && NT2 >= 1, "tile shape");
+  static_assert(NT1 >= 1 && NT2 >= 1 && CB >= 16, "tile shape");
+  static_assert(Plan<N1>::RL == 16, "rows end on a radix-16 pass");
 };
 
-template <int N1>
+template <int N2>
 struct LargeShared {
-  using C = LargeCfg<N1>;
+  using C = LargeCfg<N2>;
   float2 tile[C::LDS];
   BlockRed red;
   int hist[C::NBK];
   uint32_t ncand;
   uint32_t ngather;
   int bstar;
-  float thr;
 };
 
 // MODE 0: dense outputs only (spectrum / |X| / |X|^2 / mag+phase); MODE 1: peaks only.
-template <int N1, int MODE>
+template <int N2, int MODE>
 __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
-  using C = LargeCfg<N1>;
-  constexpr int N = C::N, N2 = C::N2, T = C::T, PPT = C::PPT, CB = C::CB, RB = C::RB;
-  constexpr int NT1 = C::NT1, NT2 = C::NT2;
+  using C = LargeCfg<N2>;
+  constexpr int N = C::N, N1 = C::N1, T = C::T, PPT = C::PPT, CB = C::CB, RB = C::RB;
+  constexpr int NT1 = C::NT1, NT2 = C::NT2, RL1 = C::RL1;
   using LAY1 = typename C::LAY1;
   using LAY2 = typename C::LAY2;
-  constexpr int R0 = Plan<N1>::R0;
-  static_assert(Plan<N1>::RL == 16 && Plan<N2>::RL == 16, "last passes are radix 16");
 
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  LargeShared<N1>& sh = *reinterpret_cast<LargeShared<N1>*>(smem_raw);
+  LargeShared<N2>& sh = *reinterpret_cast<LargeShared<N2>*>(smem_raw);
   float2* lds = sh.tile;
   float* ldsf = reinterpret_cast<float*>(sh.tile);
   const int tid0 = threadIdx.x;
@@ -440,35 +453,33 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
   for (uint32_t ch = blockIdx.x; ch < a.batch; ch += gridDim.x) {
     int tid = opaque_v(tid0);
     FusedArgs ai = a;
-    ai.tw_a = opaque(a.tw_a);
-    ai.tw_b = opaque(a.tw_b);
-    ai.tw_4 = opaque(a.tw_4);
-    ai.window = opaque(a.window);
     // ------------------------------------------------------------------
-    // phase 1: columns n1, 256-point DFT over n2, twiddle W_N^(n1*k2),
-    //          store Z[k2][n1]
+    // phase 1
     // ------------------------------------------------------------------
     float energy = 0.f;
-    {
 #pragma unroll 1
-      for (int t = 0; t < NT1; ++t) {
-        tid = opaque_v(tid0);
-        const int c = tid % CB, jj = tid / CB;  // radix-16 first pass: jj in [0,16)
-        ai.tw_b = opaque(a.tw_b);
-        ai.tw_4 = opaque(a.tw_4);
-        ai.window = opaque(a.window);
-        const int n1 = t * CB + c;
-        float2 v[PPT];
-        load_run<16>(ai, ch, (uint32_t)(n1 + N1 * jj), (uint32_t)(16 * N1), v);
+    for (int t = 0; t < NT1; ++t) {
+      tid = opaque_v(tid0);
+      ai.tw_b = opaque(a.tw_b);
+      ai.tw_4 = opaque(a.tw_4);
+      ai.window = opaque(a.window);
+      const int c = tid % CB, jj = tid / CB;  // first pass is radix 16: jj in [0, N2/16)
+      const int n1 = t * CB + c;
+      float2 v[PPT];
+      load_run<16>(ai, ch, (uint32_t)(n1 + N1 * jj), (uint32_t)(N1 * (N2 / 16)), v);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) energy = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, energy));
-        fft_tile<LAY1, N2, CB, T, PPT>(lds, v, ai.tw_b, tid);
-        // v[r] = Y[n1][k2 = jj + 16 r]
+      for (int r = 0; r < 16; ++r) energy = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, energy));
+      fft_tile<LAY1, N2, CB, T, PPT>(lds, v, ai.tw_b, tid);
+      // v[b*RL1 + r] = Y[column s][k2 = jj' + r*N2/RL1] for j = tid + b*T
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int k2 = jj + 16 * r;
-          const float2 w = gp(ai.tw_4)[k2 * N1 + n1];
-          gp(Z)[k2 * N1 + n1] = c_mul(v[r], w);
+      for (int b = 0; b < PPT / RL1; ++b) {
+        int s, jb;
+        LAY1::template map<RL1>(tid + b * T, s, jb);
+        const uint32_t col = (uint32_t)(t * CB + s);
+#pragma unroll
+        for (int r = 0; r < RL1; ++r) {
+          const uint32_t e = (uint32_t)(jb + r * (N2 / RL1)) * N1 + col;
+          gp(Z)[e] = c_mul(v[b * RL1 + r], gp(ai.tw_4)[e]);
         }
       }
     }
@@ -478,19 +489,17 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
     double P;
     {
       double e = energy, dummy = 0.0;
-      float bm = -1.f;
-      uint32_t bk = 0;
-      block_stats<T>(e, dummy, bm, bk, sh.red);
+      float bmx = -1.f;
+      uint32_t bkx = 0;
+      block_stats<T>(e, dummy, bmx, bkx, sh.red);
       P = e;  // sum_n |x_w[n]|^2 = (1/N) sum_k |X[k]|^2
     }
     float lb = 0.f;
-    if (a.peak_mode == 1 && full_band && a.nsigma >= 0.f) {
-      lb = (float)(fmin(1.0, (double)a.nsigma) * sqrt(P) * 0.999);
-    }
+    if (a.peak_mode == 1 && full_band && a.nsigma >= 0.f) lb = (float)(fmin(1.0, (double)a.nsigma) * sqrt(P) * 0.999);
     if (tid == 0) sh.ncand = 0;
 
     // ------------------------------------------------------------------
-    // phase 2: rows k2, N1-point DFT over n1 -> X[k2 + 256*k1]
+    // phase 2
     // ------------------------------------------------------------------
     float fs1 = 0.f, fs2 = 0.f;
     float bm = -1.f;
@@ -498,51 +507,35 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
     float2 bx = make_float2(0.f, 0.f);
     float bl = 0.f, br = 0.f;
 
-    auto decide = [&](uint32_t k, float2 x, float yl, float yc, float yr) {
-      if (!in_range(k, a)) return;
-      if (yc > bm || (yc == bm && k < bk)) {
-        bm = yc; bk = k; bx = x; bl = yl; br = yr;
-      }
-      if (yc > yl && yc > yr && yc >= lb) push_cand(cand, &sh.ncand, (uint32_t)C::CAND_CAP, k, x, yl, yc, yr);
-    };
-
 #pragma unroll 1
     for (int t = 0; t < NT2; ++t) {
       tid = opaque_v(tid0);
-      const int s_row = tid / (N1 / 16), jj_row = tid % (N1 / 16);  // last-pass mapping
       ai.tw_a = opaque(a.tw_a);
+      const int s_row = tid / (N1 / 16), jj_row = tid % (N1 / 16);  // radix-16 mapping of both passes
       float2 v[PPT];
+      {
+        const float2* zr = Z + (uint32_t)((t * RB + s_row) * N1 + jj_row);
 #pragma unroll
-      for (int b = 0; b < PPT / R0; ++b) {
-        const int j = tid + b * T;
-        int s, jj;
-        LAY2::template map<R0>(j, s, jj);
-        const float2* zr = Z + (uint64_t)(t * RB + s) * N1;
-#pragma unroll
-        for (int r = 0; r < R0; ++r) v[b * R0 + r] = gp(zr)[jj + r * (N1 / R0)];
+        for (int r = 0; r < 16; ++r) v[r] = gp(zr)[r * (N1 / 16)];
       }
       fft_tile<LAY2, N1, RB, T, PPT>(lds, v, ai.tw_a, tid);
-      // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + r*N1/16]
+      // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]  ->  bin k2 + N2*k1
       const int k2 = t * RB + s_row;
       float m[16];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         m[r] = mag_of(v[r]);
-        const uint32_t k = (uint32_t)k2 + 256u * (uint32_t)(jj_row + r * (N1 / 16));
-        if (in_range(k, a)) {
+        if (full_band || in_range((uint32_t)k2 + (uint32_t)N2 * (uint32_t)(jj_row + 16 * r), a)) {
           fs1 += m[r];
           fs2 = fmaf(m[r], m[r], fs2);
         }
       }
-
-      constexpr bool want_out = (MODE == 0);
       __syncthreads();  // LDS free again
-      if constexpr (want_out) {
+      if constexpr (!peaks) {
         // transpose through LDS so that natural-order runs of RB bins are written
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int k1 = jj_row + r * (N1 / 16);
-          const int e = k1 * RB + s_row;
+          const int e = (jj_row + 16 * r) * RB + s_row;
           lds[e + (e >> 4)] = v[r];
         }
         __syncthreads();
@@ -551,27 +544,22 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
         for (int i = 0; i < 16; ++i) {
           const int e = tid + i * T;
           const int k1 = e / RB, s = e % RB;
-          const uint64_t k = (uint64_t)(t * RB + s) + 256ull * k1;
-          store_dense(a, cbase + k, lds[e + (e >> 4)]);
+          store_dense(a, cbase + (uint64_t)(t * RB + s) + (uint64_t)N2 * k1, lds[e + (e >> 4)]);
         }
-      }
-      if constexpr (peaks) {
-        // magnitude tile [s][k1] and edge rows
+      } else {
+        // magnitude tile [s][k1] and the edge rows 0, 1, RB-2, RB-1
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int k1 = jj_row + r * (N1 / 16);
-          ldsf[s_row * N1 + k1] = m[r];
-        }
+        for (int r = 0; r < 16; ++r) ldsf[s_row * N1 + jj_row + 16 * r] = m[r];
         {
           int slotm = -1, slotx = -1;
           if (s_row == 0) { slotm = 0; slotx = 0; }
           else if (s_row == 1) slotm = 1;
-          if (s_row == RB - 2) slotm = 2;
-          if (s_row == RB - 1) { slotm = 3; slotx = 1; }
+          else if (s_row == RB - 2) slotm = 2;
+          else if (s_row == RB - 1) { slotm = 3; slotx = 1; }
           if (slotm >= 0) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-              const int k1 = jj_row + r * (N1 / 16);
+              const int k1 = jj_row + 16 * r;
               gp(edgeM)[(t * 4 + slotm) * N1 + k1] = m[r];
               if (slotx >= 0) gp(edgeX)[(t * 2 + slotx) * N1 + k1] = v[r];
             }
@@ -584,8 +572,8 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
           uint32_t mask = 0;
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
-            const int k1 = jj_row + r * (N1 / 16);
-            const uint32_t k = (uint32_t)k2 + 256u * (uint32_t)k1;
+            const int k1 = jj_row + 16 * r;
+            const uint32_t k = (uint32_t)k2 + (uint32_t)N2 * (uint32_t)k1;
             const float yl = ldsf[(s_row - 1) * N1 + k1];
             const float yr = ldsf[(s_row + 1) * N1 + k1];
             const bool ok = full_band || in_range(k, a);
@@ -601,7 +589,7 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
           while (mask) {
             const int r = __builtin_ctz(mask);
             mask &= mask - 1;
-            const int k1 = jj_row + r * (N1 / 16);
+            const int k1 = jj_row + 16 * r;
             float2 x = v[0];
             float mc = m[0];
 #pragma unroll
@@ -609,7 +597,7 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
               x = (q == r) ? v[q] : x;
               mc = (q == r) ? m[q] : mc;
             }
-            push_cand(cand, &sh.ncand, (uint32_t)C::CAND_CAP, (uint32_t)k2 + 256u * (uint32_t)k1, x,
+            push_cand(cand, &sh.ncand, (uint32_t)C::CAND_CAP, (uint32_t)k2 + (uint32_t)N2 * (uint32_t)k1, x,
                       ldsf[(s_row - 1) * N1 + k1], mc, ldsf[(s_row + 1) * N1 + k1]);
           }
         }
@@ -618,12 +606,11 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
     }
 
     if constexpr (peaks) {
-      tid = opaque_v(tid0);
       // phase 2b: tile-boundary rows from the edge buffers
-      constexpr int NB = 2 * NT2 * N1;
-      for (int i = opaque_v(tid0); i < NB; i += T) {
+      constexpr int NBND = 2 * NT2 * N1;
+      for (int i = opaque_v(tid0); i < NBND; i += T) {
         const int k1 = i % N1;
-        const int q = i / N1;          // 0 .. 2*NT2-1
+        const int q = i / N1;            // 0 .. 2*NT2-1
         const int t = q >> 1;
         const bool last = (q & 1) != 0;  // row RB-1 of tile t, else row 0
         float yl, yc, yr;
@@ -635,7 +622,7 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
           x = gp(edgeX)[(t * 2 + 0) * N1 + k1];
           yr = gp(edgeM)[(t * 4 + 1) * N1 + k1];
           if (t > 0) yl = gp(edgeM)[((t - 1) * 4 + 3) * N1 + k1];
-          else yl = gp(edgeM)[((NT2 - 1) * 4 + 3) * N1 + (k1 + N1 - 1) % N1];  // k-1 wraps to row 255, column k1-1
+          else yl = gp(edgeM)[((NT2 - 1) * 4 + 3) * N1 + (k1 + N1 - 1) % N1];  // k-1 wraps to row N2-1, column k1-1
         } else {
           k2 = (uint32_t)(t * RB + RB - 1);
           yc = gp(edgeM)[(t * 4 + 3) * N1 + k1];
@@ -644,7 +631,11 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
           if (t < NT2 - 1) yr = gp(edgeM)[((t + 1) * 4 + 0) * N1 + k1];
           else yr = gp(edgeM)[(0 * 4 + 0) * N1 + (k1 + 1) % N1];  // k+1 wraps to row 0, column k1+1
         }
-        decide(k2 + 256u * (uint32_t)k1, x, yl, yc, yr);
+        const uint32_t k = k2 + (uint32_t)N2 * (uint32_t)k1;
+        if (full_band || in_range(k, a)) {
+          if (yc > bm || (yc == bm && k < bk)) { bm = yc; bk = k; bx = x; bl = yl; br = yr; }
+          if (yc > yl && yc > yr && yc >= lb) push_cand(cand, &sh.ncand, (uint32_t)C::CAND_CAP, k, x, yl, yc, yr);
+        }
       }
       __syncthreads();
 
@@ -672,19 +663,19 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
           if (i0 + q * T < ncand && cd[q].mc > thr) atomicAdd(&sh.hist[cd[q].k >> C::BSH], 1);
       }
       __syncthreads();
-      // inclusive scan over buckets (one bucket per thread)
-      int hv = sh.hist[tid];
+      // exclusive scan over buckets (one bucket per thread)
+      int hv = sh.hist[tid0];
       int total;
       int excl = block_excl_scan<T>(hv, total, sh.red);
       const int kcap = (int)a.kcap;
-      if (tid == 0) { sh.bstar = C::NBK - 1; sh.ngather = 0; }
+      if (tid0 == 0) { sh.bstar = C::NBK - 1; sh.ngather = 0; }
       __syncthreads();
-      if (total > kcap && excl < kcap && excl + hv >= kcap) sh.bstar = tid;
+      if (total > kcap && excl < kcap && excl + hv >= kcap) sh.bstar = tid0;
       __syncthreads();
       const int bstar = sh.bstar;
       // gather (key = bin, value = candidate index) of survivors in buckets <= bstar
       uint2* gat = reinterpret_cast<uint2*>(lds);
-      constexpr int GMAX = C::LDS / 2 > 4096 ? 4096 : C::LDS / 2;
+      constexpr int GMAX = C::LDS > 4096 ? 4096 : C::LDS;
       for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
         Cand cd[4];
 #pragma unroll
@@ -723,7 +714,7 @@ __global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
         const Cand cd = gp(cand)[gat[i].y];
         write_record(out + i, cd.k, make_float2(cd.re, cd.im), cd.ml, cd.mc, cd.mr, a);
       }
-      if (tid == 0) {
+      if (tid0 == 0) {
         if (a.counts) {
           gp(a.counts)[2 * ch + 0] = (uint32_t)nout;
           gp(a.counts)[2 * ch + 1] = (uint32_t)total;
@@ -752,23 +743,23 @@ static hipError_t launch_small(const FusedArgs& a, int grid, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int N1, int MODE>
+template <int N2, int MODE>
 static hipError_t launch_large_mode(const FusedArgs& a, int grid, hipStream_t st) {
-  const size_t smem = sizeof(LargeShared<N1>);
+  const size_t smem = sizeof(LargeShared<N2>);
   static bool attr_done = false;  // the attribute is per kernel function
   if (!attr_done) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_fused_large<N1, MODE>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_fused_large<N2, MODE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
     attr_done = true;
   }
-  hipLaunchKernelGGL((k_fused_large<N1, MODE>), dim3(grid), dim3(1024), smem, st, a);
+  hipLaunchKernelGGL((k_fused_large<N2, MODE>), dim3(grid), dim3(1024), smem, st, a);
   return hipGetLastError();
 }
 
-template <int N1>
+template <int N2>
 static hipError_t launch_large(const FusedArgs& a, int grid, hipStream_t st) {
-  return a.peak_mode != 0 ? launch_large_mode<N1, 1>(a, grid, st) : launch_large_mode<N1, 0>(a, grid, st);
+  return a.peak_mode != 0 ? launch_large_mode<N2, 1>(a, grid, st) : launch_large_mode<N2, 0>(a, grid, st);
 }
 
 }  // namespace spx
@@ -787,6 +778,18 @@ uint64_t spx_large_slot_bytes(uint32_t nfft) {
 }
 
 int spx_fused_is_large(uint32_t nfft) { return nfft >= (1u << 15) ? 1 : 0; }
+
+// concatenated per-pass twiddle table length (float2 entries) for length L
+int spx_twiddle_table_len(uint32_t L) {
+  using namespace spx;
+  switch (L) {
+#define SPX_TWL(x) case x: return Plan<x>::TWSIZE;
+    SPX_TWL(2) SPX_TWL(4) SPX_TWL(8) SPX_TWL(16) SPX_TWL(32) SPX_TWL(64) SPX_TWL(128) SPX_TWL(256) SPX_TWL(512)
+    SPX_TWL(1024) SPX_TWL(2048) SPX_TWL(4096) SPX_TWL(8192) SPX_TWL(16384)
+#undef SPX_TWL
+    default: return -1;
+  }
+}
 
 hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st) {
   using namespace spx;

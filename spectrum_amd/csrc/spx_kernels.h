@@ -32,9 +32,9 @@ struct FusedArgs {
   spx_max_value* gmax;     // [batch]
   float* stats;            // [batch][4] {mean, std, threshold, mean |X|^2}
   // tables
-  const float2* tw_a;      // small: W_N^i ; large: W_N1^i
-  const float2* tw_b;      // large: W_256^i
-  const float2* tw_4;      // large: W_N^(n1*k2) at [k2*N1 + n1]
+  const float2* tw_a;      // per-pass twiddle tables: small path length N, large path length 256
+  const float2* tw_b;      // large path: per-pass tables of the column length N2 = N/256
+  const float2* tw_4;      // large path: W_N^(n1*k2) at [k2*256 + n1]
   void* scratch;           // large: per-workgroup slots
   uint64_t slot_bytes;
   const void* mlp;         // learned refiner weights (refine == 2)
@@ -44,6 +44,7 @@ extern "C" {
 hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st);
 uint64_t spx_large_slot_bytes(uint32_t nfft);
 int spx_fused_is_large(uint32_t nfft);
+int spx_twiddle_table_len(uint32_t L);
 hipError_t spx_launch_mag_phase(const void* in, void* out, uint64_t total, hipStream_t st);
 hipError_t spx_launch_magnitude(const void* in, float* out, uint64_t total, float inv_n, int squared,
                                 hipStream_t st);
