@@ -51,31 +51,27 @@ bool is_pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
 
 constexpr uint32_t kMaxN = 1u << 18;
 
-// LDS twiddle pack of a length-L transform (radix-16 passes first, the
-// leftover radix RL last; pass p has sub-length 16^p): digit tables
-// G(q,R)[d][r] = W_(16^q * R)^(r*d), d < 16, r < R, for R = 16 (q = 1..Q16)
-// and, when RL != 16, R = RL (q = 1..NP-1).  FP64, rounded once.
+// Per-pass twiddle tables of a length-L transform (radix-16 passes first, the
+// leftover radix last; pass p has sub-length NS = 16^p): pass p >= 1 stores
+// [k][r] = W_(NS*R)^(r*k), k < NS, r < R, passes concatenated.  FP64, rounded once.
 std::vector<float> make_twiddles(uint32_t L) {
   int log = 0;
   while ((1u << log) < L) ++log;
   const int np = (log + 3) / 4;
   const int rl = (log % 4) ? (1 << (log % 4)) : 16;
-  const int q16 = (rl == 16) ? np - 1 : std::max(np - 2, 0);
-  const int qrl = (rl == 16) ? 0 : np - 1;
   std::vector<float> t;
-  auto table = [&](int q, int R) {
-    const uint64_t m = (uint64_t)1 << (4 * q);
-    const uint64_t M = m * (uint64_t)R;
-    for (int d = 0; d < 16; ++d)
+  uint64_t ns = 16;
+  for (int p = 1; p < np; ++p, ns *= 16) {
+    const int R = (p == np - 1) ? rl : 16;
+    for (uint64_t k = 0; k < ns; ++k)
       for (int r = 0; r < R; ++r) {
-        const uint64_t e = ((uint64_t)r * (uint64_t)d) % M;
-        const double a = 2.0 * M_PI * ((double)e / (double)M);
+        const uint64_t m = ns * (uint64_t)R;
+        const uint64_t e = (k * (uint64_t)r) % m;
+        const double a = 2.0 * M_PI * ((double)e / (double)m);
         t.push_back((float)std::cos(a));
         t.push_back((float)-std::sin(a));
       }
-  };
-  for (int q = 1; q <= q16; ++q) table(q, 16);
-  for (int q = 1; q <= qrl; ++q) table(q, rl);
+  }
   if (t.empty()) t.assign(2, 0.0f);
   return t;
 }

@@ -149,8 +149,11 @@ template <> struct Dft<16> {
 // ---------------------------------------------------------------------------
 // Radix plan for a power-of-two length L = 2^a: radix-16 passes first, the
 // leftover radix (2, 4 or 8) last; passes = ceil(a/4).  With this order every
-// LDS access of every pass is base + compile-time stride (see the layouts),
-// and pass p has sub-transform length NS = 16^p.
+// LDS access of every pass is base + compile-time stride (see the layouts).
+// Twiddles: pass p >= 1 (sub-length NS = 16^p, radix R) reads a table laid
+// out [k][r] = W_(NS*R)^(r*k), k < NS, r < R, so one thread's R twiddles are
+// contiguous (R/2 16-byte loads from one address).  The tables of all passes
+// are concatenated; twoff(p) is the offset of pass p in float2 units.
 // ---------------------------------------------------------------------------
 template <int L>
 struct Plan {
@@ -159,27 +162,8 @@ struct Plan {
   static constexpr int RL = (LOG % 4) ? (1 << (LOG % 4)) : 16;  // last-pass radix
   static constexpr int radix(int p) { return p == NP - 1 ? RL : 16; }
   static constexpr int ns(int p) { return ipow16(p); }
-};
-
-// ---------------------------------------------------------------------------
-// LDS twiddle "digit tables".  Pass p (NS = 16^p, radix R) needs
-// W_(NS*R)^(r*k) for k < NS.  Writing k in base 16, k = sum_i d_i 16^i,
-//   W_(NS*R)^(r*k) = prod_{i<p} G(p-i, R)[d_i][r],  G(q, R)[d][r] = W_(16^q*R)^(r*d),
-// so p rows of 16 x R tables (products of p factors) replace an NS*R-entry
-// table: a few KB that fit in LDS next to the data.  Layout of the pack:
-// G(q,16) for q = 1..Q16 at (q-1)*256, then G(q,RL) for q = 1..NP-1 when the
-// last radix RL differs from 16.
-// ---------------------------------------------------------------------------
-template <int L>
-struct TwPack {
-  using PL = Plan<L>;
-  static constexpr int NP = PL::NP;
-  static constexpr int RL = PL::RL;
-  static constexpr int Q16 = (RL == 16) ? (NP - 1) : (NP - 2 > 0 ? NP - 2 : 0);
-  static constexpr int QRL = (RL == 16) ? 0 : (NP - 1);
-  static constexpr int SIZE_RAW = Q16 * 256 + QRL * 16 * RL;
-  static constexpr int SIZE = SIZE_RAW > 0 ? SIZE_RAW : 1;
-  static constexpr int off(int q, int R) { return (R == 16) ? (q - 1) * 256 : Q16 * 256 + (q - 1) * 16 * R; }
+  static constexpr int twoff(int p) { return p <= 1 ? 0 : twoff(p - 1) + ns(p - 1) * radix(p - 1); }
+  static constexpr int TWSIZE = NP <= 1 ? 1 : twoff(NP - 1) + ns(NP - 1) * radix(NP - 1);
 };
 
 // ---------------------------------------------------------------------------
@@ -222,29 +206,7 @@ struct LayInter {
   }
 };
 
-// R contiguous twiddles (one row of an LDS digit table), 16-byte LDS reads.
-template <int R>
-__device__ __forceinline__ void lds_tw_row(const float2* row, float2* w) {
-  if constexpr (R == 2) {
-    w[0] = make_float2(1.f, 0.f);
-    w[1] = row[1];
-  } else {
-    const float4* q = reinterpret_cast<const float4*>(row);
-#pragma unroll
-    for (int i = 0; i < R / 2; ++i) {
-      const float4 t = q[i];
-      w[2 * i] = make_float2(t.x, t.y);
-      w[2 * i + 1] = make_float2(t.z, t.w);
-    }
-  }
-}
-
-// Copy a twiddle pack (global, float2[n]) into LDS; caller synchronises.
-__device__ __forceinline__ void load_tw_pack(float2* dst, const float2* src, int n, int tid, int nthreads) {
-  for (int i = tid; i < n; i += nthreads) dst[i] = gp(src)[i];
-}
-
-// R contiguous twiddles (a row of a [k][r] table) with 16-byte global loads.
+// R contiguous twiddles (a row of a [k][r] table) with 16-byte loads.
 template <int R>
 __device__ __forceinline__ void load_tw_row(const float2* row, float2* w) {
   if constexpr (R == 2) {
@@ -266,7 +228,7 @@ __device__ __forceinline__ void load_tw_row(const float2* row, float2* w) {
 //  in : v[b*R + r] = x[s][jj + r*L/R]   (loaded from LDS unless FIRST)
 //  out: LDS y[s][(jj/NS)*NS*R + jj%NS + r*NS]           (unless LAST)
 //       or, for LAST, v[b*R + r] = X[s][jj + r*L/R] in registers.
-// Twiddles W_(NS*R)^(r*(jj%NS)) from the LDS digit tables twp (TwPack<L>).
+// Twiddles W_(NS*R)^(r*(jj%NS)) from the pass table twp[k*R + r].
 template <class LAY, int L, int NSEQ, int R, int NS, int T, int PPT, bool FIRST, bool LAST>
 __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[PPT],
                                          const float2* __restrict__ twp, int tid) {
@@ -301,15 +263,10 @@ __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[P
         int s, jj;
         LAY::template map<R>(j, s, jj);
         const int k = jj % NS;
-        constexpr int P = ilog2(NS) / 4;
-        // apply the P digit factors one row at a time (keeps register use low)
+        float2 w[R];
+        load_tw_row<R>(twp + (uint32_t)(k * R), w);
 #pragma unroll
-        for (int i = 0; i < P; ++i) {
-          float2 w[R];
-          lds_tw_row<R>(twp + TwPack<L>::off(P - i, R) + ((k >> (4 * i)) & 15) * R, w);
-#pragma unroll
-          for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], w[r]);
-        }
+        for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], w[r]);
       }
       Dft<R>::run(&v[b * R]);
     }
@@ -341,7 +298,8 @@ template <class LAY, int L, int NSEQ, int T, int PPT, int P>
 struct FftPasses {
   __device__ __forceinline__ static void run(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
     using PL = Plan<L>;
-    fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1>(lds, v, tw, tid);
+    fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1>(lds, v, tw + PL::twoff(P),
+                                                                                       tid);
     if constexpr (P + 1 < PL::NP) FftPasses<LAY, L, NSEQ, T, PPT, P + 1>::run(lds, v, tw, tid);
   }
 };
@@ -349,7 +307,7 @@ struct FftPasses {
 // Full length-L transform of the tile.  Before the call v holds the first
 // pass's inputs (x[s][jj + r*L/R0], R0 = Plan<L>::radix(0), for the mapping of
 // j = tid + b*T); after it v holds X[s][jj + r*L/RL] for the radix-RL mapping.
-// tw: the LDS twiddle pack of length L (TwPack<L>), already in LDS.
+// tw: the concatenated per-pass twiddle tables of length L (Plan<L>::TWSIZE).
 template <class LAY, int L, int NSEQ, int T, int PPT>
 __device__ __forceinline__ void fft_tile(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
   FftPasses<LAY, L, NSEQ, T, PPT, 0>::run(lds, v, tw, tid);

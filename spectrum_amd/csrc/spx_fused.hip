@@ -238,10 +238,8 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
   constexpr int T = C::T, PPT = C::PPT;
   constexpr int R0 = PL::radix(0), RL = PL::RL;
   __shared__ float2 lds[LAY::SIZE];
-  __shared__ __align__(16) float2 twl[TwPack<L>::SIZE];
   __shared__ BlockRed red;
   const int tid = threadIdx.x;
-  load_tw_pack(twl, a.tw_a, TwPack<L>::SIZE_RAW, tid, T);  // visible after the first pass barrier
 
   // one channel per workgroup: a grid-stride loop here would let the compiler
   // hoist every (channel-invariant) twiddle and window load out of the loop
@@ -258,8 +256,7 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
         load_run<R0>(a, ch, (uint32_t)j, (uint32_t)(L / R0), &v[b * R0]);
       }
     }
-    if constexpr (Plan<L>::NP == 1) __syncthreads();
-    fft_tile<LAY, L, 1, T, PPT>(lds, v, twl, tid);
+    fft_tile<LAY, L, 1, T, PPT>(lds, v, a.tw_a, tid);
     // v[b*RL + r] = X[j + r*L/RL], j = tid + b*T < L/RL
 
     const uint64_t cbase = (uint64_t)ch * L;
@@ -394,11 +391,11 @@ struct LargeCfg {
   static constexpr int N1 = 256;                // phase-2 row length (radix 16 x 16)
   static constexpr int N2 = N2_;                // phase-1 column length
   static constexpr int N = N1 * N2;
-  static constexpr int T = 512;
+  static constexpr int T = 1024;
   static constexpr int PPT = 16;
-  static constexpr int TILE = T * PPT;          // 8192 points per tile
+  static constexpr int TILE = T * PPT;          // 16384 points per tile
   static constexpr int CB = TILE / N2;          // phase-1 columns per tile
-  static constexpr int RB = TILE / N1;          // phase-2 rows per tile (32)
+  static constexpr int RB = TILE / N1;          // phase-2 rows per tile (64)
   static constexpr int NT1 = N1 / CB;           // phase-1 tiles
   static constexpr int NT2 = N2 / RB;           // phase-2 tiles
   static constexpr int RL1 = Plan<N2>::RL;      // last radix of the column transform
@@ -412,9 +409,9 @@ struct LargeCfg {
   static constexpr uint64_t EDGE_M_BYTES = (uint64_t)NT2 * 4 * N1 * 4;
   static constexpr uint64_t EDGE_X_BYTES = (uint64_t)NT2 * 2 * N1 * 8;
   static constexpr uint64_t SLOT_BYTES = Z_BYTES + CAND_BYTES + EDGE_M_BYTES + EDGE_X_BYTES;
-  static constexpr int NBK = T;                 // selection buckets (one per thread)
-  static constexpr int BSH = ilog2(N) - ilog2(NBK);  // bucket = k >> BSH
-  static_assert(NT1 >= 1 && NT2 >= 1 && CB >= 8 && RB >= 4, "tile shape");
+  static constexpr int NBK = 1024;              // selection buckets
+  static constexpr int BSH = ilog2(N) - 10;     // bucket = k >> BSH
+  static_assert(NT1 >= 1 && NT2 >= 1 && CB >= 16, "tile shape");
   static_assert(Plan<N1>::RL == 16, "rows end on a radix-16 pass");
 };
 
@@ -422,8 +419,6 @@ template <int N2>
 struct LargeShared {
   using C = LargeCfg<N2>;
   float2 tile[C::LDS];
-  float2 tw_rows[TwPack<256>::SIZE];   // 256-point row transform
-  float2 tw_cols[TwPack<N2>::SIZE];    // N2-point column transform
   BlockRed red;
   int hist[C::NBK];
   uint32_t ncand;
@@ -433,7 +428,7 @@ struct LargeShared {
 
 // MODE 0: dense outputs only (spectrum / |X| / |X|^2 / mag+phase); MODE 1: peaks only.
 template <int N2, int MODE>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_fused_large(FusedArgs a) {
+__global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
   using C = LargeCfg<N2>;
   constexpr int N = C::N, N1 = C::N1, T = C::T, PPT = C::PPT, CB = C::CB, RB = C::RB;
   constexpr int NT1 = C::NT1, NT2 = C::NT2, RL1 = C::RL1;
@@ -454,9 +449,6 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
 
   constexpr bool peaks = (MODE == 1);
   const bool full_band = (a.s0 == 0 && a.s1 == (uint32_t)N);
-  load_tw_pack(sh.tw_rows, a.tw_a, TwPack<N1>::SIZE_RAW, tid0, T);
-  load_tw_pack(sh.tw_cols, a.tw_b, TwPack<N2>::SIZE_RAW, tid0, T);
-  __syncthreads();
 
   for (uint32_t ch = blockIdx.x; ch < a.batch; ch += gridDim.x) {
     int tid = opaque_v(tid0);
@@ -468,6 +460,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll 1
     for (int t = 0; t < NT1; ++t) {
       tid = opaque_v(tid0);
+      ai.tw_b = opaque(a.tw_b);
       ai.tw_4 = opaque(a.tw_4);
       ai.window = opaque(a.window);
       const int c = tid % CB, jj = tid / CB;  // first pass is radix 16: jj in [0, N2/16)
@@ -476,7 +469,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
       load_run<16>(ai, ch, (uint32_t)(n1 + N1 * jj), (uint32_t)(N1 * (N2 / 16)), v);
 #pragma unroll
       for (int r = 0; r < 16; ++r) energy = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, energy));
-      fft_tile<LAY1, N2, CB, T, PPT>(lds, v, sh.tw_cols, tid);
+      fft_tile<LAY1, N2, CB, T, PPT>(lds, v, ai.tw_b, tid);
       // v[b*RL1 + r] = Y[column s][k2 = jj' + r*N2/RL1] for j = tid + b*T
 #pragma unroll
       for (int b = 0; b < PPT / RL1; ++b) {
@@ -517,6 +510,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll 1
     for (int t = 0; t < NT2; ++t) {
       tid = opaque_v(tid0);
+      ai.tw_a = opaque(a.tw_a);
       const int s_row = tid / (N1 / 16), jj_row = tid % (N1 / 16);  // radix-16 mapping of both passes
       float2 v[PPT];
       {
@@ -524,7 +518,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
         for (int r = 0; r < 16; ++r) v[r] = gp(zr)[r * (N1 / 16)];
       }
-      fft_tile<LAY2, N1, RB, T, PPT>(lds, v, sh.tw_rows, tid);
+      fft_tile<LAY2, N1, RB, T, PPT>(lds, v, ai.tw_a, tid);
       // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]  ->  bin k2 + N2*k1
       const int k2 = t * RB + s_row;
       float m[16];
@@ -759,7 +753,7 @@ static hipError_t launch_large_mode(const FusedArgs& a, int grid, hipStream_t st
     if (e != hipSuccess) return e;
     attr_done = true;
   }
-  hipLaunchKernelGGL((k_fused_large<N2, MODE>), dim3(grid), dim3(LargeCfg<N2>::T), smem, st, a);
+  hipLaunchKernelGGL((k_fused_large<N2, MODE>), dim3(grid), dim3(1024), smem, st, a);
   return hipGetLastError();
 }
 
@@ -785,11 +779,11 @@ uint64_t spx_large_slot_bytes(uint32_t nfft) {
 
 int spx_fused_is_large(uint32_t nfft) { return nfft >= (1u << 15) ? 1 : 0; }
 
-// LDS twiddle pack length (float2 entries) for length L
+// concatenated per-pass twiddle table length (float2 entries) for length L
 int spx_twiddle_table_len(uint32_t L) {
   using namespace spx;
   switch (L) {
-#define SPX_TWL(x) case x: return TwPack<x>::SIZE;
+#define SPX_TWL(x) case x: return Plan<x>::TWSIZE;
     SPX_TWL(2) SPX_TWL(4) SPX_TWL(8) SPX_TWL(16) SPX_TWL(32) SPX_TWL(64) SPX_TWL(128) SPX_TWL(256) SPX_TWL(512)
     SPX_TWL(1024) SPX_TWL(2048) SPX_TWL(4096) SPX_TWL(8192) SPX_TWL(16384)
 #undef SPX_TWL
