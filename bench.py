@@ -90,6 +90,9 @@ def main():
     ap.add_argument("--nfft", type=int, default=65536)
     ap.add_argument("--kpeaks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after timing, copy the input once device-to-device (a plain copy of known size, "
+                         "used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported (channels shard "
@@ -133,6 +136,13 @@ def main():
     tf = os.path.join(ROOT, "profiles", "fused_large_n65536_b1024_traffic.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    if args.calibrate_copy:
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        d_cp = ctx.alloc(x.nbytes)
+        assert hip.hipMemcpy(d_cp.ptr, d_in.ptr, x.nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+        ctx.sync()
+        d_cp.free()
     ct = d_ct.download(np.uint32, (B, 2))
     out = {
         "metric": METRIC,
