@@ -150,6 +150,51 @@ typedef struct spx_fused_desc {
 
 int spx_fused_peaks(spx_ctx* ctx, const spx_fused_desc* desc);
 
+/* ---- uneven-length batch: one persistent launch, captured in a hipGraph ----
+ * No counterpart in the reference, which builds one hipFFT plan per (nFFT,
+ * beam_count) and runs lengths one after another
+ * (src/fft_func/src/spectrum_processor_rocm.cpp:1129-1150).  Channels of
+ * lengths 1024 .. 131072 (powers of two, any mix, any order) are searched with
+ * the SPX_PEAKS_THRESHOLD rules of spx_fused_peaks (whole band, nfft = length)
+ * in one launch of a persistent kernel over a cost-sorted work list.  The
+ * radix order per length comes from an autotuner cache (JSON, written by
+ * spx_batch_autotune); without one every length uses the default order.
+ * A plan binds its buffers: execute re-runs the captured graph on whatever
+ * the input buffer holds at that time. */
+typedef struct spx_batch_plan spx_batch_plan;
+
+#define SPX_BATCH_NO_GRAPH 1u   /* launch the kernels directly on every execute */
+
+typedef struct spx_batch_desc {
+  const void* input;          /* device: packed channels (dtype) */
+  const uint64_t* offsets;    /* host [n_chan]: element offset of channel c in input */
+  const uint32_t* lengths;    /* host [n_chan]: length of channel c (power of two, 1024..131072) */
+  uint32_t n_chan;
+  uint32_t window;            /* SPX_WINDOW_* (symmetric over the channel's length) */
+  uint32_t dtype;             /* SPX_DTYPE_* */
+  float sample_rate;
+  float nsigma;
+  uint32_t max_peaks;         /* <= 2048 */
+  uint32_t refine;            /* SPX_REFINE_NONE or SPX_REFINE_PARABOLIC */
+  spx_max_value* peaks_out;   /* device [n_chan][max_peaks] */
+  uint32_t* counts_out;       /* device [n_chan][2] = {stored, found} */
+  spx_max_value* gmax_out;    /* optional device [n_chan] */
+  float* stats_out;           /* optional device [n_chan][4] */
+  const char* tune_cache;     /* optional path of the autotuner's JSON cache */
+  uint32_t flags;             /* SPX_BATCH_* */
+} spx_batch_desc;
+
+int spx_batch_plan_create(spx_ctx* ctx, const spx_batch_desc* desc, spx_batch_plan** out);
+int spx_batch_execute(spx_batch_plan* plan);
+int spx_batch_plan_destroy(spx_batch_plan* plan);
+/* JSON description of the plan (items per length, radix order, grid, scratch). */
+int spx_batch_plan_info(spx_batch_plan* plan, char* buf, size_t len);
+/* Times every radix order of every length present in desc (on desc's own
+ * buffers, each length alone), keeps the fastest and writes the choices with
+ * the measured times to cache_path as JSON.  Lengths whose transforms have a
+ * single radix order are recorded with order 0. */
+int spx_batch_autotune(spx_ctx* ctx, const spx_batch_desc* desc, const char* cache_path);
+
 /* ---- synthetic signals (host side, deterministic) --------------------------
  * Sum of complex tones plus complex white Gaussian noise, generated in FP64
  * with counter-based SplitMix64 streams keyed by (seed, channel) and rounded
@@ -162,6 +207,12 @@ int spx_fused_peaks(spx_ctx* ctx, const spx_fused_desc* desc);
 #define SPX_GEN_FRACTIONAL 4    /* 1-16 tones at (k+u)/N, A~U[.5,2], >= 3 bins apart, noise 20 dB below weakest */
 int spx_siggen(uint32_t kind, uint64_t seed, uint32_t n_chan, uint32_t n_point, uint32_t chan_offset,
                void* out_c64, uint32_t max_tones, double* tone_freq, double* tone_amp, int threads);
+/* Per-channel lengths 2^(lo_log2 + U{0 .. hi_log2-lo_log2}) from the (seed, channel) stream. */
+int spx_siggen_lengths(uint64_t seed, uint32_t n_chan, uint32_t lo_log2, uint32_t hi_log2, uint32_t* lengths_out);
+/* spx_siggen for channels of individual lengths: channel c (generator channel
+ * index c) is written at out_c64 + offsets[c] complex elements. */
+int spx_siggen_uneven(uint32_t kind, uint64_t seed, uint32_t n_chan, const uint32_t* lengths,
+                      const uint64_t* offsets, void* out_c64, int threads);
 
 /* ---- version ---- */
 const char* spx_version(void);

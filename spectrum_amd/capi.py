@@ -46,6 +46,18 @@ class SpxFusedDesc(C.Structure):
     ]
 
 
+class SpxBatchDesc(C.Structure):
+    _fields_ = [
+        ("input", C.c_void_p), ("offsets", C.c_void_p), ("lengths", C.c_void_p),
+        ("n_chan", C.c_uint32), ("window", C.c_uint32), ("dtype", C.c_uint32),
+        ("sample_rate", C.c_float), ("nsigma", C.c_float), ("max_peaks", C.c_uint32), ("refine", C.c_uint32),
+        ("peaks_out", C.c_void_p), ("counts_out", C.c_void_p), ("gmax_out", C.c_void_p),
+        ("stats_out", C.c_void_p), ("tune_cache", C.c_char_p), ("flags", C.c_uint32),
+    ]
+
+
+BATCH_NO_GRAPH = 1
+
 # (name, restype, argtypes) for every symbol declared in include/spectrum_mi355x.h
 _P, _U32, _U64, _I, _F = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float
 SIGNATURES = [
@@ -68,7 +80,14 @@ SIGNATURES = [
     ("spx_peaks_top", _I, [_P, _P, _P, _U32, _U32, _U32, _F, _I]),
     ("spx_all_maxima", _I, [_P, _P, _P, _P, _U32, _U32, _F, _U32, _U32, _U32]),
     ("spx_fused_peaks", _I, [_P, C.POINTER(SpxFusedDesc)]),
+    ("spx_batch_plan_create", _I, [_P, C.POINTER(SpxBatchDesc), C.POINTER(C.c_void_p)]),
+    ("spx_batch_execute", _I, [_P]),
+    ("spx_batch_plan_destroy", _I, [_P]),
+    ("spx_batch_plan_info", _I, [_P, C.c_char_p, C.c_size_t]),
+    ("spx_batch_autotune", _I, [_P, C.POINTER(SpxBatchDesc), C.c_char_p]),
     ("spx_siggen", _I, [_U32, _U64, _U32, _U32, _U32, _P, _U32, _P, _P, _I]),
+    ("spx_siggen_lengths", _I, [_U64, _U32, _U32, _U32, _P]),
+    ("spx_siggen_uneven", _I, [_U32, _U64, _U32, _P, _P, _P, _I]),
     ("spx_version", C.c_char_p, []),
 ]
 
@@ -215,6 +234,66 @@ class Context:
         self.check(self.lib.spx_fused_peaks(self.h, C.byref(d)), "spx_fused_peaks")
 
 
+    # -- uneven-length batch
+    def batch_desc(self, inp, offsets: np.ndarray, lengths: np.ndarray, peaks, counts, *, window=1,
+                   dtype=DTYPE_C64, fs=1.0, nsigma=6.0, max_peaks=64, refine=REFINE_PARABOLIC, gmax=None,
+                   stats=None, tune_cache: Optional[str] = None, graph=True):
+        """Descriptor for spx_batch_*; keeps the host arrays alive on the returned object."""
+        d = SpxBatchDesc()
+        d._offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        d._lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+        d.input = _ptr(inp)
+        d.offsets = d._offsets.ctypes.data
+        d.lengths = d._lengths.ctypes.data
+        d.n_chan = len(d._lengths)
+        d.window, d.dtype, d.sample_rate = window, dtype, float(fs)
+        d.nsigma, d.max_peaks, d.refine = float(nsigma), max_peaks, refine
+        d.peaks_out, d.counts_out = _ptr(peaks), _ptr(counts)
+        d.gmax_out, d.stats_out = _ptr(gmax), _ptr(stats)
+        d._cache = tune_cache.encode() if tune_cache else None
+        d.tune_cache = d._cache
+        d.flags = 0 if graph else BATCH_NO_GRAPH
+        return d
+
+    def batch_plan(self, desc: SpxBatchDesc) -> "BatchPlan":
+        return BatchPlan(self, desc)
+
+    def batch_autotune(self, desc: SpxBatchDesc, cache_path: str):
+        self.check(self.lib.spx_batch_autotune(self.h, C.byref(desc), cache_path.encode()), "spx_batch_autotune")
+
+
+class BatchPlan:
+    """spx_batch_plan: work list + scratch + captured graph for one uneven batch."""
+
+    def __init__(self, ctx: Context, desc: SpxBatchDesc):
+        self.ctx = ctx
+        self.desc = desc
+        h = C.c_void_p()
+        ctx.check(ctx.lib.spx_batch_plan_create(ctx.h, C.byref(desc), C.byref(h)), "spx_batch_plan_create")
+        self.h = h
+
+    def execute(self):
+        self.ctx.check(self.ctx.lib.spx_batch_execute(self.h), "spx_batch_execute")
+
+    def info(self) -> dict:
+        import json
+
+        buf = C.create_string_buffer(8192)
+        self.ctx.check(self.ctx.lib.spx_batch_plan_info(self.h, buf, 8192), "spx_batch_plan_info")
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.spx_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceBuffer:
     """Caller-owned device allocation made through spx_alloc (freed on close/GC)."""
 
@@ -263,3 +342,43 @@ def siggen(kind: int, seed: int, n_chan: int, n_point: int, chan_offset: int = 0
     if rc != SPX_OK:
         raise ValueError(f"spx_siggen failed with status {rc}")
     return (out, tf, ta) if with_tones else out
+
+
+def siggen_lengths(seed: int, n_chan: int, lo_log2: int = 10, hi_log2: int = 17) -> np.ndarray:
+    """Per-channel power-of-two lengths 2^(lo + U{0..hi-lo}) (spx_siggen_lengths)."""
+    out = np.empty(n_chan, dtype=np.uint32)
+    rc = load().spx_siggen_lengths(seed, n_chan, lo_log2, hi_log2, out.ctypes.data)
+    if rc != SPX_OK:
+        raise ValueError(f"spx_siggen_lengths failed with status {rc}")
+    return out
+
+
+def length_major_offsets(lengths: np.ndarray):
+    """Offsets that pack channels grouped by length (ascending length, then channel id).
+
+    Returns (offsets[n_chan] in complex elements, total elements, {length: (first offset, channel ids)})."""
+    lengths = np.asarray(lengths, dtype=np.uint64)
+    order = np.lexsort((np.arange(len(lengths)), lengths))
+    offsets = np.empty(len(lengths), dtype=np.uint64)
+    pos = 0
+    groups = {}
+    for c in order:
+        L = int(lengths[c])
+        if L not in groups:
+            groups[L] = (pos, [])
+        groups[L][1].append(int(c))
+        offsets[c] = pos
+        pos += L
+    return offsets, pos, {L: (o, np.array(ids, dtype=np.int64)) for L, (o, ids) in groups.items()}
+
+
+def siggen_uneven(kind: int, seed: int, lengths: np.ndarray, offsets: np.ndarray, total: int, threads: int = 0):
+    """Channels of individual lengths packed at `offsets` (spx_siggen_uneven); complex64 [total]."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    out = np.empty(int(total), dtype=np.complex64)
+    rc = load().spx_siggen_uneven(kind, seed, len(lengths), lengths.ctypes.data, offsets.ctypes.data,
+                                  out.ctypes.data, threads)
+    if rc != SPX_OK:
+        raise ValueError(f"spx_siggen_uneven failed with status {rc}")
+    return out

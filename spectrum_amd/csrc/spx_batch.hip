@@ -1,0 +1,303 @@
+// Persistent uneven-length batch: every channel of a batch whose lengths range
+// over 2^10 .. 2^17 runs through the fused window -> FFT -> |X| -> threshold
+// peak search in ONE launch (and, through the host plan, one hipGraph node).
+//
+// The host builds a work list sorted by decreasing cost (four-step channels of
+// the longest length first, LDS-resident tiles of the shortest last: a
+// longest-processing-time-first schedule) and the grid is one 1024-thread
+// workgroup per CU.  Each workgroup takes the next item from a global cursor
+// (one relaxed atomic per item, no other cross-workgroup traffic):
+//   * lengths 1024 .. 16384: a tile of G = 16384/L whole channels in LDS,
+//     the same Stockham passes as the single-channel kernel but over G
+//     sequences, then per-channel (segmented) statistics, peak flags and
+//     scan, so every channel's records equal the single-channel kernel's;
+//   * lengths 32768 .. 131072: the four-step channel body of the fused large
+//     kernel, in this workgroup's scratch slot.
+// The radix order of each length (Plan<L, V>) is chosen per bucket by the
+// host autotuner and carried in the item word.
+#include "spx_fused_common.hpp"
+
+namespace spx {
+
+template <int DT, int R>
+__device__ __forceinline__ void load_pts_t(const void* cb, const float* win, uint32_t n0, uint32_t stride, float2* v) {
+  float2 raw[R];
+  float wv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) raw[r] = ld_raw<DT>(cb, n0 + r * stride);
+  if (win != nullptr) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) wv[r] = gp(win)[n0 + r * stride];
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) wv[r] = 1.0f;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = c_scale(raw[r], wv[r]);
+}
+
+template <int R>
+__device__ __forceinline__ void load_pts(const void* cb, uint32_t dtype, const float* win, uint32_t n0, uint32_t stride,
+                                         float2* v) {
+  if (dtype == SPX_DTYPE_C64) load_pts_t<SPX_DTYPE_C64, R>(cb, win, n0, stride, v);
+  else if (dtype == SPX_DTYPE_C32F16) load_pts_t<SPX_DTYPE_C32F16, R>(cb, win, n0, stride, v);
+  else load_pts_t<SPX_DTYPE_C32BF16, R>(cb, win, n0, stride, v);
+}
+
+__device__ __forceinline__ const void* chan_base(const BatchArgs& b, uint32_t ch) {
+  const uint64_t es = b.dtype == SPX_DTYPE_C64 ? 8 : 4;
+  return reinterpret_cast<const char*>(b.in) + gp(b.offsets)[ch] * es;
+}
+
+// G = 16384/L channels of length L, 1024 threads, 16 points per thread.
+template <int L, int V>
+__device__ __forceinline__ void batch_small_tile(const BatchArgs& b, const uint4 it, float2* lds, BlockRed& red) {
+  using PL = Plan<L, V>;
+  constexpr int T = 1024, PPT = 16, G = 16384 / L;
+  constexpr int TPC = L / 16;      // threads per channel in the peak stage (16 bins each)
+  constexpr int WPC = TPC / 64;    // waves per channel
+  constexpr int R0 = PL::R0, RL = PL::RL;
+  using LAY = LayContig<L, G>;
+  static_assert(WPC >= 1 && G * L == T * PPT, "tile shape");
+  const int tid = opaque_v((int)threadIdx.x);
+  const uint32_t bucket = it.x & 0xffu;
+  const float2* tw = opaque(b.bk[bucket].tw);
+  const float* win = opaque(b.bk[bucket].window);
+  const uint32_t first = it.y, count = it.z;
+
+  float2 v[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT / R0; ++q) {
+    const int j = tid + q * T;
+    const int s = j / (L / R0), jj = j % (L / R0);
+    if ((uint32_t)s < count) {
+      const uint32_t ch = gp(b.perm)[first + s];
+      load_pts<R0>(chan_base(b, ch), b.dtype, win, (uint32_t)jj, (uint32_t)(L / R0), &v[q * R0]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[q * R0 + r] = make_float2(0.f, 0.f);
+    }
+  }
+  fft_tile<LAY, L, G, T, PPT, V>(lds, v, tw, tid);
+  __syncthreads();  // the last pass read LDS
+#pragma unroll
+  for (int q = 0; q < PPT / RL; ++q) {
+    int s, jj;
+    LAY::template map<RL>(tid + q * T, s, jj);
+#pragma unroll
+    for (int r = 0; r < RL; ++r) lds[LAY::idx(s, jj + r * (L / RL))] = v[q * RL + r];
+  }
+  __syncthreads();
+
+  // peak stage: thread owns bins k0 .. k0+15 of channel slot s
+  const int s = tid / TPC;
+  const uint32_t k0 = (uint32_t)(tid % TPC) * 16u;
+  const bool active = (uint32_t)s < count;
+  const int w = tid >> 6, lane = tid & 63;
+  float m[16];
+  float fs1 = 0.f, fs2 = 0.f, bm = -1.0f;
+  uint32_t bk = 0xffffffffu;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const uint32_t k = k0 + e;
+    m[e] = mag_of(lds[LAY::idx(s, k)]);
+    fs1 += m[e];
+    fs2 = fmaf(m[e], m[e], fs2);
+    if (m[e] > bm) { bm = m[e]; bk = k; }
+  }
+  double s1 = wave_sum((double)fs1), s2 = wave_sum((double)fs2);
+  wave_argmax(bm, bk);
+  if (lane == 0) {
+    red.s1[w] = s1;
+    red.s2[w] = s2;
+    red.m[w] = bm;
+    red.k[w] = bk;
+  }
+  __syncthreads();
+  s1 = 0.0;
+  s2 = 0.0;
+  bm = -1.0f;
+  bk = 0xffffffffu;
+#pragma unroll
+  for (int i = 0; i < WPC; ++i) {
+    const int wi = s * WPC + i;
+    s1 += red.s1[wi];
+    s2 += red.s2[wi];
+    if (red.m[wi] > bm || (red.m[wi] == bm && red.k[wi] < bk)) {
+      bm = red.m[wi];
+      bk = red.k[wi];
+    }
+  }
+  __syncthreads();
+  const double mu = s1 / (double)L;
+  double var = s2 / (double)L - mu * mu;
+  if (var < 0) var = 0;
+  const double sd = sqrt(var);
+  const float thr = (float)(mu + (double)b.nsigma * sd);
+
+  const float ml = mag_of(lds[LAY::idx(s, (k0 + L - 1) % L)]);
+  const float mr = mag_of(lds[LAY::idx(s, (k0 + 16) % L)]);
+  uint32_t flags = 0;
+  int c = 0;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const float yl = (e == 0) ? ml : m[e - 1];
+    const float yr = (e == 15) ? mr : m[e + 1];
+    if (m[e] > yl && m[e] > yr && m[e] > thr) {
+      flags |= 1u << e;
+      ++c;
+    }
+  }
+  // segmented exclusive scan over the channel's waves
+  const int inc = wave_incl_scan(c);
+  if (lane == 63) red.c[w] = inc;
+  __syncthreads();
+  int off = 0, total = 0;
+#pragma unroll
+  for (int i = 0; i < WPC; ++i) {
+    const int wi = s * WPC + i;
+    if (wi < w) off += red.c[wi];
+    total += red.c[wi];
+  }
+  off += inc - c;
+  const float bin_hz = b.fs / (float)L;
+  if (active) {
+    const uint32_t ch = gp(b.perm)[first + s];
+    if (flags && off < (int)b.kcap) {
+      MaxValue* out = b.peaks + (uint64_t)ch * b.kcap;
+      int pos = off;
+      while (flags && pos < (int)b.kcap) {
+        const int e = __builtin_ctz(flags);
+        flags &= flags - 1;
+        const uint32_t k = k0 + e;
+        const float yc = mag_of(lds[LAY::idx(s, k)]);
+        const float yl = mag_of(lds[LAY::idx(s, (k + L - 1) % L)]);
+        const float yr = mag_of(lds[LAY::idx(s, (k + 1) % L)]);
+        write_record_hz(out + pos, k, lds[LAY::idx(s, k)], yl, yc, yr, b.refine, bin_hz);
+        ++pos;
+      }
+    }
+    if ((tid % TPC) == 0) {
+      if (b.counts) {
+        gp(b.counts)[2 * (uint64_t)ch + 0] = (uint32_t)min(total, (int)b.kcap);
+        gp(b.counts)[2 * (uint64_t)ch + 1] = (uint32_t)total;
+      }
+      if (b.stats) {
+        gp(b.stats)[4 * (uint64_t)ch + 0] = (float)mu;
+        gp(b.stats)[4 * (uint64_t)ch + 1] = (float)sd;
+        gp(b.stats)[4 * (uint64_t)ch + 2] = thr;
+        gp(b.stats)[4 * (uint64_t)ch + 3] = (float)(s2 / (double)L);
+      }
+      if (b.gmax && bk != 0xffffffffu) {
+        const float yl = mag_of(lds[LAY::idx(s, (bk + L - 1) % L)]);
+        const float yr = mag_of(lds[LAY::idx(s, (bk + 1) % L)]);
+        write_record_hz(b.gmax + ch, bk, lds[LAY::idx(s, bk)], yl, bm, yr, b.refine, bin_hz);
+      }
+    }
+  }
+  __syncthreads();  // LDS and red are reused by the next item
+}
+
+template <int N2, int V>
+__device__ __forceinline__ void batch_large_item(const BatchArgs& b, const uint4 it, unsigned char* slot,
+                                                 unsigned char* smem) {
+  constexpr uint32_t N = 256u * N2;
+  const uint32_t bucket = it.x & 0xffu;
+  const uint32_t ch = gp(b.perm)[it.y];
+  FusedArgs a;
+  a.in = chan_base(b, ch);
+  a.in_scale = nullptr;
+  a.batch = 1;
+  a.n_point = N;
+  a.nfft = N;
+  a.dtype = b.dtype;
+  a.window = b.bk[bucket].window;
+  a.fs = b.fs;
+  a.spec_out = nullptr;
+  a.mag_out = nullptr;
+  a.mag_kind = 0;
+  a.mag_scale = 1.0f;
+  a.peak_mode = 1;
+  a.nsigma = b.nsigma;
+  a.kcap = b.kcap;
+  a.s0 = 0;
+  a.s1 = N;
+  a.refine = b.refine;
+  a.peaks = b.peaks;
+  a.counts = b.counts;
+  a.gmax = b.gmax;
+  a.stats = b.stats;
+  a.tw_a = b.tw256;
+  a.tw_b = b.bk[bucket].tw;
+  a.tw_4 = b.bk[bucket].tw4;
+  a.scratch = b.scratch;
+  a.slot_bytes = b.slot_bytes;
+  a.mlp = nullptr;
+  large_channel<N2, 1, V>(a, 0, ch, slot, *reinterpret_cast<LargeShared<N2>*>(smem));
+}
+
+__global__ void k_batch_reset(uint32_t* counter) {
+  if (threadIdx.x == 0) gp(counter)[0] = 0u;
+}
+
+__global__ void __launch_bounds__(1024) k_batch(BatchArgs b) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ uint32_t item_s;
+  LargeShared<512>& sh = *reinterpret_cast<LargeShared<512>*>(smem_raw);
+  unsigned char* slot = reinterpret_cast<unsigned char*>(b.scratch) + (uint64_t)blockIdx.x * b.slot_bytes;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(b.counter, 1u);
+    __syncthreads();
+    const uint32_t i = __builtin_amdgcn_readfirstlane(item_s);
+    __syncthreads();
+    if (i >= b.n_items) break;  // uniform: every thread read the same i
+    uint4 it = gp(b.items)[i];
+    it.x = __builtin_amdgcn_readfirstlane(it.x);
+    it.y = __builtin_amdgcn_readfirstlane(it.y);
+    it.z = __builtin_amdgcn_readfirstlane(it.z);
+    switch (it.x & 0xffffu) {
+      case 0x000: batch_small_tile<1024, 0>(b, it, sh.tile, sh.red); break;
+      case 0x100: batch_small_tile<1024, 1>(b, it, sh.tile, sh.red); break;
+      case 0x001: batch_small_tile<2048, 0>(b, it, sh.tile, sh.red); break;
+      case 0x101: batch_small_tile<2048, 1>(b, it, sh.tile, sh.red); break;
+      case 0x002: batch_small_tile<4096, 0>(b, it, sh.tile, sh.red); break;
+      case 0x003: batch_small_tile<8192, 0>(b, it, sh.tile, sh.red); break;
+      case 0x103: batch_small_tile<8192, 1>(b, it, sh.tile, sh.red); break;
+      case 0x004: batch_small_tile<16384, 0>(b, it, sh.tile, sh.red); break;
+      case 0x104: batch_small_tile<16384, 1>(b, it, sh.tile, sh.red); break;
+      case 0x005: batch_large_item<128, 0>(b, it, slot, smem_raw); break;
+      case 0x105: batch_large_item<128, 1>(b, it, slot, smem_raw); break;
+      case 0x006: batch_large_item<256, 0>(b, it, slot, smem_raw); break;
+      case 0x007: batch_large_item<512, 0>(b, it, slot, smem_raw); break;
+      case 0x107: batch_large_item<512, 1>(b, it, slot, smem_raw); break;
+      default: break;  // the host validates every item word
+    }
+  }
+}
+
+}  // namespace spx
+
+extern "C" {
+
+// Dynamic LDS of the batch kernel: the largest per-item layout.
+static size_t batch_smem() { return sizeof(spx::LargeShared<512>); }
+
+// Allows the batch kernel its dynamic LDS; call once per process before the
+// first launch and outside any stream capture.
+hipError_t spx_batch_kernel_init(void) {
+  using namespace spx;
+  static_assert(sizeof(LargeShared<128>) == sizeof(LargeShared<512>), "one LDS layout for all items");
+  static_assert(sizeof(LargeShared<256>) == sizeof(LargeShared<512>), "one LDS layout for all items");
+  return hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)batch_smem());
+}
+
+hipError_t spx_launch_batch(const BatchArgs* b, int grid, hipStream_t st) {
+  using namespace spx;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(1024), batch_smem(), st, *b);
+  return hipGetLastError();
+}
+
+}  // extern "C"

@@ -15,22 +15,9 @@
 
 #include "spx_kernels.h"
 
-struct spx_ctx {
-  int device = 0;
-  hipStream_t own_stream = nullptr;
-  hipStream_t stream = nullptr;
-  int cu_count = 0;
-  std::string name;
-  uint64_t hbm = 0;
-  std::string err;
-  std::map<std::tuple<int, uint32_t, uint32_t>, void*> tables;  // (kind, a, b) -> device table
-  void* scratch = nullptr;
-  uint64_t scratch_bytes = 0;
-};
+#include "spx_host.h"
 
-namespace {
-
-enum TableKind { TW = 0, TW4 = 1, WIN = 2 };
+namespace spx_host {
 
 int fail(spx_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -41,28 +28,24 @@ int hip_fail(spx_ctx* c, hipError_t e, const char* what) {
   return fail(c, SPX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define SPX_HIP(ctx, expr)                              \
-  do {                                                  \
-    hipError_t e_ = (expr);                             \
-    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
-  } while (0)
-
 bool is_pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
 
 constexpr uint32_t kMaxN = 1u << 18;
 
-// Per-pass twiddle tables of a length-L transform (radix-16 passes first, the
-// leftover radix last; pass p has sub-length NS = 16^p): pass p >= 1 stores
-// [k][r] = W_(NS*R)^(r*k), k < NS, r < R, passes concatenated.  FP64, rounded once.
-std::vector<float> make_twiddles(uint32_t L) {
+// Per-pass twiddle tables of a length-L transform under radix order V (V = 0:
+// radix-16 passes, the leftover radix last; V = 1: the leftover first).  Pass
+// p >= 1, radix R, sub-length NS = product of the earlier radices, stores
+// [k][r] = W_(NS*R)^(r*k), k < NS, r < R; passes concatenated.  FP64, rounded once.
+std::vector<float> make_twiddles(uint32_t L, int V) {
   int log = 0;
   while ((1u << log) < L) ++log;
   const int np = (log + 3) / 4;
-  const int rl = (log % 4) ? (1 << (log % 4)) : 16;
+  const int rx = (log % 4) ? (1 << (log % 4)) : 16;
+  auto radix = [&](int p) { return np == 1 ? rx : (V == 0 ? (p == np - 1 ? rx : 16) : (p == 0 ? rx : 16)); };
   std::vector<float> t;
-  uint64_t ns = 16;
-  for (int p = 1; p < np; ++p, ns *= 16) {
-    const int R = (p == np - 1) ? rl : 16;
+  uint64_t ns = (uint64_t)radix(0);
+  for (int p = 1; p < np; ++p) {
+    const int R = radix(p);
     for (uint64_t k = 0; k < ns; ++k)
       for (int r = 0; r < R; ++r) {
         const uint64_t m = ns * (uint64_t)R;
@@ -71,6 +54,7 @@ std::vector<float> make_twiddles(uint32_t L) {
         t.push_back((float)std::cos(a));
         t.push_back((float)-std::sin(a));
       }
+    ns *= (uint64_t)R;
   }
   if (t.empty()) t.assign(2, 0.0f);
   return t;
@@ -117,8 +101,8 @@ int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out) {
   }
   std::vector<float> host;
   if (kind == TW) {
-    host = make_twiddles(a);
-    const int want = spx_twiddle_table_len(a);
+    host = make_twiddles(a, (int)b);
+    const int want = spx_twiddle_table_len(a, (int)b);
     if (want < 0 || (size_t)std::max(want, 1) * 2 != host.size())
       return fail(c, SPX_EINVAL, "twiddle table size mismatch");
   }
@@ -234,7 +218,9 @@ int dense_fft(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_t n_
   return run_fused(c, a);
 }
 
-}  // namespace
+}  // namespace spx_host
+
+using namespace spx_host;
 
 extern "C" {
 

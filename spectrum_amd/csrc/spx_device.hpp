@@ -147,23 +147,29 @@ template <> struct Dft<16> {
 };
 
 // ---------------------------------------------------------------------------
-// Radix plan for a power-of-two length L = 2^a: radix-16 passes first, the
-// leftover radix (2, 4 or 8) last; passes = ceil(a/4).  With this order every
-// LDS access of every pass is base + compile-time stride (see the layouts).
-// Twiddles: pass p >= 1 (sub-length NS = 16^p, radix R) reads a table laid
-// out [k][r] = W_(NS*R)^(r*k), k < NS, r < R, so one thread's R twiddles are
-// contiguous (R/2 16-byte loads from one address).  The tables of all passes
-// are concatenated; twoff(p) is the offset of pass p in float2 units.
+// Radix plan for a power-of-two length L = 2^a: ceil(a/4) passes, all radix 16
+// except one leftover radix RX (2, 4 or 8) when 4 does not divide a.
+//   V = 0: radix-16 passes first, the leftover last (every LDS access of every
+//          pass is then base + compile-time stride, see the layouts);
+//   V = 1: the leftover first, then radix 16 (an alternative the batch
+//          autotuner may pick per length).
+// Twiddles: pass p >= 1 (sub-length NS = product of the earlier radices, radix
+// R) reads a table laid out [k][r] = W_(NS*R)^(r*k), k < NS, r < R, so one
+// thread's R twiddles are contiguous (R/2 16-byte loads from one address).
+// The tables of all passes are concatenated; twoff(p) is the offset of pass p
+// in float2 units.
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, int V = 0>
 struct Plan {
   static constexpr int LOG = ilog2(L);
   static constexpr int NP = (LOG + 3) / 4;
-  static constexpr int RL = (LOG % 4) ? (1 << (LOG % 4)) : 16;  // last-pass radix
-  static constexpr int radix(int p) { return p == NP - 1 ? RL : 16; }
-  static constexpr int ns(int p) { return ipow16(p); }
+  static constexpr int RX = (LOG % 4) ? (1 << (LOG % 4)) : 16;
+  static constexpr int radix(int p) { return NP == 1 ? RX : (V == 0 ? (p == NP - 1 ? RX : 16) : (p == 0 ? RX : 16)); }
+  static constexpr int ns(int p) { return p <= 0 ? 1 : ns(p - 1) * radix(p - 1); }
   static constexpr int twoff(int p) { return p <= 1 ? 0 : twoff(p - 1) + ns(p - 1) * radix(p - 1); }
   static constexpr int TWSIZE = NP <= 1 ? 1 : twoff(NP - 1) + ns(NP - 1) * radix(NP - 1);
+  static constexpr int R0 = (NP == 1) ? RX : (V == 0 ? 16 : RX);   // first-pass radix
+  static constexpr int RL = (NP == 1) ? RX : (V == 0 ? RX : 16);   // last-pass radix
 };
 
 // ---------------------------------------------------------------------------
@@ -294,23 +300,24 @@ __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[P
   }
 }
 
-template <class LAY, int L, int NSEQ, int T, int PPT, int P>
+template <class LAY, int L, int NSEQ, int T, int PPT, int P, int V>
 struct FftPasses {
   __device__ __forceinline__ static void run(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
-    using PL = Plan<L>;
+    using PL = Plan<L, V>;
     fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1>(lds, v, tw + PL::twoff(P),
                                                                                        tid);
-    if constexpr (P + 1 < PL::NP) FftPasses<LAY, L, NSEQ, T, PPT, P + 1>::run(lds, v, tw, tid);
+    if constexpr (P + 1 < PL::NP) FftPasses<LAY, L, NSEQ, T, PPT, P + 1, V>::run(lds, v, tw, tid);
   }
 };
 
 // Full length-L transform of the tile.  Before the call v holds the first
-// pass's inputs (x[s][jj + r*L/R0], R0 = Plan<L>::radix(0), for the mapping of
-// j = tid + b*T); after it v holds X[s][jj + r*L/RL] for the radix-RL mapping.
-// tw: the concatenated per-pass twiddle tables of length L (Plan<L>::TWSIZE).
-template <class LAY, int L, int NSEQ, int T, int PPT>
+// pass's inputs (x[s][jj + r*L/R0], R0 = Plan<L,V>::R0, for the mapping of
+// j = tid + b*T); after it v holds X[s][jj + r*L/RL] for the radix-RL mapping
+// (RL = Plan<L,V>::RL).  tw: the concatenated per-pass twiddle tables of plan
+// (L, V) (Plan<L,V>::TWSIZE entries).
+template <class LAY, int L, int NSEQ, int T, int PPT, int V = 0>
 __device__ __forceinline__ void fft_tile(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
-  FftPasses<LAY, L, NSEQ, T, PPT, 0>::run(lds, v, tw, tid);
+  FftPasses<LAY, L, NSEQ, T, PPT, 0, V>::run(lds, v, tw, tid);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,43 @@
+// Host-internal interface shared by the C-ABI translation units: the
+// context object and its table / scratch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <tuple>
+
+#include "spx_kernels.h"
+
+struct spx_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  std::string name;
+  uint64_t hbm = 0;
+  std::string err;
+  std::map<std::tuple<int, uint32_t, uint32_t>, void*> tables;  // (kind, a, b) -> device table
+  void* scratch = nullptr;
+  uint64_t scratch_bytes = 0;
+};
+
+namespace spx_host {
+
+enum TableKind { TW = 0, TW4 = 1, WIN = 2 };
+
+int fail(spx_ctx* c, int code, const std::string& msg);
+int hip_fail(spx_ctx* c, hipError_t e, const char* what);
+// Device table of kind TW (a = length, b = radix order), TW4 (a = length) or
+// WIN (a = window type, b = length), generated in FP64 once per context.
+int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out);
+int window_ptr(spx_ctx* c, uint32_t window, uint32_t n_point, const float** w);
+
+}  // namespace spx_host
+
+#define SPX_HIP(ctx, expr)                                        \
+  do {                                                            \
+    hipError_t e_ = (expr);                                       \
+    if (e_ != hipSuccess) return spx_host::hip_fail(ctx, e_, #expr); \
+  } while (0)

@@ -40,11 +40,45 @@ struct FusedArgs {
   const void* mlp;         // learned refiner weights (refine == 2)
 };
 
+// Uneven-length batch (one persistent launch over a work list).
+// Bucket b holds length L = 2^(10+b): b 0..4 run as tiles of 16384/L channels
+// (whole channels in LDS), b 5..7 one channel per item on the four-step path.
+#define SPX_BATCH_BUCKETS 8
+#define SPX_BATCH_LOG_MIN 10
+#define SPX_BATCH_FIRST_LARGE 5
+
+struct BatchBucket {
+  const float2* tw;        // small: tables of plan (L, V); large: tables of the column length N2 = L/256 under V
+  const float* window;     // [L] or null
+  const float2* tw4;       // large only: W_L^(n1*k2) at [k2*256 + n1]
+  uint64_t pad;
+};
+
+struct BatchArgs {
+  const void* in;          // packed channels (dtype)
+  const uint64_t* offsets; // [n_chan] element offset of each channel in `in`
+  const uint32_t* perm;    // channel ids in work-list order
+  const uint4* items;      // {bucket | variant << 8, first index into perm, channel count, 0}
+  uint32_t* counter;       // work-list cursor (zeroed by spx_launch_batch before the batch kernel)
+  const float2* tw256;     // large path: tables of the 256-point row transform
+  BatchBucket bk[SPX_BATCH_BUCKETS];
+  uint32_t n_items, dtype, kcap, refine;
+  float fs, nsigma;
+  spx_max_value* peaks;    // [n_chan][kcap]
+  uint32_t* counts;        // [n_chan][2]
+  spx_max_value* gmax;     // [n_chan] or null
+  float* stats;            // [n_chan][4] or null
+  void* scratch;           // grid * slot_bytes (large items)
+  uint64_t slot_bytes;
+};
+
 extern "C" {
+hipError_t spx_batch_kernel_init(void);
+hipError_t spx_launch_batch(const BatchArgs* b, int grid, hipStream_t st);
 hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st);
 uint64_t spx_large_slot_bytes(uint32_t nfft);
 int spx_fused_is_large(uint32_t nfft);
-int spx_twiddle_table_len(uint32_t L);
+int spx_twiddle_table_len(uint32_t L, int V);
 hipError_t spx_launch_mag_phase(const void* in, void* out, uint64_t total, hipStream_t st);
 hipError_t spx_launch_magnitude(const void* in, float* out, uint64_t total, float inv_n, int squared,
                                 hipStream_t st);

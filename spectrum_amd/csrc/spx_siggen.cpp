@@ -181,3 +181,37 @@ extern "C" int spx_siggen(uint32_t kind, uint64_t seed, uint32_t n_chan, uint32_
   for (auto& th : pool) th.join();
   return SPX_OK;
 }
+
+extern "C" int spx_siggen_lengths(uint64_t seed, uint32_t n_chan, uint32_t lo_log2, uint32_t hi_log2,
+                                  uint32_t* lengths_out) {
+  if (!lengths_out || lo_log2 > hi_log2 || hi_log2 > 30) return SPX_EINVAL;
+  const uint32_t span = hi_log2 - lo_log2 + 1;
+  for (uint32_t c = 0; c < n_chan; ++c) {
+    Rng r(seed, c, 2);
+    uint32_t k = (uint32_t)(r.uniform() * (double)span);
+    if (k >= span) k = span - 1;
+    lengths_out[c] = 1u << (lo_log2 + k);
+  }
+  return SPX_OK;
+}
+
+extern "C" int spx_siggen_uneven(uint32_t kind, uint64_t seed, uint32_t n_chan, const uint32_t* lengths,
+                                 const uint64_t* offsets, void* out_c64, int threads) {
+  if (kind < SPX_GEN_THREE_TONES || kind > SPX_GEN_FRACTIONAL) return SPX_EINVAL;
+  if (!out_c64 || !lengths || !offsets) return SPX_EINVAL;
+  for (uint32_t c = 0; c < n_chan; ++c)
+    if (lengths[c] == 0) return SPX_EINVAL;
+  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+  if (threads < 1) threads = 1;
+  if ((uint32_t)threads > n_chan) threads = (int)(n_chan ? n_chan : 1);
+  float* out = (float*)out_c64;
+  auto work = [&](int t) {
+    for (uint32_t c = (uint32_t)t; c < n_chan; c += (uint32_t)threads)
+      gen_channel(kind, seed, c, lengths[c], out + 2 * offsets[c], 0, nullptr, nullptr);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  return SPX_OK;
+}

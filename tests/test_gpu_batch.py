@@ -1,0 +1,200 @@
+"""GPU parity of the uneven-length persistent batch (spx_batch_*).
+
+Every channel's counts, statistics and peak bins must equal what
+spx_fused_peaks gives for that channel alone (the same passes and reduction
+order; record values to within a few ulps) and, on sampled channels, the
+oracle's tie-tolerant peak list.  Graph replays are checked against the eager
+result on every replay.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from spectrum_amd import capi
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+K = 64
+
+
+def make_batch(ctx, n_chan, seed, lo=10, hi=17):
+    L = capi.siggen_lengths(seed, n_chan, lo, hi)
+    off, total, groups = capi.length_major_offsets(L)
+    x = capi.siggen_uneven(capi.GEN_MULTI_TONE, seed, L, off, total)
+    return L, off, total, groups, x, ctx.to_device(x)
+
+
+def outputs(ctx, n):
+    return ctx.alloc(n * K * 32), ctx.alloc(n * 8), ctx.alloc(n * 32), ctx.alloc(n * 16)
+
+
+def read(ctx, bufs, n):
+    pk, ct, gm, st = bufs
+    return (pk.download(capi.MAXVALUE_DTYPE, (n, K)), ct.download(np.uint32, (n, 2)),
+            gm.download(capi.MAXVALUE_DTYPE, (n,)), st.download(np.float32, (n, 4)))
+
+
+def sequential(ctx, d_in, groups, n):
+    """Per-length spx_fused_peaks launches over the length-major layout -> arrays indexed by channel."""
+    pk = np.zeros((n, K), capi.MAXVALUE_DTYPE)
+    ct = np.zeros((n, 2), np.uint32)
+    gm = np.zeros(n, capi.MAXVALUE_DTYPE)
+    st = np.zeros((n, 4), np.float32)
+    for L, (o, ids) in sorted(groups.items()):
+        B = len(ids)
+        bufs = outputs(ctx, B)
+        ctx.fused_peaks(d_in.ptr + 8 * o, B, L, L, bufs[0], bufs[1], window=1, fs=1.0, nsigma=6.0, max_peaks=K,
+                        gmax=bufs[2], stats=bufs[3])
+        ctx.sync()
+        p, c, g, s = read(ctx, bufs, B)
+        pk[ids], ct[ids], gm[ids], st[ids] = p, c, g, s
+    return pk, ct, gm, st
+
+
+def assert_same(a, b, what):
+    """Counts and peak bins identical; statistics and record values equal to within a few ulps (the four-step
+    body is the same source in both kernels, but instruction selection may fuse a multiply-add
+    differently when it is inlined into the batch kernel)."""
+    pa, ca, ga, sa = a
+    pb, cb, gb, sb = b
+    assert np.array_equal(ca, cb), f"{what}: counts differ"
+    assert np.allclose(sa, sb, rtol=2e-6, atol=0), f"{what}: stats differ"
+    assert np.array_equal(ga["index"], gb["index"]), f"{what}: global max bins differ"
+    for c in range(len(ca)):
+        n = ca[c, 0]
+        ra, rb = pa[c, :n], pb[c, :n]
+        if ra.tobytes() == rb.tobytes():
+            continue
+        assert np.array_equal(ra["index"], rb["index"]), f"{what}: channel {c} peak bins differ"
+        scale = float(np.abs(rb["magnitude"]).max())
+        for f in ("real", "imag", "magnitude"):
+            assert np.max(np.abs(ra[f] - rb[f])) <= 4e-7 * scale, f"{what}: channel {c} field {f}"
+        assert np.max(np.abs(ra["freq_offset"] - rb["freq_offset"])) <= 1e-6, f"{what}: channel {c} freq_offset"
+        dph = (ra["phase"] - rb["phase"] + 180.0) % 360.0 - 180.0
+        assert np.max(np.abs(dph)) <= 1e-3, f"{what}: channel {c} phase"
+
+
+def oracle_check(x, off, L, res, chans):
+    pk, ct, gm, st = res
+    for c in chans:
+        n = int(L[c])
+        X = orc.spectrum64(x[int(off[c]): int(off[c]) + n][None], n, 1)[0]
+        recs, found, g, stats, mags = orc.fused_peaks(X, 1.0, nsigma=6.0, kcap=K)
+        tau = 8 * 2.0 ** -24 * np.log2(n) * mags.max()
+        ok, why = orc.compare_peak_lists(pk[c, : ct[c, 0]]["index"], recs["index"], mags, stats[2], tau, K)
+        assert ok, f"channel {c} (N={n}): {why}"
+        assert abs(st[c, 2] - stats[2]) <= 1e-5 * stats[2] + tau
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_per_length_launches(ctx, graph):
+    n = 96
+    L, off, total, groups, x, d_in = make_batch(ctx, n, 0x5EED0103)
+    print(f"[gpu] batch n={n} lengths={sorted(groups)} graph={graph}", flush=True)
+    bufs = outputs(ctx, n)
+    desc = ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], gmax=bufs[2], stats=bufs[3], max_peaks=K, graph=graph)
+    plan = ctx.batch_plan(desc)
+    info = plan.info()
+    assert info["graph"] == graph and sum(b["channels"] for b in info["lengths"]) == n
+    plan.execute()
+    ctx.sync()
+    got = read(ctx, bufs, n)
+    ref = sequential(ctx, d_in, groups, n)
+    assert_same(got, ref, "batch vs per-length")
+    pick = [int(ids[0]) for _, (_, ids) in sorted(groups.items())]
+    oracle_check(x, off, L, got, pick)
+    plan.close()
+
+
+def test_graph_replays_match_eager(ctx):
+    n = 40
+    L, off, total, groups, x, d_in = make_batch(ctx, n, 0x5EED0203)
+    bufs = outputs(ctx, n)
+    eager = ctx.batch_plan(ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], gmax=bufs[2], stats=bufs[3],
+                                          max_peaks=K, graph=False))
+    eager.execute()
+    ctx.sync()
+    ref = read(ctx, bufs, n)
+    plan = ctx.batch_plan(ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], gmax=bufs[2], stats=bufs[3],
+                                         max_peaks=K, graph=True))
+    for rep in range(3):
+        # clobber the outputs with eager work between replays
+        ctx.fused_peaks(d_in, 1, 1024, 1024, bufs[0], bufs[1], max_peaks=K, gmax=bufs[2], stats=bufs[3])
+        ctx.sync()
+        plan.execute()
+        ctx.sync()
+        assert_same(read(ctx, bufs, n), ref, f"replay {rep}")
+    plan.close()
+    eager.close()
+
+
+def test_alternate_radix_order_and_autotune(ctx, tmp_path):
+    n = 64
+    L, off, total, groups, x, d_in = make_batch(ctx, n, 0x5EED0303)
+    bufs = outputs(ctx, n)
+    # a cache that forces the leftover-radix-first order everywhere
+    cache = tmp_path / "order1.json"
+    cache.write_text(json.dumps({"device": "any", "lengths": [{"length": 1 << b, "order": 1} for b in range(10, 18)]}))
+    desc = ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], gmax=bufs[2], stats=bufs[3], max_peaks=K,
+                          tune_cache=str(cache))
+    plan = ctx.batch_plan(desc)
+    orders = {b["length"]: b["order"] for b in plan.info()["lengths"]}
+    for Ln, o in orders.items():
+        tuned = Ln // 256 if Ln >= 32768 else Ln
+        assert o == (0 if (tuned.bit_length() - 1) % 4 == 0 else 1), orders
+    plan.execute()
+    ctx.sync()
+    got = read(ctx, bufs, n)
+    pick = [int(ids[i]) for _, (_, ids) in sorted(groups.items()) for i in range(min(2, len(ids)))]
+    oracle_check(x, off, L, got, pick)
+    plan.close()
+    # the autotuner measures both orders per length and writes its cache
+    tuned = tmp_path / "tuned.json"
+    ctx.batch_autotune(ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], max_peaks=K), str(tuned))
+    data = json.loads(tuned.read_text())
+    assert {e["length"] for e in data["lengths"]} == set(int(v) for v in np.unique(L))
+    for e in data["lengths"]:
+        assert e["order"] in (0, 1) and all(t > 0 for t in e["us"])
+        if len(e["us"]) == 2:
+            assert e["us"][e["order"]] == min(e["us"])
+    plan = ctx.batch_plan(ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], max_peaks=K, tune_cache=str(tuned)))
+    assert {b["length"]: b["order"] for b in plan.info()["lengths"]} == {e["length"]: e["order"] for e in data["lengths"]}
+    plan.close()
+
+
+def test_full_uneven_config_properties(ctx):
+    """4096 channels, lengths log-uniform 2^10..2^17 (about 1.07 GB): equal to per-length launches."""
+    n = 4096
+    L, off, total, groups, x, d_in = make_batch(ctx, n, 0x5EED0003)
+    print(f"[gpu] uneven config: {n} channels, {total * 8 / 1e9:.2f} GB", flush=True)
+    bufs = outputs(ctx, n)
+    plan = ctx.batch_plan(ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], gmax=bufs[2], stats=bufs[3], max_peaks=K))
+    plan.execute()
+    ctx.sync()
+    got = read(ctx, bufs, n)
+    ref = sequential(ctx, d_in, groups, n)
+    assert_same(got, ref, "uneven config")
+    pk, ct, gm, st = got
+    assert np.all(ct[:, 0] == np.minimum(ct[:, 1], K))
+    for c in range(0, n, 97):
+        idx = pk[c, : ct[c, 0]]["index"]
+        assert np.all(np.diff(idx.astype(np.int64)) > 0)
+        assert np.all(pk[c, : ct[c, 0]]["magnitude"] > st[c, 2])
+    oracle_check(x, off, L, got, [int(ids[-1]) for Lx, (o, ids) in sorted(groups.items())])
+    plan.close()
+
+
+def test_batch_rejects_bad_lengths(ctx):
+    d = ctx.alloc(4096 * 8)
+    bufs = outputs(ctx, 2)
+    for bad in ([1024, 512], [1024, 3000], [1024, 1 << 18]):
+        desc = ctx.batch_desc(d, np.array([0, 1024], np.uint64), np.array(bad, np.uint32), bufs[0], bufs[1])
+        with pytest.raises(ValueError):
+            ctx.batch_plan(desc)
+    desc = ctx.batch_desc(d, np.array([0, 1024], np.uint64), np.array([1024, 1024], np.uint32), bufs[0], bufs[1],
+                          max_peaks=4096)
+    with pytest.raises(capi.SpxError):
+        ctx.batch_plan(desc)
