@@ -150,6 +150,22 @@ typedef struct spx_fused_desc {
 
 int spx_fused_peaks(spx_ctx* ctx, const spx_fused_desc* desc);
 
+/* ---- mixed precision: FP16/BF16 storage, DFT-as-GEMM on the matrix cores --
+ * No counterpart in the reference (its FFT is hipFFT FP32 C2C,
+ * src/fft_func/src/fft_processor_rocm.cpp:353).  N = 16384 = 16 x 32 x 32:
+ * every butterfly stage is a real DFT-as-GEMM on MFMA with FP32 accumulation;
+ * the input is fp16 or bf16 complex ([batch][16384], re/im interleaved),
+ * multiplied by the window and a per-channel power-of-two pre-scale, stage
+ * operands are fp16 (compute F16) or split bf16 hi+lo (compute BF16).
+ * spx_fft_mixed writes the FP32 spectrum (complex64); spx_fused_peaks_mixed
+ * runs the spx_fused_peaks rules on it (desc->dtype fp16/bf16, n_point =
+ * nfft = 16384; refine NONE or PARABOLIC). */
+#define SPX_COMPUTE_F16 0
+#define SPX_COMPUTE_BF16 1
+int spx_fft_mixed(spx_ctx* ctx, const void* in, void* out_c64, uint32_t batch, uint32_t nfft, uint32_t window,
+                  uint32_t dtype, uint32_t compute);
+int spx_fused_peaks_mixed(spx_ctx* ctx, const spx_fused_desc* desc, uint32_t compute);
+
 /* ---- uneven-length batch: one persistent launch, captured in a hipGraph ----
  * No counterpart in the reference, which builds one hipFFT plan per (nFFT,
  * beam_count) and runs lengths one after another

@@ -25,6 +25,7 @@ DTYPE_C64, DTYPE_C32F16, DTYPE_C32BF16 = 0, 1, 2
 PEAKS_THRESHOLD, PEAKS_ALL_MAXIMA = 1, 2
 REFINE_NONE, REFINE_PARABOLIC, REFINE_LEARNED = 0, 1, 2
 GEN_THREE_TONES, GEN_MULTI_TONE, GEN_DYNAMIC, GEN_FRACTIONAL = 1, 2, 3, 4
+COMPUTE_F16, COMPUTE_BF16 = 0, 1
 
 # 32-byte peak record, same layout as the reference MaxValue
 MAXVALUE_DTYPE = np.dtype([
@@ -80,6 +81,8 @@ SIGNATURES = [
     ("spx_peaks_top", _I, [_P, _P, _P, _U32, _U32, _U32, _F, _I]),
     ("spx_all_maxima", _I, [_P, _P, _P, _P, _U32, _U32, _F, _U32, _U32, _U32]),
     ("spx_fused_peaks", _I, [_P, C.POINTER(SpxFusedDesc)]),
+    ("spx_fft_mixed", _I, [_P, _P, _P, _U32, _U32, _U32, _U32, _U32]),
+    ("spx_fused_peaks_mixed", _I, [_P, C.POINTER(SpxFusedDesc), _U32]),
     ("spx_batch_plan_create", _I, [_P, C.POINTER(SpxBatchDesc), C.POINTER(C.c_void_p)]),
     ("spx_batch_execute", _I, [_P]),
     ("spx_batch_plan_destroy", _I, [_P]),
@@ -234,6 +237,25 @@ class Context:
         self.check(self.lib.spx_fused_peaks(self.h, C.byref(d)), "spx_fused_peaks")
 
 
+    # -- mixed precision (fp16 / bf16 storage, MFMA DFT-as-GEMM), N = 16384
+    def fft_mixed(self, inp, out, batch, nfft=16384, window=1, dtype=DTYPE_C32F16, compute=COMPUTE_F16):
+        self.check(self.lib.spx_fft_mixed(self.h, _ptr(inp), _ptr(out), batch, nfft, window, dtype, compute),
+                   "spx_fft_mixed")
+
+    def fused_peaks_mixed(self, inp, batch, nfft, peaks, counts, *, compute=COMPUTE_F16, window=1,
+                          dtype=DTYPE_C32F16, fs=1.0, nsigma=6.0, max_peaks=32, s0=0, s1=0,
+                          refine=REFINE_PARABOLIC, gmax=None, stats=None, in_scale=None):
+        d = SpxFusedDesc()
+        d.input = _ptr(inp)
+        d.in_scale = _ptr(in_scale)
+        d.batch, d.n_point, d.nfft = batch, nfft, nfft
+        d.window, d.dtype, d.sample_rate = window, dtype, float(fs)
+        d.peak_mode, d.nsigma, d.max_peaks = PEAKS_THRESHOLD, float(nsigma), max_peaks
+        d.search_start, d.search_end, d.refine = s0, s1, refine
+        d.peaks_out, d.counts_out = _ptr(peaks), _ptr(counts)
+        d.gmax_out, d.stats_out = _ptr(gmax), _ptr(stats)
+        self.check(self.lib.spx_fused_peaks_mixed(self.h, C.byref(d), compute), "spx_fused_peaks_mixed")
+
     # -- uneven-length batch
     def batch_desc(self, inp, offsets: np.ndarray, lengths: np.ndarray, peaks, counts, *, window=1,
                    dtype=DTYPE_C64, fs=1.0, nsigma=6.0, max_peaks=64, refine=REFINE_PARABOLIC, gmax=None,
@@ -382,3 +404,30 @@ def siggen_uneven(kind: int, seed: int, lengths: np.ndarray, offsets: np.ndarray
     if rc != SPX_OK:
         raise ValueError(f"spx_siggen_uneven failed with status {rc}")
     return out
+
+
+def to_half_storage(x: np.ndarray, dtype: int) -> np.ndarray:
+    """complex64 -> packed 16-bit complex storage (uint32 per element: re in the low half, im in the high half),
+    IEEE fp16 (DTYPE_C32F16) or bfloat16 (DTYPE_C32BF16), round to nearest even."""
+    x = np.ascontiguousarray(x, dtype=np.complex64)
+    f = x.view(np.float32).reshape(x.shape + (2,))
+    if dtype == DTYPE_C32F16:
+        h = f.astype(np.float16).view(np.uint16).astype(np.uint32)
+    elif dtype == DTYPE_C32BF16:
+        u = f.view(np.uint32).astype(np.uint64)
+        h = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint32) & 0xFFFF
+    else:
+        raise ValueError("dtype must be DTYPE_C32F16 or DTYPE_C32BF16")
+    return (h[..., 0] | (h[..., 1] << 16)).astype(np.uint32)
+
+
+def from_half_storage(u: np.ndarray, dtype: int) -> np.ndarray:
+    """Inverse of to_half_storage (exact): packed 16-bit complex -> complex64."""
+    u = np.asarray(u, dtype=np.uint32)
+    lo, hi = (u & 0xFFFF).astype(np.uint16), (u >> 16).astype(np.uint16)
+    if dtype == DTYPE_C32F16:
+        re, im = lo.view(np.float16).astype(np.float32), hi.view(np.float16).astype(np.float32)
+    else:
+        re = (lo.astype(np.uint32) << 16).view(np.float32)
+        im = (hi.astype(np.uint32) << 16).view(np.float32)
+    return (re + 1j * im).astype(np.complex64)

@@ -30,6 +30,57 @@ int hip_fail(spx_ctx* c, hipError_t e, const char* what) {
 
 bool is_pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
 
+// IEEE binary16 and bfloat16 encodings, round to nearest even (finite inputs).
+uint16_t f32_to_f16_bits(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const int32_t e = (int32_t)((x >> 23) & 0xffu) - 127 + 15;
+  uint32_t mant = x & 0x7fffffu;
+  if (((x >> 23) & 0xffu) == 0 && mant == 0) return (uint16_t)sign;
+  if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+  if (e <= 0) {  // subnormal half
+    if (e < -10) return (uint16_t)sign;
+    mant |= 0x800000u;
+    const int shift = 14 - e;
+    uint32_t h = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (h & 1))) ++h;
+    return (uint16_t)(sign | h);
+  }
+  uint32_t h = ((uint32_t)e << 10) | (mant >> 13);
+  const uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1))) ++h;
+  return (uint16_t)(sign | h);
+}
+float f16_bits_to_f32(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  float v;
+  if (e == 0) v = std::ldexp((float)m, -24);
+  else if (e == 31) v = INFINITY;
+  else v = std::ldexp((float)(m | 0x400u), (int)e - 25);
+  uint32_t x;
+  std::memcpy(&x, &v, 4);
+  x |= sign;
+  std::memcpy(&v, &x, 4);
+  return v;
+}
+uint16_t f32_to_bf16_bits(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  x += 0x7fffu + ((x >> 16) & 1u);
+  return (uint16_t)(x >> 16);
+}
+std::vector<float> make_mixed_tables(uint32_t compute);
+
+float bf16_bits_to_f32(uint16_t h) {
+  const uint32_t x = (uint32_t)h << 16;
+  float v;
+  std::memcpy(&v, &x, 4);
+  return v;
+}
+
 constexpr uint32_t kMaxN = 1u << 18;
 
 // Per-pass twiddle tables of a length-L transform under radix order V (V = 0:
@@ -107,6 +158,7 @@ int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out) {
       return fail(c, SPX_EINVAL, "twiddle table size mismatch");
   }
   else if (kind == TW4) host = make_fourstep(a);
+  else if (kind == MIX) host = make_mixed_tables(a);
   else host = make_window(a, b);
   void* d = nullptr;
   SPX_HIP(c, hipSetDevice(c->device));
@@ -373,7 +425,13 @@ int spx_all_maxima(spx_ctx* c, const void* spec, void* out, uint32_t* counts, ui
   return SPX_OK;
 }
 
-int spx_fused_peaks(spx_ctx* c, const spx_fused_desc* d) {
+}  // extern "C"
+
+namespace spx_host {
+
+// Validates a fused descriptor and fills the kernel arguments (no tables).
+int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a) {
+  std::memset(&a, 0, sizeof(a));
   if (!c) return SPX_EINVAL;
   if (!d) return fail(c, SPX_EINVAL, "null descriptor");
   if (d->batch == 0) return SPX_OK;
@@ -398,8 +456,6 @@ int spx_fused_peaks(spx_ctx* c, const spx_fused_desc* d) {
   if (s0 >= s1 || s1 > d->nfft) return fail(c, SPX_EINVAL, "empty or out-of-range search window");
   if (spx_fused_is_large(d->nfft) && d->max_peaks > 2048)
     return fail(c, SPX_ECAP, "max_peaks above 2048 is not supported for nfft >= 32768");
-  FusedArgs a;
-  std::memset(&a, 0, sizeof(a));
   a.in = d->input;
   a.in_scale = d->in_scale;
   a.batch = d->batch;
@@ -418,7 +474,140 @@ int spx_fused_peaks(spx_ctx* c, const spx_fused_desc* d) {
   a.counts = d->counts_out;
   a.gmax = d->gmax_out;
   a.stats = d->stats_out;
+  return SPX_OK;
+}
+
+// Mixed-precision table block for N = 16384 (layout documented in spx_mixed.hip):
+// TB[j1][m2] = 2^-5 W_1024^(m2 j1), TAL[e] = W_16384^e and TAH[e] = W_128^e (e < 128)
+// as float2, then the DFT-16 and DFT-32 real-matrix MFMA fragments, hi and lo
+// parts, in the compute type (fp16 or bf16).  FP64, each value rounded once.
+std::vector<float> make_mixed_tables(uint32_t compute) {
+  const size_t bytes = spx_mixed_table_bytes();
+  std::vector<float> t((bytes + 3) / 4, 0.0f);
+  unsigned char* p = reinterpret_cast<unsigned char*>(t.data());
+  auto tw = [](uint64_t e, uint64_t n, float* o) {
+    const double a = 2.0 * M_PI * (double)(e % n) / (double)n;
+    o[0] = (float)std::cos(a);
+    o[1] = (float)-std::sin(a);
+  };
+  float* TB = reinterpret_cast<float*>(p);
+  for (uint32_t j1 = 0; j1 < 32; ++j1)
+    for (uint32_t m2 = 0; m2 < 32; ++m2) {
+      float* o = TB + 2 * (j1 * 32 + m2);
+      tw((uint64_t)m2 * j1, 1024, o);
+      o[0] *= 0.03125f;  // the stage-2 scale 2^-5 (exact)
+      o[1] *= 0.03125f;
+    }
+  float* TAL = TB + 2 * 1024;
+  float* TAH = TAL + 2 * 128;
+  for (uint32_t e = 0; e < 128; ++e) {
+    tw(e, 16384, TAL + 2 * e);
+    tw(e, 128, TAH + 2 * e);
+  }
+  // real 2R x 2R DFT matrix [[C, S], [-S, C]]
+  auto M = [](int R, int r, int k) -> double {
+    const int rr = r % R, kk = k % R;
+    const double a = 2.0 * M_PI * (double)((rr * kk) % R) / (double)R;
+    if (r < R) return k < R ? std::cos(a) : std::sin(a);
+    return k < R ? -std::sin(a) : std::cos(a);
+  };
+  auto put = [&](uint16_t* hi, uint16_t* lo, double v) {
+    if (compute == 0) {
+      const uint16_t h = f32_to_f16_bits((float)v);
+      *hi = h;
+      *lo = f32_to_f16_bits((float)(v - (double)f16_bits_to_f32(h)));
+    } else {
+      const uint16_t h = f32_to_bf16_bits((float)v);
+      *hi = h;
+      *lo = f32_to_bf16_bits((float)(v - (double)bf16_bits_to_f32(h)));
+    }
+  };
+  uint16_t* W16 = reinterpret_cast<uint16_t*>(p + 1024 * 8 + 256 * 8);
+  for (int rb = 0; rb < 2; ++rb)
+    for (int l = 0; l < 64; ++l)
+      for (int j = 0; j < 8; ++j) {
+        const size_t i = ((size_t)rb * 64 + l) * 8 + j;
+        put(W16 + i, W16 + 2 * 64 * 8 + i, M(16, 16 * rb + l % 16, 8 * (l / 16) + j));
+      }
+  uint16_t* W32 = W16 + 2 * 2 * 64 * 8;
+  for (int rb = 0; rb < 4; ++rb)
+    for (int ks = 0; ks < 2; ++ks)
+      for (int l = 0; l < 64; ++l)
+        for (int j = 0; j < 8; ++j) {
+          const size_t i = (((size_t)(rb * 2 + ks)) * 64 + l) * 8 + j;
+          put(W32 + i, W32 + 8 * 64 * 8 + i, M(32, 16 * rb + l % 16, 32 * ks + 8 * (l / 16) + j));
+        }
+  return t;
+}
+
+}  // namespace spx_host
+
+extern "C" {
+
+int spx_fused_peaks(spx_ctx* c, const spx_fused_desc* d) {
+  FusedArgs a;
+  int rc = fused_args(c, d, a);
+  if (rc || d->batch == 0) return rc;
   return run_fused(c, a);
 }
+
+// Window coefficients (w = c0 - c1 cos(2 pi n/(n-1)) + c2 cos(4 pi n/(n-1)),
+// the formulas of make_window), tables, persistent grid, launch.
+static int run_mixed(spx_ctx* c, FusedArgs& a, uint32_t window, uint32_t compute) {
+  a.window = nullptr;
+  a.wkind = window;
+  a.wc0 = a.wc1 = a.wc2 = 0.0f;
+  if (window == SPX_WINDOW_HANN) { a.wc0 = 0.5f; a.wc1 = 0.5f; }
+  else if (window == SPX_WINDOW_HAMMING) { a.wc0 = 0.54f; a.wc1 = 0.46f; }
+  else if (window == SPX_WINDOW_BLACKMAN) { a.wc0 = 0.42f; a.wc1 = 0.5f; a.wc2 = 0.08f; }
+  const void* t;
+  int rc = get_table(c, MIX, compute, 0, &t);
+  if (rc) return rc;
+  a.tw_a = (const float2*)t;
+  int grid = c->cu_count > 0 ? c->cu_count : 256;
+  if ((uint32_t)grid > a.batch) grid = (int)a.batch;
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_mixed(&a, (int)compute, grid, c->stream));
+  return SPX_OK;
+}
+
+static int mixed_check(spx_ctx* c, uint32_t nfft, uint32_t n_point, uint32_t dtype, uint32_t compute) {
+  if (nfft != 16384 || n_point != 16384)
+    return fail(c, SPX_ENOTSUP, "mixed-precision path: nfft = n_point = 16384 only");
+  if (dtype != SPX_DTYPE_C32F16 && dtype != SPX_DTYPE_C32BF16)
+    return fail(c, SPX_EINVAL, "mixed-precision path: input dtype must be fp16 or bf16");
+  if (compute > SPX_COMPUTE_BF16) return fail(c, SPX_EINVAL, "unknown compute type");
+  return SPX_OK;
+}
+
+int spx_fused_peaks_mixed(spx_ctx* c, const spx_fused_desc* d, uint32_t compute) {
+  FusedArgs a;
+  int rc = fused_args(c, d, a);
+  if (rc || d->batch == 0) return rc;
+  if ((rc = mixed_check(c, d->nfft, d->n_point, d->dtype, compute))) return rc;
+  return run_mixed(c, a, d->window, compute);
+}
+
+int spx_fft_mixed(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_t nfft, uint32_t window,
+                  uint32_t dtype, uint32_t compute) {
+  if (!c) return SPX_EINVAL;
+  if (batch == 0) return SPX_OK;
+  if (!in || !out) return fail(c, SPX_EINVAL, "null pointer");
+  int rc = mixed_check(c, nfft, nfft, dtype, compute);
+  if (rc) return rc;
+  if (window > SPX_WINDOW_BLACKMAN) return fail(c, SPX_EINVAL, "unknown window");
+  FusedArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in;
+  a.batch = batch;
+  a.n_point = nfft;
+  a.nfft = nfft;
+  a.dtype = dtype;
+  a.fs = 1.0f;
+  a.spec_out = (float2*)out;
+  a.peak_mode = 0;
+  return run_mixed(c, a, window, compute);
+}
+
 
 }  // extern "C"

@@ -38,6 +38,10 @@ struct FusedArgs {
   void* scratch;           // large: per-workgroup slots
   uint64_t slot_bytes;
   const void* mlp;         // learned refiner weights (refine == 2)
+  // window as w[n] = c0 - c1 cos(2 pi n/(n_point-1)) + c2 cos(4 pi n/(n_point-1))
+  // (kernels that evaluate it instead of reading `window`; wkind 0 = none)
+  uint32_t wkind;
+  float wc0, wc1, wc2;
 };
 
 // Uneven-length batch (one persistent launch over a work list).
@@ -74,6 +78,8 @@ struct BatchArgs {
 
 extern "C" {
 hipError_t spx_batch_kernel_init(void);
+uint64_t spx_mixed_table_bytes(void);
+hipError_t spx_launch_mixed(const FusedArgs* a, int compute, int grid, hipStream_t st);
 hipError_t spx_launch_batch(const BatchArgs* b, int grid, hipStream_t st);
 hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st);
 uint64_t spx_large_slot_bytes(uint32_t nfft);

@@ -1,0 +1,456 @@
+// Mixed-precision FFT -> |X| -> peak search for N = 16384 = 16 x 32 x 32 on
+// the gfx950 matrix cores: every butterfly stage is a real DFT-as-GEMM on
+// MFMA (v_mfma_f32_16x16x32_{f16,bf16}) with FP32 accumulation; twiddles,
+// scaling, |X| and the peak rules run in FP32 on the vector units.
+//
+// Index map (Cooley-Tukey, natural-order input):
+//   n = 1024*n1 + n2,  n2 = 32*m1 + m2;   k = k1 + 16*(j1 + 32*j2)
+//   A [k1][n2]     = sum_n1 x[1024 n1 + n2] W16^(n1 k1)          (stage 1, DFT-16)
+//   A'[k1][n2]     = A W_16384^(n2 k1)
+//   B [k1][j1][m2] = sum_m1 A'[k1][32 m1 + m2] W32^(m1 j1)       (stage 2, DFT-32)
+//   B'             = B W_1024^(m2 j1)
+//   X[k]           = sum_m2 B'[k1][j1][m2] W32^(m2 j2)            (stage 3, DFT-32)
+// A complex DFT-R is the real 2R x 2R matrix [[C, S], [-S, C]] (C = cos,
+// S = sin of 2 pi r q / R) acting on [Re; Im]; the data of 16 sequences are
+// the B operand (K = 32 real inputs per MFMA), the DFT matrix the A operand.
+// MFMA 16x16x32 lane layout (checked on the device): lane l holds
+// A[l%16][8(l/16) + j], B[8(l/16) + j][l%16], D[4(l/16) + i][l%16].
+//
+// Precision: the input (fp16 or bf16 storage) is multiplied by the window and
+// a per-channel power-of-two pre-scale s (max component * s < 2^11), and
+// every stage's operands are fp16: stage-1 outputs stay below 16*sqrt(2)*2^11,
+// stage-2 outputs are scaled by 2^-5 (exact) before rounding, so no operand
+// exceeds the fp16 range.  The fp16 variant uses the fp16-rounded DFT
+// matrices directly (data rounding, 2^-11 per stage, dominates); the bf16
+// variant splits both operands into hi + lo (hi*hi + hi*lo + lo*hi, three
+// MFMAs), which reproduces the fp16 data and a 16-bit matrix.  Twiddles
+// W_16384^e come from two 128-entry LDS tables (e = 128 e_hi + e_lo), the
+// stage-2 twiddles from a 1024-entry LDS table; every table is generated in
+// FP64 on the host and rounded once.  The window is evaluated in FP32 from
+// its cosine coefficients (hardware cosine in revolutions).
+#include "spx_fused_common.hpp"
+
+namespace spx {
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace mx {
+constexpr int N = 16384;
+constexpr int T = 512;             // 8 waves
+constexpr int PSTR = 36;           // stage-2 operand row stride in complex (144 B: conflict-free stage-1 stores)
+constexpr int PING_BYTES = 512 * PSTR * 4;     // [k1][m2] rows of 32 (+4) half2 {re, im}
+constexpr int PONG_BYTES = 512 * PSTR * 4;     // [j1][k1] rows of 32 (+4) half2
+constexpr int XPAD = N + N / 32;               // spectrum plane, bin k at k + k/32 (conflict-free per-thread runs)
+constexpr int TB_LDS = PING_BYTES + PONG_BYTES;   // 2^-5 W_1024^(m2 j1), [j1][m2] float2
+constexpr int TAL_LDS = TB_LDS + 1024 * 8;        // W_16384^e, e < 128
+constexpr int TAH_LDS = TAL_LDS + 128 * 8;        // W_128^e, e < 128
+constexpr int LDS_BYTES = TAH_LDS + 128 * 8;
+static_assert(XPAD * 4 <= PING_BYTES && XPAD * 4 <= PONG_BYTES, "FP32 spectrum planes fit the operand buffers");
+// global table block (bytes): TB [32][32] float2 | TAL [128] float2 | TAH [128] float2 | W16 frags | W32 frags
+constexpr int TB_OFF = 0;
+constexpr int TAL_OFF = TB_OFF + 1024 * 8;
+constexpr int TAH_OFF = TAL_OFF + 128 * 8;
+constexpr int W16_OFF = TAH_OFF + 128 * 8;
+constexpr int W16_BYTES = 2 * 2 * 64 * 16;      // [hi|lo][rb 2][lane 64][8 x 16 bit]
+constexpr int W32_OFF = W16_OFF + W16_BYTES;
+constexpr int W32_BYTES = 2 * 8 * 64 * 16;      // [hi|lo][rb 4 x ks 2][lane 64][8 x 16 bit]
+constexpr int TABLE_BYTES = W32_OFF + W32_BYTES;
+}  // namespace mx
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// D += W * x for one 16x16x32 block; W = whi (+ wlo).
+template <int CT>
+struct Mma;
+template <>
+struct Mma<0> {  // fp16 operands; the fp16 DFT matrix (2^-12 rounding) is used unsplit
+  using W = f16x8;
+  static constexpr bool SPLIT = false;
+  __device__ __forceinline__ static f32x4 run(const W& whi, const W&, const f16x8& x, f32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(whi, x, acc, 0, 0, 0);
+  }
+};
+template <>
+struct Mma<1> {  // bf16 operands, both split into hi + lo
+  using W = bf16x8;
+  static constexpr bool SPLIT = true;
+  __device__ __forceinline__ static f32x4 run(const W& whi, const W& wlo, const f16x8& x, f32x4 acc) {
+    bf16x8 xh, xl;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float f = (float)x[j];
+      xh[j] = (__bf16)f;
+      xl[j] = (__bf16)(f - (float)xh[j]);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(whi, xh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(whi, xl, acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(wlo, xh, acc, 0, 0, 0);
+  }
+};
+
+template <int DT>
+__device__ __forceinline__ float2 unpack_in(uint32_t u) {
+  if constexpr (DT == SPX_DTYPE_C32F16) {
+    return make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu)),
+                       (float)__builtin_bit_cast(_Float16, (uint16_t)(u >> 16)));
+  } else {
+    return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+  }
+}
+
+__device__ __forceinline__ uint32_t pack_h2(float re, float im) {
+  const f16x2 h = {(_Float16)re, (_Float16)im};
+  return __builtin_bit_cast(uint32_t, h);
+}
+
+// 8 consecutive half2 {re, im} at p (16-byte aligned) -> re and im operand vectors
+__device__ __forceinline__ void load_ops(const uint32_t* p, f16x8& re, f16x8& im) {
+  const uint4 q0 = *reinterpret_cast<const uint4*>(p);
+  const uint4 q1 = *reinterpret_cast<const uint4*>(p + 4);
+  const uint32_t u[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const f16x2 h = __builtin_bit_cast(f16x2, u[j]);
+    re[j] = h[0];
+    im[j] = h[1];
+  }
+}
+
+// one channel's stage-1 input for this lane: x[1024 n1 + n2], n1 = 8(g&1)+j, n2 = 16(w+8t)+c
+__device__ __forceinline__ void load_raw(const FusedArgs& a, uint32_t ch, int w, int g, int c, uint32_t (&raw)[8][8]) {
+  const uint32_t* in = reinterpret_cast<const uint32_t*>(a.in) + (uint64_t)ch * mx::N;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int n2 = 16 * (w + 8 * t) + c;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) raw[t][j] = gp(in)[(8 * (g & 1) + j) * 1024 + n2];
+  }
+}
+
+// Threshold peak search over the FP32 spectrum planes (bin k at k + k/32);
+// thread t owns natural bins 32t .. 32t+31.
+__device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, const float* xre, const float* xim,
+                                            BlockRed& red) {
+  constexpr int N = mx::N, T = mx::T, E = N / T;
+  const int tid = threadIdx.x;
+  auto adr = [](uint32_t k) -> int { return (int)(k + (k >> 5)); };
+  auto xat = [&](uint32_t k) -> float2 { const int q = adr(k); return make_float2(xre[q], xim[q]); };
+  const uint32_t k0 = (uint32_t)tid * E;
+  float fs1 = 0.f, fs2 = 0.f, bm = -1.0f;
+  uint32_t bk = 0xffffffffu;
+#pragma unroll 8
+  for (int e = 0; e < E; ++e) {
+    const uint32_t k = k0 + e;
+    const float m = mag_of(xat(k));
+    if (in_range(k, a)) {
+      fs1 += m;
+      fs2 = fmaf(m, m, fs2);
+      if (m > bm) { bm = m; bk = k; }
+    }
+  }
+  double s1 = fs1, s2 = fs2;
+  block_stats<T>(s1, s2, bm, bk, red);
+  const double cnt = (double)(a.s1 - a.s0);
+  float mu, sd;
+  const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
+  // second pass re-reads the magnitudes (LDS) instead of keeping 32 of them live
+  float yl = mag_of(xat((k0 + N - 1) % N));
+  float yc = mag_of(xat(k0));
+  uint32_t flags = 0;
+  int cpk = 0;
+#pragma unroll 8
+  for (int e = 0; e < E; ++e) {
+    const uint32_t k = k0 + e;
+    const float yr = mag_of(xat((k + 1) % N));
+    if (in_range(k, a) && yc > yl && yc > yr && yc > thr) {
+      flags |= 1u << e;
+      ++cpk;
+    }
+    yl = yc;
+    yc = yr;
+  }
+  int total;
+  const int off = block_excl_scan<T>(cpk, total, red);
+  if (flags && off < (int)a.kcap) {
+    MaxValue* out = a.peaks + (uint64_t)ch * a.kcap;
+    int pos = off;
+    while (flags && pos < (int)a.kcap) {
+      const int e = __builtin_ctz(flags);
+      flags &= flags - 1;
+      const uint32_t k = k0 + e;
+      const float yc = mag_of(xat(k));
+      const float yl = mag_of(xat((k + N - 1) % N));
+      const float yr = mag_of(xat((k + 1) % N));
+      write_record(out + pos, k, xat(k), yl, yc, yr, a);
+      ++pos;
+    }
+  }
+  if (tid == 0) {
+    if (a.counts) {
+      gp(a.counts)[2 * ch + 0] = (uint32_t)min(total, (int)a.kcap);
+      gp(a.counts)[2 * ch + 1] = (uint32_t)total;
+    }
+    if (a.stats) {
+      gp(a.stats)[4 * ch + 0] = mu;
+      gp(a.stats)[4 * ch + 1] = sd;
+      gp(a.stats)[4 * ch + 2] = thr;
+      gp(a.stats)[4 * ch + 3] = (float)(cnt > 0 ? s2 / cnt : 0.0);
+    }
+    if (a.gmax && bk != 0xffffffffu) {
+      const float yl = mag_of(xat((bk + N - 1) % N));
+      const float yr = mag_of(xat((bk + 1) % N));
+      write_record(a.gmax + ch, bk, xat(bk), yl, bm, yr, a);
+    }
+  }
+}
+
+// MODE 0: dense FP32 spectrum to a.spec_out; MODE 1: threshold peak search.
+// Persistent: workgroup b runs channels b, b + grid, ...; the next channel's
+// input is loaded while the current one goes through stages 2, 3 and the peaks.
+template <int DT, int CT, int MODE>
+__global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
+  using namespace mx;
+  using WT = typename Mma<CT>::W;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ BlockRed red;
+  uint32_t* ping = reinterpret_cast<uint32_t*>(smem);
+  uint32_t* pong = reinterpret_cast<uint32_t*>(smem + PING_BYTES);
+  float* xre = reinterpret_cast<float*>(smem);                // spectrum planes, natural order padded
+  float* xim = reinterpret_cast<float*>(smem + PING_BYTES);
+  float2* tb = reinterpret_cast<float2*>(smem + TB_LDS);
+  float2* tal = reinterpret_cast<float2*>(smem + TAL_LDS);
+  float2* tah = reinterpret_cast<float2*>(smem + TAH_LDS);
+
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
+  const unsigned char* tabs = reinterpret_cast<const unsigned char*>(a.tw_a);
+  {
+    const float2* gt = reinterpret_cast<const float2*>(tabs);   // TB | TAL | TAH are contiguous
+    for (int i = tid; i < 1024 + 256; i += T) tb[i] = gp(gt)[i];
+  }
+  uint32_t ch = blockIdx.x;
+  if (ch >= a.batch) return;
+  uint32_t raw[8][8];
+  load_raw(a, ch, w, g, c, raw);
+  __syncthreads();
+
+  for (; ch < a.batch; ch += gridDim.x) {
+    // recompute every index from a laundered thread id so that the
+    // channel-invariant window values and twiddles are not hoisted out of
+    // the loop and kept live (they would not fit in registers)
+    const int tid = opaque_v((int)threadIdx.x);
+    const int w = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
+    const WT* W16 = opaque(reinterpret_cast<const WT*>(tabs + W16_OFF));
+    const WT* W32 = opaque(reinterpret_cast<const WT*>(tabs + W32_OFF));
+    // keep the prefetched words packed until they are used
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) raw[t][j] = (uint32_t)opaque_v((int)raw[t][j]);
+    // ---- per-channel power-of-two pre-scale from the largest component
+    const float isc = (a.in_scale != nullptr) ? gp(a.in_scale)[ch] : 1.0f;
+    float mx_ = 0.f;
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float2 v = unpack_in<DT>(raw[t][j]);
+        mx_ = fmaxf(mx_, fmaxf(fabsf(v.x), fabsf(v.y)));
+      }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx_ = fmaxf(mx_, __shfl_xor(mx_, o, 64));
+    if (l == 0) red.m[w] = mx_;
+    __syncthreads();
+    float M = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) M = fmaxf(M, red.m[i]);
+    M *= fabsf(isc);
+    int e2 = 0;
+    if (M > 0.f && M < __builtin_inff()) (void)frexpf(M, &e2);
+    e2 = max(min(e2, 100), -100);
+    const float s_in = ldexpf(isc, 11 - e2);   // max component * s_in < 2^11
+    const float s_out = ldexpf(1.0f, e2 - 6);  // undoes s_in / isc and the stage-2 2^-5
+
+    // ---- stage 1: DFT-16 over n1, twiddle W_16384^(n2 k1), store half2 rows [k1][m2][m1]
+    {
+      const WT w0h = gp(W16)[0 * 64 + l], w1h = gp(W16)[1 * 64 + l];
+      WT w0l = w0h, w1l = w1h;
+      if constexpr (Mma<CT>::SPLIT) {
+        w0l = gp(W16)[128 + 0 * 64 + l];
+        w1l = gp(W16)[128 + 1 * 64 + l];
+      }
+      // window by rotation: the lane's 8 samples of block t are n0 + 1024 j, so
+      // e^(i theta_j) = e^(i theta_0) * e^(i j delta) (FP32, 7 steps)
+      const float winv = 1.0f / (float)(N - 1);
+      const float dr = 1024.0f * winv;  // revolutions per step
+      const float dc = __builtin_amdgcn_cosf(dr), ds = __builtin_amdgcn_sinf(dr);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int n2 = 16 * (w + 8 * t) + c;
+        const float r0 = (float)((8 * (g & 1)) * 1024 + n2) * winv;
+        float cr = __builtin_amdgcn_cosf(r0), ci = __builtin_amdgcn_sinf(r0);
+        f16x8 b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const float2 v = unpack_in<DT>(raw[t][j]);
+          float wn = 1.0f;
+          if (a.wkind != 0) wn = a.wc0 - a.wc1 * cr + a.wc2 * fmaf(2.0f * cr, cr, -1.0f);
+          b[j] = (_Float16)(((g < 2) ? v.x : v.y) * (wn * s_in));
+          const float nr = fmaf(cr, dc, -ci * ds);
+          ci = fmaf(cr, ds, ci * dc);
+          cr = nr;
+        }
+        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        const f32x4 dre = Mma<CT>::run(w0h, w0l, b, z);
+        const f32x4 dim = Mma<CT>::run(w1h, w1l, b, z);
+        const int m1 = n2 >> 5, m2 = n2 & 31;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k1 = 4 * g + i;
+          const int e = n2 * k1;  // < 16384
+          const float2 tw = c_mul(tal[e & 127], tah[e >> 7]);
+          const float2 y = c_mul(make_float2(dre[i], dim[i]), tw);
+          ping[(k1 * 32 + m2) * PSTR + m1] = pack_h2(y.x, y.y);
+        }
+      }
+    }
+    __syncthreads();
+    // the next channel's input streams in while the rest of this channel runs
+    // (from here for the dense output; after stage 3 for the peak search,
+    // whose registers it would otherwise crowd)
+    if constexpr (MODE == 0) {
+      const uint32_t nx = ch + gridDim.x;
+      if (nx < a.batch) load_raw(a, nx, w, g, c, raw);
+    }
+
+    // DFT-32 matrix fragments: [rb 0..3][ks 0..1]
+    WT wh[8], wl[8];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+      wh[f] = gp(W32)[f * 64 + l];
+      wl[f] = wh[f];
+      if constexpr (Mma<CT>::SPLIT) wl[f] = gp(W32)[512 + f * 64 + l];
+    }
+
+    // ---- stage 2: DFT-32 over m1 for the 512 columns (k1, m2), twiddle, store rows [j1][k1][m2]
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int cb = w + 8 * t;
+      const int k1 = cb >> 1, m2 = 16 * (cb & 1) + c;
+      f16x8 b0, b1;
+      load_ops(ping + (k1 * 32 + m2) * PSTR + 8 * g, b0, b1);
+      f32x4 d[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) {
+        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        d[rb] = Mma<CT>::run(wh[rb * 2 + 0], wl[rb * 2 + 0], b0, z);
+        d[rb] = Mma<CT>::run(wh[rb * 2 + 1], wl[rb * 2 + 1], b1, d[rb]);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int j1 = 16 * h + 4 * g + i;
+          const float2 y = c_mul(make_float2(d[h][i], d[2 + h][i]), tb[j1 * 32 + m2]);  // tb carries the 2^-5
+          pong[(j1 * 16 + k1) * PSTR + m2] = pack_h2(y.x, y.y);
+        }
+    }
+    __syncthreads();
+
+    // ---- stage 3: DFT-32 over m2 for the 512 columns (j1, k1).  The re plane of
+    // the spectrum goes to the (free) stage-2 buffer at once; the im plane
+    // overlays the stage-3 operands and is held until every wave has read them.
+    f32x4 dim3[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int j1 = w + 8 * t;
+      f16x8 b0, b1;
+      load_ops(pong + (j1 * 16 + c) * PSTR + 8 * g, b0, b1);
+      f32x4 d[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) {
+        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        d[rb] = Mma<CT>::run(wh[rb * 2 + 0], wl[rb * 2 + 0], b0, z);
+        d[rb] = Mma<CT>::run(wh[rb * 2 + 1], wl[rb * 2 + 1], b1, d[rb]);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int j2 = 16 * h + 4 * g + i;
+          if constexpr (MODE == 0) {
+            gp(a.spec_out + (uint64_t)ch * N)[c + 16 * j1 + 512 * j2] =
+                make_float2(d[h][i] * s_out, d[2 + h][i] * s_out);
+          } else {
+            const int k = c + 16 * j1 + 512 * j2;
+            xre[k + (k >> 5)] = d[h][i] * s_out;
+          }
+        }
+      dim3[t][0] = d[2];
+      dim3[t][1] = d[3];
+    }
+    __syncthreads();  // every stage-3 operand has been read
+    if constexpr (MODE == 1) {
+      {
+        const uint32_t nx = ch + gridDim.x;
+        if (nx < a.batch) load_raw(a, nx, w, g, c, raw);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int j1 = w + 8 * t;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int j2 = 16 * h + 4 * g + i;
+            const int k = c + 16 * j1 + 512 * j2;
+            xim[k + (k >> 5)] = dim3[t][h][i] * s_out;
+          }
+      }
+      __syncthreads();
+      mixed_peaks(a, ch, xre, xim, red);
+      __syncthreads();  // the planes are overwritten by the next channel
+    }
+  }
+}
+template <int DT, int CT, int MODE>
+static hipError_t launch_mixed_t(const FusedArgs& a, int grid, hipStream_t st) {
+  static bool attr_done = false;
+  if (!attr_done) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_mixed16k<DT, CT, MODE>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, mx::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    attr_done = true;
+  }
+  hipLaunchKernelGGL((k_mixed16k<DT, CT, MODE>), dim3(grid), dim3(mx::T), mx::LDS_BYTES, st, a);
+  return hipGetLastError();
+}
+
+template <int DT, int CT>
+static hipError_t launch_mixed_m(const FusedArgs& a, int grid, hipStream_t st) {
+  return a.peak_mode != 0 ? launch_mixed_t<DT, CT, 1>(a, grid, st) : launch_mixed_t<DT, CT, 0>(a, grid, st);
+}
+
+}  // namespace spx
+
+extern "C" {
+
+uint64_t spx_mixed_table_bytes(void) { return (uint64_t)spx::mx::TABLE_BYTES; }
+
+// compute: 0 = fp16 MFMA, 1 = bf16 MFMA (split operands).  a->tw_a holds the
+// mixed-precision table block; a->nfft must be 16384 and a->dtype fp16/bf16;
+// grid = persistent workgroups (one per CU).
+hipError_t spx_launch_mixed(const FusedArgs* a, int compute, int grid, hipStream_t st) {
+  using namespace spx;
+  if (a->nfft != (uint32_t)mx::N || a->n_point != (uint32_t)mx::N || grid <= 0) return hipErrorInvalidValue;
+  if (a->dtype == SPX_DTYPE_C32F16)
+    return compute ? launch_mixed_m<SPX_DTYPE_C32F16, 1>(*a, grid, st) : launch_mixed_m<SPX_DTYPE_C32F16, 0>(*a, grid, st);
+  if (a->dtype == SPX_DTYPE_C32BF16)
+    return compute ? launch_mixed_m<SPX_DTYPE_C32BF16, 1>(*a, grid, st)
+                   : launch_mixed_m<SPX_DTYPE_C32BF16, 0>(*a, grid, st);
+  return hipErrorInvalidValue;
+}
+
+}  // extern "C"
