@@ -150,6 +150,37 @@ typedef struct spx_fused_desc {
 
 int spx_fused_peaks(spx_ctx* ctx, const spx_fused_desc* desc);
 
+/* ---- learned sub-bin refiner ---------------------------------------------
+ * No counterpart in the reference (its refinement is parabolic only,
+ * include/spectrum/kernels/fft_kernel_sources_rocm.hpp:228-239).  A
+ * 10 -> 64 -> 64 -> 1 ReLU network maps the 5-bin complex neighbourhood of a
+ * peak (X[k+j]/X[k], j = -2..2) to its offset in bins.  Training runs on the
+ * GPU (own forward/backward and Adam kernels, FP32 master weights); the
+ * deployed block holds FP16 weights for MFMA inference with FP32
+ * accumulation.  With spx_set_refiner, spx_fused_peaks accepts
+ * refine = SPX_REFINE_LEARNED (threshold search, nfft 1024..16384,
+ * max_peaks <= 64); predictions outside [-0.5, 0.5] fall back to parabolic. */
+int spx_mlp_param_count(void);    /* 4929 FP32 master parameters */
+int spx_mlp_deployed_size(void);  /* bytes of the deployed block */
+int spx_mlp_init(spx_ctx* ctx, float* params, uint64_t seed);
+/* grad: param_count + 1 floats (d mean-squared-error / d params, then the loss) */
+int spx_mlp_grad(spx_ctx* ctx, const float* params, const float* X, const float* Y, uint32_t n, float* grad);
+int spx_mlp_train(spx_ctx* ctx, float* params, float* adam_m, float* adam_v, float* grad_buf, const float* X,
+                  const float* Y, uint32_t n, uint32_t batch, uint32_t steps, uint32_t step0, float lr,
+                  uint64_t seed, float* loss_out_host);
+int spx_mlp_export(spx_ctx* ctx, const float* params, void* deployed_out);
+int spx_mlp_predict(spx_ctx* ctx, const void* deployed, const float* X, uint32_t n, float* out);
+int spx_set_refiner(spx_ctx* ctx, const void* deployed);
+/* Features, zero-padded-FFT labels and parabolic offsets of stored peaks:
+ * spec [batch][nfft] complex64 (windowed), mag16 [batch][16 nfft] |X| of the
+ * 16x zero-padded transform of the same windowed signal, peaks/counts from
+ * spx_fused_peaks; outputs per (channel, slot): feat [..][10], label, parab,
+ * valid (1 when the slot holds a peak whose padded maximum lies inside
+ * +-11 fine bins of 16k). */
+int spx_mlp_dataset(spx_ctx* ctx, const void* spec, const float* mag16, const spx_max_value* peaks,
+                    const uint32_t* counts, uint32_t batch, uint32_t nfft, uint32_t max_peaks, float* feat,
+                    float* label, float* parab, uint32_t* valid);
+
 /* ---- mixed precision: FP16/BF16 storage, DFT-as-GEMM on the matrix cores --
  * No counterpart in the reference (its FFT is hipFFT FP32 C2C,
  * src/fft_func/src/fft_processor_rocm.cpp:353).  N = 16384 = 16 x 32 x 32:

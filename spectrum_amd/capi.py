@@ -81,6 +81,15 @@ SIGNATURES = [
     ("spx_peaks_top", _I, [_P, _P, _P, _U32, _U32, _U32, _F, _I]),
     ("spx_all_maxima", _I, [_P, _P, _P, _P, _U32, _U32, _F, _U32, _U32, _U32]),
     ("spx_fused_peaks", _I, [_P, C.POINTER(SpxFusedDesc)]),
+    ("spx_mlp_param_count", _I, []),
+    ("spx_mlp_deployed_size", _I, []),
+    ("spx_mlp_init", _I, [_P, _P, _U64]),
+    ("spx_mlp_grad", _I, [_P, _P, _P, _P, _U32, _P]),
+    ("spx_mlp_train", _I, [_P, _P, _P, _P, _P, _P, _P, _U32, _U32, _U32, _U32, _F, _U64, _P]),
+    ("spx_mlp_export", _I, [_P, _P, _P]),
+    ("spx_mlp_predict", _I, [_P, _P, _P, _U32, _P]),
+    ("spx_set_refiner", _I, [_P, _P]),
+    ("spx_mlp_dataset", _I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, _P, _P, _P, _P]),
     ("spx_fft_mixed", _I, [_P, _P, _P, _U32, _U32, _U32, _U32, _U32]),
     ("spx_fused_peaks_mixed", _I, [_P, C.POINTER(SpxFusedDesc), _U32]),
     ("spx_batch_plan_create", _I, [_P, C.POINTER(SpxBatchDesc), C.POINTER(C.c_void_p)]),
@@ -236,6 +245,34 @@ class Context:
         d.gmax_out, d.stats_out = _ptr(gmax), _ptr(stats)
         self.check(self.lib.spx_fused_peaks(self.h, C.byref(d)), "spx_fused_peaks")
 
+
+    # -- learned sub-bin refiner
+    def set_refiner(self, deployed):
+        self.check(self.lib.spx_set_refiner(self.h, _ptr(deployed)), "spx_set_refiner")
+
+    def mlp_init(self, params, seed):
+        self.check(self.lib.spx_mlp_init(self.h, _ptr(params), seed), "spx_mlp_init")
+
+    def mlp_grad(self, params, X, Y, n, grad):
+        self.check(self.lib.spx_mlp_grad(self.h, _ptr(params), _ptr(X), _ptr(Y), n, _ptr(grad)), "spx_mlp_grad")
+
+    def mlp_train(self, params, m, v, grad, X, Y, n, batch, steps, step0, lr, seed):
+        losses = np.zeros(steps, dtype=np.float32)
+        self.check(self.lib.spx_mlp_train(self.h, _ptr(params), _ptr(m), _ptr(v), _ptr(grad), _ptr(X), _ptr(Y), n,
+                                          batch, steps, step0, float(lr), seed, losses.ctypes.data),
+                   "spx_mlp_train")
+        return losses
+
+    def mlp_export(self, params, deployed):
+        self.check(self.lib.spx_mlp_export(self.h, _ptr(params), _ptr(deployed)), "spx_mlp_export")
+
+    def mlp_predict(self, deployed, X, n, out):
+        self.check(self.lib.spx_mlp_predict(self.h, _ptr(deployed), _ptr(X), n, _ptr(out)), "spx_mlp_predict")
+
+    def mlp_dataset(self, spec, mag16, peaks, counts, batch, nfft, max_peaks, feat, label, parab, valid):
+        self.check(self.lib.spx_mlp_dataset(self.h, _ptr(spec), _ptr(mag16), _ptr(peaks), _ptr(counts), batch, nfft,
+                                            max_peaks, _ptr(feat), _ptr(label), _ptr(parab), _ptr(valid)),
+                   "spx_mlp_dataset")
 
     # -- mixed precision (fp16 / bf16 storage, MFMA DFT-as-GEMM), N = 16384
     def fft_mixed(self, inp, out, batch, nfft=16384, window=1, dtype=DTYPE_C32F16, compute=COMPUTE_F16):

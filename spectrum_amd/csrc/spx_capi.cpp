@@ -304,6 +304,7 @@ int spx_destroy(spx_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->tables) (void)hipFree(kv.second);
   if (c->scratch) (void)hipFree(c->scratch);
+  if (c->mlp_partial) (void)hipFree(c->mlp_partial);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return SPX_OK;
@@ -443,7 +444,11 @@ int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a) {
   if (d->peak_mode != SPX_PEAKS_THRESHOLD && d->peak_mode != SPX_PEAKS_ALL_MAXIMA)
     return fail(c, SPX_EINVAL, "unknown peak_mode");
   if (d->refine > SPX_REFINE_LEARNED) return fail(c, SPX_EINVAL, "unknown refine mode");
-  if (d->refine == SPX_REFINE_LEARNED) return fail(c, SPX_ENOTSUP, "learned refinement needs spx_fused_peaks_mlp");
+  if (d->refine == SPX_REFINE_LEARNED) {
+    if (!c->refiner) return fail(c, SPX_EINVAL, "learned refinement requested but no refiner is set (spx_set_refiner)");
+    if (d->peak_mode != SPX_PEAKS_THRESHOLD || d->nfft < 1024 || d->nfft > 16384 || d->max_peaks > 64)
+      return fail(c, SPX_ENOTSUP, "learned refinement: threshold search, nfft 1024..16384 and max_peaks <= 64 only");
+  }
   uint32_t s0 = d->search_start, s1 = d->search_end;
   if (d->peak_mode == SPX_PEAKS_ALL_MAXIMA) {
     // reference range normalisation, all_maxima_pipeline_rocm.cpp:274-276
@@ -474,6 +479,7 @@ int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a) {
   a.counts = d->counts_out;
   a.gmax = d->gmax_out;
   a.stats = d->stats_out;
+  a.mlp = c->refiner;
   return SPX_OK;
 }
 
@@ -609,5 +615,142 @@ int spx_fft_mixed(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_
   return run_mixed(c, a, window, compute);
 }
 
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned sub-bin refiner: data set, training, export, inference
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int spx_mlp_param_count(void) { return spx_mlp_nparam(); }
+int spx_mlp_deployed_size(void) { return spx_mlp_deployed_bytes(); }
+
+int spx_set_refiner(spx_ctx* c, const void* deployed) {
+  if (!c) return SPX_EINVAL;
+  c->refiner = deployed;
+  return SPX_OK;
+}
+
+// He-uniform initial weights (bound sqrt(6 / fan_in)), zero biases, from a
+// SplitMix64 stream; written to params_dev (FP32 master layout).
+int spx_mlp_init(spx_ctx* c, float* params_dev, uint64_t seed) {
+  if (!c || !params_dev) return SPX_EINVAL;
+  const int P = spx_mlp_nparam();
+  std::vector<float> p((size_t)P, 0.0f);
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  auto uni = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+  };
+  const int IN = 10, H = 64;
+  int o = 0;
+  for (int i = 0; i < H * IN; ++i) p[o++] = (float)(uni() * std::sqrt(6.0 / IN));
+  o += H;  // b1
+  for (int i = 0; i < H * H; ++i) p[o++] = (float)(uni() * std::sqrt(6.0 / H));
+  o += H;  // b2
+  for (int i = 0; i < H; ++i) p[o++] = (float)(uni() * std::sqrt(6.0 / H));
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, hipMemcpyAsync(params_dev, p.data(), p.size() * 4, hipMemcpyHostToDevice, c->stream));
+  SPX_HIP(c, hipStreamSynchronize(c->stream));
+  return SPX_OK;
+}
+
+static int mlp_groups(spx_ctx* c, uint32_t n) {
+  int g = c->cu_count > 0 ? c->cu_count : 256;
+  const int steps = (int)((n + 63) / 64);
+  return std::max(1, std::min(g, steps));
+}
+
+static int mlp_partial(spx_ctx* c, int groups) {
+  const uint64_t need = (uint64_t)groups * (uint64_t)(spx_mlp_nparam() + 1) * 4;
+  if (c->mlp_partial_bytes >= need) return SPX_OK;
+  SPX_HIP(c, hipSetDevice(c->device));
+  if (c->mlp_partial) {
+    SPX_HIP(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->mlp_partial);
+    c->mlp_partial = nullptr;
+    c->mlp_partial_bytes = 0;
+  }
+  if (hipMalloc(&c->mlp_partial, need) != hipSuccess) return fail(c, SPX_ENOMEM, "mlp partial buffer");
+  c->mlp_partial_bytes = need;
+  return SPX_OK;
+}
+
+// grad_out: NPARAM + 1 floats (gradient of the mean squared error, then the loss)
+int spx_mlp_grad(spx_ctx* c, const float* params, const float* X, const float* Y, uint32_t n, float* grad_out) {
+  if (!c) return SPX_EINVAL;
+  if (!params || !X || !Y || !grad_out || n == 0) return fail(c, SPX_EINVAL, "spx_mlp_grad: bad arguments");
+  const int g = mlp_groups(c, n);
+  int rc = mlp_partial(c, g);
+  if (rc) return rc;
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_mlp_grad(params, X, Y, n, (float*)c->mlp_partial, g, grad_out, c->stream));
+  return SPX_OK;
+}
+
+// Adam on mini-batches: step s (1-based, continuing from step0) uses samples
+// [off, off + batch) with off from a SplitMix64 stream (seed, s); grad_buf
+// holds NPARAM + 1 floats; loss_out (host, optional) receives each step's loss.
+int spx_mlp_train(spx_ctx* c, float* params, float* m, float* v, float* grad_buf, const float* X, const float* Y,
+                  uint32_t n, uint32_t batch, uint32_t steps, uint32_t step0, float lr, uint64_t seed,
+                  float* loss_out) {
+  if (!c) return SPX_EINVAL;
+  if (!params || !m || !v || !grad_buf || !X || !Y || n == 0 || batch == 0 || batch > n)
+    return fail(c, SPX_EINVAL, "spx_mlp_train: bad arguments");
+  const int g = mlp_groups(c, batch);
+  int rc = mlp_partial(c, g);
+  if (rc) return rc;
+  SPX_HIP(c, hipSetDevice(c->device));
+  const int P = spx_mlp_nparam();
+  for (uint32_t s = 1; s <= steps; ++s) {
+    uint64_t z = (seed + (uint64_t)(step0 + s) * 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const uint32_t off = (uint32_t)(z % (uint64_t)(n - batch + 1));
+    SPX_HIP(c, spx_launch_mlp_grad(params, X + (uint64_t)off * 10, Y + off, batch, (float*)c->mlp_partial, g,
+                                   grad_buf, c->stream));
+    SPX_HIP(c, spx_launch_mlp_adam(params, grad_buf, m, v, lr, 0.9f, 0.999f, 1e-8f, (int)(step0 + s), c->stream));
+    if (loss_out) {
+      SPX_HIP(c, hipMemcpyAsync(loss_out + (s - 1), grad_buf + P, 4, hipMemcpyDeviceToHost, c->stream));
+    }
+  }
+  SPX_HIP(c, hipStreamSynchronize(c->stream));
+  return SPX_OK;
+}
+
+int spx_mlp_export(spx_ctx* c, const float* params, void* deployed_out) {
+  if (!c) return SPX_EINVAL;
+  if (!params || !deployed_out) return fail(c, SPX_EINVAL, "spx_mlp_export: null pointer");
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_mlp_export(params, deployed_out, c->stream));
+  return SPX_OK;
+}
+
+int spx_mlp_predict(spx_ctx* c, const void* deployed, const float* X, uint32_t n, float* out) {
+  if (!c) return SPX_EINVAL;
+  if (!deployed || !X || !out) return fail(c, SPX_EINVAL, "spx_mlp_predict: null pointer");
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_mlp_predict(deployed, X, n, out, c->stream));
+  return SPX_OK;
+}
+
+int spx_mlp_dataset(spx_ctx* c, const void* spec, const float* mag16, const spx_max_value* peaks,
+                    const uint32_t* counts, uint32_t batch, uint32_t nfft, uint32_t max_peaks, float* feat,
+                    float* label, float* parab, uint32_t* valid) {
+  if (!c) return SPX_EINVAL;
+  if (!spec || !mag16 || !peaks || !counts || !feat || !label || !parab || !valid)
+    return fail(c, SPX_EINVAL, "spx_mlp_dataset: null pointer");
+  if (nfft < 8 || (nfft & (nfft - 1))) return fail(c, SPX_EINVAL, "spx_mlp_dataset: nfft must be a power of two");
+  SPX_HIP(c, hipSetDevice(c->device));
+  SPX_HIP(c, spx_launch_mlp_dataset(spec, mag16, peaks, counts, batch, nfft, max_peaks, feat, label, parab, valid,
+                                    c->stream));
+  return SPX_OK;
+}
 
 }  // extern "C"

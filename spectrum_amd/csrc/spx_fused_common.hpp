@@ -96,10 +96,8 @@ __device__ __forceinline__ float parabolic(float yl, float yc, float yr) {
   return d;
 }
 
-// One MaxValue record; bin_hz = fs / nfft.
-__device__ __forceinline__ void write_record_hz(MaxValue* dst, uint32_t k, float2 x, float yl, float yc, float yr,
-                                                uint32_t refine, float bin_hz) {
-  float d = (refine != 0) ? parabolic(yl, yc, yr) : 0.0f;
+// One MaxValue record with sub-bin offset d; bin_hz = fs / nfft.
+__device__ __forceinline__ void write_record_d(MaxValue* dst, uint32_t k, float2 x, float yc, float d, float bin_hz) {
   MaxValue mv;
   mv.index = k;
   mv.real = x.x;
@@ -110,6 +108,11 @@ __device__ __forceinline__ void write_record_hz(MaxValue* dst, uint32_t k, float
   mv.refined_frequency = ((float)k + d) * bin_hz;
   mv.pad = 0;
   *gp(dst) = mv;
+}
+
+__device__ __forceinline__ void write_record_hz(MaxValue* dst, uint32_t k, float2 x, float yl, float yc, float yr,
+                                                uint32_t refine, float bin_hz) {
+  write_record_d(dst, k, x, yc, (refine != 0) ? parabolic(yl, yc, yr) : 0.0f, bin_hz);
 }
 
 __device__ __forceinline__ void write_record(MaxValue* dst, uint32_t k, float2 x, float yl, float yc, float yr,

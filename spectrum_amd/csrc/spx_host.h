@@ -21,6 +21,9 @@ struct spx_ctx {
   std::map<std::tuple<int, uint32_t, uint32_t>, void*> tables;  // (kind, a, b) -> device table
   void* scratch = nullptr;
   uint64_t scratch_bytes = 0;
+  const void* refiner = nullptr;  // deployed learned-refiner block (caller-owned device memory)
+  void* mlp_partial = nullptr;    // per-workgroup gradient partials
+  uint64_t mlp_partial_bytes = 0;
 };
 
 namespace spx_host {

@@ -79,6 +79,17 @@ struct BatchArgs {
 extern "C" {
 hipError_t spx_batch_kernel_init(void);
 uint64_t spx_mixed_table_bytes(void);
+int spx_mlp_nparam(void);
+int spx_mlp_deployed_bytes(void);
+hipError_t spx_launch_mlp_dataset(const void* spec, const float* mag16, const void* peaks, const uint32_t* counts,
+                                  uint32_t B, uint32_t N, uint32_t K, float* feat, float* label, float* parab,
+                                  uint32_t* valid, hipStream_t st);
+hipError_t spx_launch_mlp_grad(const float* params, const float* X, const float* Y, uint32_t n, float* partial,
+                               int groups, float* grad, hipStream_t st);
+hipError_t spx_launch_mlp_adam(float* params, const float* grad, float* m, float* v, float lr, float b1, float b2,
+                               float eps, int step, hipStream_t st);
+hipError_t spx_launch_mlp_export(const float* params, void* dep, hipStream_t st);
+hipError_t spx_launch_mlp_predict(const void* dep, const float* X, uint32_t n, float* out, hipStream_t st);
 hipError_t spx_launch_mixed(const FusedArgs* a, int compute, int grid, hipStream_t st);
 hipError_t spx_launch_batch(const BatchArgs* b, int grid, hipStream_t st);
 hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st);

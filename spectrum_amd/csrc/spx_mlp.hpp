@@ -1,0 +1,108 @@
+// Learned sub-bin refiner: a 10 -> 64 -> 64 -> 1 ReLU network that maps the
+// 5-bin complex neighbourhood of a spectral peak to its frequency offset in
+// bins.  Shared by the training kernels (spx_mlp.hip) and the fused peak
+// kernel, so training-time evaluation and fused inference run the same code.
+//
+// Input features (10 reals): z_j = X[k+j] / X[k], j = -2..2 (circular), as
+// (re, im) pairs in order j = -2, -1, 0, 1, 2; dividing by the complex peak
+// value normalises magnitude and phase (z_0 = 1 + 0i).
+//
+// Parameters (FP32 master copy, 4929 values): W1[64][10] b1[64] W2[64][64]
+// b2[64] w3[64] b3.  Deployed copy (inference): FP16 MFMA B-operand
+// fragments of W1^T (K padded 10 -> 32) and W2^T plus FP16-valued biases and
+// w3 held as float; FP32 accumulation (v_mfma_f32_16x16x32_f16).
+#pragma once
+#include "spx_device.hpp"
+
+namespace spx {
+namespace mlp {
+constexpr int IN = 10, H = 64;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + H * IN, OFF_W2 = OFF_B1 + H, OFF_B2 = OFF_W2 + H * H,
+              OFF_W3 = OFF_B2 + H, OFF_B3 = OFF_W3 + H, NPARAM = OFF_B3 + 1;  // 4929
+// deployed block (bytes)
+constexpr int D_W1 = 0;                      // [cb 4][lane 64][8 x fp16]
+constexpr int D_W2 = D_W1 + 4 * 64 * 16;     // [ks 2][cb 4][lane 64][8 x fp16]
+constexpr int D_B1 = D_W2 + 8 * 64 * 16;     // float[64]
+constexpr int D_B2 = D_B1 + 256;
+constexpr int D_W3 = D_B2 + 256;
+constexpr int D_B3 = D_W3 + 256;             // float, padded to 16 B
+constexpr int D_BYTES = D_B3 + 16;
+}  // namespace mlp
+
+typedef _Float16 mlp_h8 __attribute__((ext_vector_type(8)));
+typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+
+// features of the peak at k from any accessor X(k) (indices already circular)
+template <class XF>
+__device__ __forceinline__ void mlp_features(XF X, uint32_t k, uint32_t n, float* f) {
+  const float2 c = X(k);
+  const float den = fmaf(c.x, c.x, c.y * c.y);
+  const float inv = den > 0.f ? 1.0f / den : 0.f;
+#pragma unroll
+  for (int j = -2; j <= 2; ++j) {
+    const float2 x = X((k + n + (uint32_t)j) % n);
+    // x * conj(c) / |c|^2
+    f[2 * (j + 2) + 0] = fmaf(x.x, c.x, x.y * c.y) * inv;
+    f[2 * (j + 2) + 1] = fmaf(x.y, c.x, -x.x * c.y) * inv;
+  }
+}
+
+// One wave evaluates the network for 16 rows.  feat: 16 rows x 16 halves in
+// LDS (features in columns 0..9, zeros in 10..15; K columns 16..31 of the
+// MFMA operand are zero), hbuf: 16 x 64 halves of LDS scratch; out[row]
+// receives the offsets (rows >= the caller's count are computed but not
+// meaningful).  All 64 lanes of the wave must call it.
+__device__ __forceinline__ void mlp_rows16(const _Float16* feat, _Float16* hbuf, float* out,
+                                           const unsigned char* wts) {
+  using namespace mlp;
+  const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+  const mlp_h8* W1F = reinterpret_cast<const mlp_h8*>(wts + D_W1);
+  const mlp_h8* W2F = reinterpret_cast<const mlp_h8*>(wts + D_W2);
+  const float* B1 = reinterpret_cast<const float*>(wts + D_B1);
+  const float* B2 = reinterpret_cast<const float*>(wts + D_B2);
+  const float* W3 = reinterpret_cast<const float*>(wts + D_W3);
+  const float b3 = gp(reinterpret_cast<const float*>(wts + D_B3))[0];
+  // layer 1: A = feat (row l%16, k = 8g + j), B = W1^T fragments
+  mlp_h8 a1 = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (g < 2) a1 = *reinterpret_cast<const mlp_h8*>(feat + c * 16 + 8 * g);
+  mlp_f4 h[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const mlp_f4 z = {0.f, 0.f, 0.f, 0.f};
+    h[cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, gp(W1F)[cb * 64 + l], z, 0, 0, 0);
+  }
+  // lane holds h[cb][i] = H1[row 4g+i][unit 16cb+c]: bias, ReLU, stage as fp16 rows
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const float bb = gp(B1)[16 * cb + c];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) hbuf[(4 * g + i) * 64 + 16 * cb + c] = (_Float16)fmaxf(h[cb][i] + bb, 0.f);
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // LDS writes of this wave complete before the reads below
+  __builtin_amdgcn_wave_barrier();
+  // layer 2: A = H1 rows (two K-steps of 32), B = W2^T fragments
+  const mlp_h8 a20 = *reinterpret_cast<const mlp_h8*>(hbuf + c * 64 + 8 * g);
+  const mlp_h8 a21 = *reinterpret_cast<const mlp_h8*>(hbuf + c * 64 + 32 + 8 * g);
+  float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const mlp_f4 z = {0.f, 0.f, 0.f, 0.f};
+    mlp_f4 d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a20, gp(W2F)[(0 * 4 + cb) * 64 + l], z, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a21, gp(W2F)[(1 * 4 + cb) * 64 + l], d, 0, 0, 0);
+    const float bb = gp(B2)[16 * cb + c], ww = gp(W3)[16 * cb + c];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) part[i] = fmaf(fmaxf(d[i] + bb, 0.f), ww, part[i]);
+  }
+  // layer 3: sum the 16 unit-lanes of each row group
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) part[i] += __shfl_xor(part[i], o, 64);
+  }
+  if (c == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[4 * g + i] = part[i] + b3;
+  }
+}
+
+}  // namespace spx

@@ -1,0 +1,118 @@
+"""GPU checks of the learned sub-bin refiner.
+
+* the forward/backward kernel's gradient and loss equal the FP64 oracle's
+  (which is itself pinned by finite differences on the CPU);
+* deployed FP16 inference equals the oracle forward pass of the FP16-rounded
+  weights to FP16 accuracy;
+* features and labels of detected peaks equal the oracle's, and the labels
+  agree with the generator's true tone frequencies;
+* a network trained on the GPU beats parabolic refinement on held-out peaks,
+  and the fused peak kernel with refine = LEARNED reproduces its predictions.
+"""
+import numpy as np
+import pytest
+
+from spectrum_amd import capi
+from oracle import mlp as omlp
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_gradient_matches_oracle(ctx):
+    from spectrum_amd.refiner import Refiner
+
+    rng = np.random.default_rng(3)
+    r = Refiner(ctx, seed=11)
+    p = r.get_params().astype(np.float64)
+    p[omlp.OFF_B1:omlp.OFF_W2] = rng.normal(0, 0.1, omlp.H)  # non-zero biases exercise every term
+    p[omlp.OFF_B2:omlp.OFF_W3] = rng.normal(0, 0.1, omlp.H)
+    p[omlp.OFF_B3] = 0.05
+    r.set_params(p.astype(np.float32))
+    p32 = r.get_params().astype(np.float64)
+    n = 3000
+    X = rng.normal(0, 0.7, (n, 10)).astype(np.float32)
+    Y = rng.uniform(-0.5, 0.5, n).astype(np.float32)
+    dX, dY = ctx.to_device(X), ctx.to_device(Y)
+    g = ctx.alloc((omlp.NPARAM + 1) * 4)
+    ctx.mlp_grad(r.params, dX, dY, n, g)
+    ctx.sync()
+    got = g.download(np.float32, (omlp.NPARAM + 1,))
+    ref = omlp.grad(p32, X.astype(np.float64), Y.astype(np.float64))
+    ref_loss = omlp.loss(p32, X.astype(np.float64), Y.astype(np.float64))
+    assert abs(got[-1] - ref_loss) <= 1e-5 * ref_loss
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(got[:-1] - ref)) <= 2e-5 * scale, np.max(np.abs(got[:-1] - ref)) / scale
+
+
+def test_mlp_deployed_inference_matches_oracle(ctx):
+    from spectrum_amd.refiner import Refiner
+
+    rng = np.random.default_rng(4)
+    r = Refiner(ctx, seed=12)
+    r.export()
+    X = rng.normal(0, 0.5, (1000, 10)).astype(np.float32)
+    got = r.predict(X)
+    p = r.get_params().astype(np.float16).astype(np.float64)
+    Xh = X.astype(np.float16).astype(np.float64)
+    ref, _ = omlp.forward(p, Xh)
+    err = np.abs(got - ref)
+    assert np.max(err) <= 5e-3 * (1.0 + np.abs(ref).max()), np.max(err)
+
+
+def test_dataset_features_and_labels(ctx):
+    from spectrum_amd.refiner import make_peaks
+
+    N, K, B = 8192, 64, 6
+    ps = make_peaks(ctx, B, 0x5EED0105, N=N, K=K, chunk=B)
+    assert len(ps) > 10
+    x = capi.siggen(capi.GEN_FRACTIONAL, 0x5EED0105, B, N)
+    X64 = orc.spectrum64(x, N, 1)
+    w = orc.window(1, N)
+    checked = 0
+    for i in range(0, len(ps), max(1, len(ps) // 12)):
+        c, k = int(ps.channel[i]), int(ps.bin[i])
+        Xc = X64[c]
+        assert np.allclose(omlp.features(Xc, k), ps.feat[i], atol=2e-4)
+        xp = np.zeros(16 * N, np.complex128)
+        xp[:N] = x[c] * w
+        lab = omlp.label(np.abs(orc.fft64(xp)), k)
+        assert lab is not None and abs(lab - ps.label[i]) < 2e-3
+        checked += 1
+    assert checked >= 5
+    ok = np.isfinite(ps.truth)
+    assert np.sqrt(np.mean((ps.label[ok] - ps.truth[ok]) ** 2)) < 0.01
+
+
+def test_trained_refiner_beats_parabolic_and_fused_inference(ctx):
+    from spectrum_amd.refiner import Refiner, make_peaks, rmse
+
+    N, K = 8192, 64
+    train = make_peaks(ctx, 6144, 0x5EED1005, N=N, K=K)
+    test = make_peaks(ctx, 512, 0x5EED0005, N=N, K=K)
+    print(f"[gpu] refiner data: {len(train)} training peaks, {len(test)} held-out", flush=True)
+    r = Refiner(ctx, seed=5)
+    losses = r.train(train.feat, train.label, steps=4000, batch=1024, lr=2e-3)
+    assert losses[-200:].mean() < 0.5 * losses[:50].mean()
+    r.export()
+    pred = r.predict(test.feat)
+    e_learn, e_par = rmse(pred, test.label), rmse(test.parab, test.label)
+    print(f"[gpu] held-out RMSE vs 16x padded label: learned {e_learn:.4f} bin, parabolic {e_par:.4f} bin",
+          flush=True)
+    assert e_learn < 0.7 * e_par
+    assert e_learn < 0.03
+    # fused kernel with the learned refiner: its records carry the same predictions
+    B = 512
+    x = capi.siggen(capi.GEN_FRACTIONAL, 0x5EED0005, B, N)
+    d_in = ctx.to_device(x)
+    pk, ct = ctx.alloc(B * K * 32), ctx.alloc(B * 8)
+    ctx.set_refiner(r.deployed)
+    ctx.fused_peaks(d_in, B, N, N, pk, ct, window=1, nsigma=6.0, max_peaks=K, refine=capi.REFINE_LEARNED)
+    ctx.sync()
+    ctx.set_refiner(None)
+    recs = pk.download(capi.MAXVALUE_DTYPE, (B, K))
+    got = recs["freq_offset"][test.channel, test.slot]
+    use = np.abs(pred) <= 0.5
+    assert np.max(np.abs(got[use] - pred[use])) < 2e-3
+    assert np.allclose(recs["refined_frequency"][test.channel, test.slot],
+                       (recs["index"][test.channel, test.slot] + got) / N, rtol=1e-5, atol=1e-7)
