@@ -29,6 +29,7 @@
 // FP64 on the host and rounded once.  The window is evaluated in FP32 from
 // its cosine coefficients (hardware cosine in revolutions).
 #include "spx_fused_common.hpp"
+#include "spx_mlp.hpp"
 
 namespace spx {
 
@@ -48,6 +49,7 @@ constexpr int TAL_LDS = TB_LDS + 1024 * 8;        // W_16384^e, e < 128
 constexpr int TAH_LDS = TAL_LDS + 128 * 8;        // W_128^e, e < 128
 constexpr int LDS_BYTES = TAH_LDS + 128 * 8;
 static_assert(XPAD * 4 <= PING_BYTES && XPAD * 4 <= PONG_BYTES, "FP32 spectrum planes fit the operand buffers");
+static_assert(XPAD * 4 + 4352 <= PING_BYTES, "refiner staging fits behind the re plane");
 // global table block (bytes): TB [32][32] float2 | TAL [128] float2 | TAH [128] float2 | W16 frags | W32 frags
 constexpr int TB_OFF = 0;
 constexpr int TAL_OFF = TB_OFF + 1024 * 8;
@@ -130,9 +132,12 @@ __device__ __forceinline__ void load_raw(const FusedArgs& a, uint32_t ch, int w,
 }
 
 // Threshold peak search over the FP32 spectrum planes (bin k at k + k/32);
-// thread t owns natural bins 32t .. 32t+31.
+// thread t owns natural bins 32t .. 32t+31.  LEARN: stored peaks are refined
+// by the learned refiner (a.mlp) evaluated on MFMA by wave 0 from features
+// staged in `stage` (>= 4352 bytes of LDS), parabolic as the fallback.
+template <bool LEARN>
 __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, const float* xre, const float* xim,
-                                            BlockRed& red) {
+                                            BlockRed& red, unsigned char* stage) {
   constexpr int N = mx::N, T = mx::T, E = N / T;
   const int tid = threadIdx.x;
   auto adr = [](uint32_t k) -> int { return (int)(k + (k >> 5)); };
@@ -173,6 +178,31 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
   }
   int total;
   const int off = block_excl_scan<T>(cpk, total, red);
+  [[maybe_unused]] float* lout = reinterpret_cast<float*>(stage + 4096);
+  if constexpr (LEARN) {
+    _Float16* lfeat = reinterpret_cast<_Float16*>(stage);         // 64 x 16 halves
+    _Float16* lh = reinterpret_cast<_Float16*>(stage + 2048);     // 16 x 64 halves
+    if (flags && off < (int)a.kcap) {
+      uint32_t fl = flags;
+      int pos = off;
+      while (fl && pos < (int)a.kcap) {
+        const int e = __builtin_ctz(fl);
+        fl &= fl - 1;
+        float f[mlp::IN];
+        mlp_features(xat, k0 + e, (uint32_t)N, f);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) lfeat[pos * 16 + j] = (_Float16)(j < mlp::IN ? f[j] : 0.f);
+        ++pos;
+      }
+    }
+    __syncthreads();
+    const int nrb = (min(total, (int)a.kcap) + 15) / 16;
+    if (tid < 64) {
+      for (int rb = 0; rb < nrb; ++rb)
+        mlp_rows16(lfeat + rb * 16 * 16, lh, lout + rb * 16, reinterpret_cast<const unsigned char*>(a.mlp));
+    }
+    __syncthreads();
+  }
   if (flags && off < (int)a.kcap) {
     MaxValue* out = a.peaks + (uint64_t)ch * a.kcap;
     int pos = off;
@@ -183,7 +213,13 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
       const float yc = mag_of(xat(k));
       const float yl = mag_of(xat((k + N - 1) % N));
       const float yr = mag_of(xat((k + 1) % N));
-      write_record(out + pos, k, xat(k), yl, yc, yr, a);
+      if constexpr (LEARN) {
+        const float dl = lout[pos];
+        const float d = (fabsf(dl) <= 0.5f) ? dl : parabolic(yl, yc, yr);
+        write_record_d(out + pos, k, xat(k), yc, d, a.fs / (float)a.nfft);
+      } else {
+        write_record(out + pos, k, xat(k), yl, yc, yr, a);
+      }
       ++pos;
     }
   }
@@ -209,7 +245,7 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
 // MODE 0: dense FP32 spectrum to a.spec_out; MODE 1: threshold peak search.
 // Persistent: workgroup b runs channels b, b + grid, ...; the next channel's
 // input is loaded while the current one goes through stages 2, 3 and the peaks.
-template <int DT, int CT, int MODE>
+template <int DT, int CT, int MODE, bool LEARN = false>
 __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
   using namespace mx;
   using WT = typename Mma<CT>::W;
@@ -410,27 +446,30 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
           }
       }
       __syncthreads();
-      mixed_peaks(a, ch, xre, xim, red);
+      // the spectrum planes leave 6 KiB at the end of the stage-2 buffer for the refiner staging
+      mixed_peaks<LEARN>(a, ch, xre, xim, red, smem + XPAD * 4);
       __syncthreads();  // the planes are overwritten by the next channel
     }
   }
 }
-template <int DT, int CT, int MODE>
+template <int DT, int CT, int MODE, bool LEARN>
 static hipError_t launch_mixed_t(const FusedArgs& a, int grid, hipStream_t st) {
   static bool attr_done = false;
   if (!attr_done) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_mixed16k<DT, CT, MODE>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_mixed16k<DT, CT, MODE, LEARN>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, mx::LDS_BYTES);
     if (e != hipSuccess) return e;
     attr_done = true;
   }
-  hipLaunchKernelGGL((k_mixed16k<DT, CT, MODE>), dim3(grid), dim3(mx::T), mx::LDS_BYTES, st, a);
+  hipLaunchKernelGGL((k_mixed16k<DT, CT, MODE, LEARN>), dim3(grid), dim3(mx::T), mx::LDS_BYTES, st, a);
   return hipGetLastError();
 }
 
 template <int DT, int CT>
 static hipError_t launch_mixed_m(const FusedArgs& a, int grid, hipStream_t st) {
-  return a.peak_mode != 0 ? launch_mixed_t<DT, CT, 1>(a, grid, st) : launch_mixed_t<DT, CT, 0>(a, grid, st);
+  if (a.peak_mode == 0) return launch_mixed_t<DT, CT, 0, false>(a, grid, st);
+  if (a.refine == 2) return launch_mixed_t<DT, CT, 1, true>(a, grid, st);
+  return launch_mixed_t<DT, CT, 1, false>(a, grid, st);
 }
 
 }  // namespace spx
