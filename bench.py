@@ -6,11 +6,19 @@ Workload (the metric's configuration): 1024 channels x 65536-point complex64,
 global-maximum record.  One step = one launch over the whole batch, input
 already resident in HBM.
 
-Prints one JSON line (see README of the driver contract): value = channels/s,
-roofline = algorithmic bytes / average kernel time vs 8 TB/s, cpu_baseline =
-the reference's CPU path (its pocketfft CpuFFT, compiled from the reference
-sources into oracle/_ref) plus the oracle's C peak rules, timed on a bounded
-sample on this host.
+Prints one JSON line: value = channels/s, roofline = algorithmic bytes /
+average kernel time vs 8 TB/s, cpu_baseline = the reference's CPU path (its
+pocketfft CpuFFT, compiled from the reference sources into oracle/_ref) plus
+the oracle's C peak rules, timed on a bounded sample on this host (one
+thread, plus a multi-core variant).  Extra objects on the same line (skipped
+with --no-extras):
+  uneven     4096 channels of lengths 2^10..2^17 in one hipGraph launch vs
+             per-length launches;
+  mixed_mlp  2048 x 16384: the FP16-MFMA + learned-refinement form against
+             the plain FP32 form (complex64, parabolic), with its spectral
+             error against FP64;
+  refiner    4096 x 8192 fractional tones: learned vs parabolic refinement
+             (time per launch, frequency RMSE against the true tones).
 """
 import argparse
 import ctypes as C
@@ -81,6 +89,136 @@ def cpu_baseline(x, nfft, K, budget_s=10.0):
     }
 
 
+def cpu_baseline_mt(x, nfft, K, threads, budget_s=8.0):
+    """Best-effort multi-core CPU path: pocketfft c2c over [B, N] with `threads` threads
+    (reference sources, oracle/_ref) + the oracle's peak rules over channels on a thread pool."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as orc
+
+    if not orc.have_reference():
+        return None
+    w = orc.window(1, nfft).astype(np.float32)
+    chunk = 64
+    done = 0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as pool:
+        for c0 in range(0, x.shape[0], chunk):
+            xw = (x[c0:c0 + chunk] * w).astype(np.complex64)
+            X = orc.ref_pocketfft_mt(xw, threads)
+            list(pool.map(lambda r: orc.fused_peaks(r, 1.0, nsigma=6.0, kcap=K), list(X)))
+            done += len(xw)
+            if time.perf_counter() - t0 > budget_s:
+                break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "channels/s", "cores": threads, "kind": "reference",
+            "sample": f"{done} channels x {nfft}: pocketfft c2c<float> with {threads} threads (reference sources), "
+                      f"then the peak rules on a {threads}-thread pool"}
+
+
+def _timer_ms(ctx, fn, reps=10, warm=3):
+    t = HipTimer(ctx.stream())
+    for _ in range(warm):
+        fn()
+    ctx.sync()
+    ts = []
+    for _ in range(reps):
+        t.start()
+        fn()
+        ts.append(t.stop_ms())
+    return float(np.median(ts))
+
+
+def extra_uneven(ctx, K=64):
+    from spectrum_amd import capi
+
+    n = 4096
+    L = capi.siggen_lengths(0x5EED0003, n)
+    off, total, groups = capi.length_major_offsets(L)
+    x = capi.siggen_uneven(capi.GEN_MULTI_TONE, 0x5EED0003, L, off, total)
+    d = ctx.to_device(x)
+    pk, ct = ctx.alloc(n * K * 32), ctx.alloc(n * 8)
+    plan = ctx.batch_plan(ctx.batch_desc(d, off, L, pk, ct, max_peaks=K))
+    t_batch = _timer_ms(ctx, plan.execute)
+    info = plan.info()
+    plan.close()
+    bufs = {Lx: (ctx.alloc(len(ids) * K * 32), ctx.alloc(len(ids) * 8)) for Lx, (o, ids) in groups.items()}
+
+    def seq():
+        for Lx, (o, ids) in sorted(groups.items()):
+            ctx.fused_peaks(d.ptr + 8 * o, len(ids), Lx, Lx, bufs[Lx][0], bufs[Lx][1], max_peaks=K)
+
+    t_seq = _timer_ms(ctx, seq)
+    alg = total * 8 + n * (K * 32 + 8)
+    return {"workload": f"{n} channels, lengths 2^10..2^17 log-uniform (complex64, {total * 8 / 1e9:.2f} GB), "
+                        f"Hann, mean+6std, K={K}", "batch_graph_ms": t_batch, "per_length_launches_ms": t_seq,
+            "speedup": t_seq / t_batch, "channels_per_s": n / (t_batch * 1e-3),
+            "achieved_GBps": alg / (t_batch * 1e-3) / 1e9, "plan": info}
+
+
+def extra_mixed_mlp(ctx, K=32):
+    from spectrum_amd import capi
+    from spectrum_amd.refiner import DEFAULT_WEIGHTS, Refiner
+    from oracle import oracle as orc
+
+    B, N = 2048, 16384
+    x = capi.siggen(capi.GEN_DYNAMIC, 0x5EED0004, B, N)
+    d64 = ctx.to_device(x)
+    u = capi.to_half_storage(x, capi.DTYPE_C32F16)
+    d16 = ctx.to_device(u)
+    pk, ct = ctx.alloc(B * K * 32), ctx.alloc(B * 8)
+    t_fp32 = _timer_ms(ctx, lambda: ctx.fused_peaks(d64, B, N, N, pk, ct, max_peaks=K))
+    r = Refiner(ctx).load(DEFAULT_WEIGHTS)
+    r.export()
+    ctx.set_refiner(r.deployed)
+    t_mix = _timer_ms(ctx, lambda: ctx.fused_peaks_mixed(d16, B, N, pk, ct, max_peaks=K,
+                                                         refine=capi.REFINE_LEARNED))
+    t_mix_par = _timer_ms(ctx, lambda: ctx.fused_peaks_mixed(d16, B, N, pk, ct, max_peaks=K))
+    ctx.set_refiner(None)
+    # spectral error on a sample of channels
+    S = 16
+    out = ctx.alloc(S * N * 8)
+    ctx.fft_mixed(d16, out, S, N)
+    ctx.sync()
+    X = out.download(np.complex64, (S, N))
+    ref = orc.spectrum64(capi.from_half_storage(u[:S], capi.DTYPE_C32F16), N, 1)
+    err = np.linalg.norm(X - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return {"workload": f"{B} x {N}, strong tone + tone at -60 dB + noise at -80 dB, Hann, mean+6std, K={K}",
+            "fp32_c64_parabolic_ms": t_fp32, "mfma_fp16_learned_ms": t_mix, "mfma_fp16_parabolic_ms": t_mix_par,
+            "speedup_vs_fp32": t_fp32 / t_mix, "spectral_rel_l2_max": float(err.max()),
+            "spectral_rel_l2_mean": float(err.mean()), "error_reference": "FP64 FFT of the fp16-stored input"}
+
+
+def extra_refiner(ctx, K=64):
+    from spectrum_amd import capi
+    from spectrum_amd.refiner import DEFAULT_WEIGHTS, Refiner
+
+    B, N = 4096, 8192
+    x, tf, ta = capi.siggen(capi.GEN_FRACTIONAL, 0x5EED0005, B, N, max_tones=16, with_tones=True)
+    d = ctx.to_device(x)
+    pk, ct = ctx.alloc(B * K * 32), ctx.alloc(B * 8)
+    r = Refiner(ctx).load(DEFAULT_WEIGHTS)
+    r.export()
+    out = {"workload": f"{B} x {N} complex64, 1-16 fractional-bin tones, Hann, mean+6std, K={K}"}
+    for name, refine in (("learned", capi.REFINE_LEARNED), ("parabolic", capi.REFINE_PARABOLIC)):
+        ctx.set_refiner(r.deployed if refine == capi.REFINE_LEARNED else None)
+        out[f"{name}_ms"] = _timer_ms(ctx, lambda: ctx.fused_peaks(d, B, N, N, pk, ct, max_peaks=K, refine=refine))
+        recs = pk.download(capi.MAXVALUE_DTYPE, (B, K))
+        cnt = ct.download(np.uint32, (B, 2))[:, 0]
+        errs = []
+        for b in range(B):
+            f = recs["refined_frequency"][b, : cnt[b]].astype(np.float64) * N
+            tb = np.mod(tf[b][ta[b] > 0] * N, N)
+            if len(f) and len(tb):
+                dd = (f[:, None] - tb[None, :] + N / 2) % N - N / 2
+                e = dd[np.arange(len(f)), np.argmin(np.abs(dd), axis=1)]
+                errs.append(e[np.abs(e) <= 1.0])
+        e = np.concatenate(errs)
+        out[f"{name}_rmse_bins"] = float(np.sqrt(np.mean(e * e)))
+    ctx.set_refiner(None)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +228,7 @@ def main():
     ap.add_argument("--nfft", type=int, default=65536)
     ap.add_argument("--kpeaks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, copy the input once device-to-device (a plain copy of known size, "
                          "used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -166,6 +305,20 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, N, K)
+        try:
+            mt = cpu_baseline_mt(x, N, K, min(16, os.cpu_count() or 1))
+            if mt:
+                out["cpu_baseline"]["multicore"] = mt
+        except Exception as e:  # the single-thread baseline stands on its own
+            out["cpu_baseline"]["multicore"] = {"error": repr(e)}
+    if not args.no_extras:
+        del d_in, x
+        for name, fn in (("uneven", extra_uneven), ("mixed_mlp", extra_mixed_mlp), ("refiner", extra_refiner)):
+            print(f"[bench] {name} ...", file=sys.stderr, flush=True)
+            try:
+                out[name] = fn(ctx)
+            except Exception as e:
+                out[name] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 
