@@ -233,7 +233,7 @@ __device__ __forceinline__ void batch_large_item(const BatchArgs& b, const uint4
   a.scratch = b.scratch;
   a.slot_bytes = b.slot_bytes;
   a.mlp = nullptr;
-  large_channel<N2, 1, V>(a, 0, ch, slot, *reinterpret_cast<LargeShared<N2>*>(smem));
+  large_channel<N2, 1, V, 1024, true>(a, 0, ch, slot, *reinterpret_cast<LargeShared<N2, 1024, true>*>(smem));
 }
 
 __global__ void k_batch_reset(uint32_t* counter) {
@@ -279,15 +279,20 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs b) {
 
 extern "C" {
 
-// Dynamic LDS of the batch kernel: the largest per-item layout.
-static size_t batch_smem() { return sizeof(spx::LargeShared<512>); }
+// Dynamic LDS of the batch kernel: the largest per-item layout (each item
+// reinterprets the buffer with its own layout; nothing carries over).
+static size_t batch_smem() {
+  using namespace spx;
+  size_t m = sizeof(LargeShared<512>);
+  if (sizeof(LargeShared<256>) > m) m = sizeof(LargeShared<256>);
+  if (sizeof(LargeShared<128>) > m) m = sizeof(LargeShared<128>);
+  return m;
+}
 
 // Allows the batch kernel its dynamic LDS; call once per process before the
 // first launch and outside any stream capture.
 hipError_t spx_batch_kernel_init(void) {
   using namespace spx;
-  static_assert(sizeof(LargeShared<128>) == sizeof(LargeShared<512>), "one LDS layout for all items");
-  static_assert(sizeof(LargeShared<256>) == sizeof(LargeShared<512>), "one LDS layout for all items");
   return hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)batch_smem());
 }
 

@@ -211,7 +211,7 @@ int prepare(spx_ctx* c, FusedArgs& a, int* grid) {
     a.tw_b = (const float2*)t;
     if ((rc = get_table(c, TW4, a.nfft, 0, &t))) return rc;
     a.tw_4 = (const float2*)t;
-    int g = c->cu_count > 0 ? c->cu_count : 256;
+    int g = (c->cu_count > 0 ? c->cu_count : 256) * spx_large_groups_per_cu(a.nfft);
     if ((uint32_t)g > a.batch) g = (int)a.batch;
     a.slot_bytes = spx_large_slot_bytes(a.nfft);
     if ((rc = ensure_scratch(c, a.slot_bytes * (uint64_t)g))) return rc;

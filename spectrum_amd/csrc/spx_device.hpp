@@ -168,6 +168,10 @@ struct Plan {
   static constexpr int ns(int p) { return p <= 0 ? 1 : ns(p - 1) * radix(p - 1); }
   static constexpr int twoff(int p) { return p <= 1 ? 0 : twoff(p - 1) + ns(p - 1) * radix(p - 1); }
   static constexpr int TWSIZE = NP <= 1 ? 1 : twoff(NP - 1) + ns(NP - 1) * radix(NP - 1);
+  // LDS copies pad every [k][r] row by two entries (a 16-byte multiple that
+  // spreads the rows of neighbouring k over the banks)
+  static constexpr int twoff_lds(int p) { return p <= 1 ? 0 : twoff_lds(p - 1) + ns(p - 1) * (radix(p - 1) + 2); }
+  static constexpr int TWSIZE_LDS = NP <= 1 ? 1 : twoff_lds(NP - 1) + ns(NP - 1) * (radix(NP - 1) + 2);
   static constexpr int R0 = (NP == 1) ? RX : (V == 0 ? 16 : RX);   // first-pass radix
   static constexpr int RL = (NP == 1) ? RX : (V == 0 ? RX : 16);   // last-pass radix
 };
@@ -212,9 +216,25 @@ struct LayInter {
   }
 };
 
-// R contiguous twiddles (a row of a [k][r] table) with 16-byte loads.
-template <int R>
+// R contiguous twiddles (a row of a [k][r] table) with 16-byte loads; TWLDS:
+// the table lives in LDS (plain loads) instead of global memory.
+template <int R, bool TWLDS = false>
 __device__ __forceinline__ void load_tw_row(const float2* row, float2* w) {
+  if constexpr (TWLDS) {
+    if constexpr (R == 2) {
+      w[0] = make_float2(1.f, 0.f);
+      w[1] = row[1];
+    } else {
+      const float4* q = reinterpret_cast<const float4*>(row);
+#pragma unroll
+      for (int i = 0; i < R / 2; ++i) {
+        const float4 t = q[i];
+        w[2 * i] = make_float2(t.x, t.y);
+        w[2 * i + 1] = make_float2(t.z, t.w);
+      }
+    }
+    return;
+  }
   if constexpr (R == 2) {
     w[0] = make_float2(1.f, 0.f);
     w[1] = gp(row)[1];
@@ -235,7 +255,7 @@ __device__ __forceinline__ void load_tw_row(const float2* row, float2* w) {
 //  out: LDS y[s][(jj/NS)*NS*R + jj%NS + r*NS]           (unless LAST)
 //       or, for LAST, v[b*R + r] = X[s][jj + r*L/R] in registers.
 // Twiddles W_(NS*R)^(r*(jj%NS)) from the pass table twp[k*R + r].
-template <class LAY, int L, int NSEQ, int R, int NS, int T, int PPT, bool FIRST, bool LAST>
+template <class LAY, int L, int NSEQ, int R, int NS, int T, int PPT, bool FIRST, bool LAST, bool TWLDS = false>
 __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[PPT],
                                          const float2* __restrict__ twp, int tid) {
   constexpr int BPT = PPT / R;
@@ -270,7 +290,7 @@ __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[P
         LAY::template map<R>(j, s, jj);
         const int k = jj % NS;
         float2 w[R];
-        load_tw_row<R>(twp + (uint32_t)(k * R), w);
+        load_tw_row<R, TWLDS>(twp + (uint32_t)(k * (TWLDS ? R + 2 : R)), w);
 #pragma unroll
         for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], w[r]);
       }
@@ -300,13 +320,13 @@ __device__ __forceinline__ void fft_pass(float2* __restrict__ lds, float2 (&v)[P
   }
 }
 
-template <class LAY, int L, int NSEQ, int T, int PPT, int P, int V>
+template <class LAY, int L, int NSEQ, int T, int PPT, int P, int V, bool TWLDS>
 struct FftPasses {
   __device__ __forceinline__ static void run(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
     using PL = Plan<L, V>;
-    fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1>(lds, v, tw + PL::twoff(P),
-                                                                                       tid);
-    if constexpr (P + 1 < PL::NP) FftPasses<LAY, L, NSEQ, T, PPT, P + 1, V>::run(lds, v, tw, tid);
+    fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1, TWLDS>(
+        lds, v, tw + (TWLDS ? PL::twoff_lds(P) : PL::twoff(P)), tid);
+    if constexpr (P + 1 < PL::NP) FftPasses<LAY, L, NSEQ, T, PPT, P + 1, V, TWLDS>::run(lds, v, tw, tid);
   }
 };
 
@@ -315,9 +335,9 @@ struct FftPasses {
 // j = tid + b*T); after it v holds X[s][jj + r*L/RL] for the radix-RL mapping
 // (RL = Plan<L,V>::RL).  tw: the concatenated per-pass twiddle tables of plan
 // (L, V) (Plan<L,V>::TWSIZE entries).
-template <class LAY, int L, int NSEQ, int T, int PPT, int V = 0>
+template <class LAY, int L, int NSEQ, int T, int PPT, int V = 0, bool TWLDS = false>
 __device__ __forceinline__ void fft_tile(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
-  FftPasses<LAY, L, NSEQ, T, PPT, 0, V>::run(lds, v, tw, tid);
+  FftPasses<LAY, L, NSEQ, T, PPT, 0, V, TWLDS>::run(lds, v, tw, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -361,4 +381,20 @@ struct BlockRed {
   int total;
 };
 
+}  // namespace spx
+
+namespace spx {
+// Copies the per-pass tables of plan (L, V) from global memory into the padded
+// LDS layout (rows of R + 2 entries); all T threads of the block take part.
+template <int L, int V, int T>
+__device__ __forceinline__ void copy_tw_to_lds(const float2* glob, float2* lds, int tid) {
+  using PL = Plan<L, V>;
+  for (int i = tid; i < PL::TWSIZE; i += T) {
+    int p = 1, base = 0;
+    while (p + 1 < PL::NP && i >= PL::twoff(p + 1)) ++p;
+    base = i - PL::twoff(p);
+    const int R = PL::radix(p);
+    lds[PL::twoff_lds(p) + (base / R) * (R + 2) + base % R] = gp(glob)[i];
+  }
+}
 }  // namespace spx

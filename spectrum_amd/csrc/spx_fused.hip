@@ -222,13 +222,26 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
   }
 }
 
+// Launch shape of the fused large path per column length N2: workgroup size
+// (1024: one workgroup per CU; 512: two per CU, each with half the tile) and
+// whether the twiddle tables are copied to LDS.  Chosen from measurements on
+// MI355X (1024 x 65536, 2048 x 32768, 512 x 131072 channels).
+template <int N2> struct LargeShape { static constexpr int T = 1024; static constexpr bool TWL = true; };
+template <> struct LargeShape<128> { static constexpr int T = 512; static constexpr bool TWL = true; };
+template <> struct LargeShape<512> { static constexpr int T = 512; static constexpr bool TWL = false; };
+
 template <int N2, int MODE>
-__global__ void __launch_bounds__(1024) k_fused_large(FusedArgs a) {
+__global__ void __launch_bounds__(LargeShape<N2>::T) __attribute__((amdgpu_waves_per_eu(4))) k_fused_large(FusedArgs a) {
+  constexpr int T = LargeShape<N2>::T;
+  constexpr bool TWL = LargeShape<N2>::TWL;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  LargeShared<N2>& sh = *reinterpret_cast<LargeShared<N2>*>(smem_raw);
+  LargeShared<N2, T, TWL>& sh = *reinterpret_cast<LargeShared<N2, T, TWL>*>(smem_raw);
   unsigned char* slot = reinterpret_cast<unsigned char*>(a.scratch) + (uint64_t)blockIdx.x * a.slot_bytes;
-  for (uint32_t ch = blockIdx.x; ch < a.batch; ch += gridDim.x) large_channel<N2, MODE, 0>(a, ch, ch, slot, sh);
+  for (uint32_t ch = blockIdx.x; ch < a.batch; ch += gridDim.x)
+    large_channel<N2, MODE, 0, T, TWL>(a, ch, ch, slot, sh);
 }
+
+constexpr uint64_t cmax(uint64_t x, uint64_t y) { return x > y ? x : y; }
 
 // ===========================================================================
 // dispatch
@@ -250,7 +263,8 @@ static hipError_t launch_small(const FusedArgs& a, int grid, hipStream_t st) {
 
 template <int N2, int MODE>
 static hipError_t launch_large_mode(const FusedArgs& a, int grid, hipStream_t st) {
-  const size_t smem = sizeof(LargeShared<N2>);
+  constexpr int T = LargeShape<N2>::T;
+  const size_t smem = sizeof(LargeShared<N2, T, LargeShape<N2>::TWL>);
   static bool attr_done = false;  // the attribute is per kernel function
   if (!attr_done) {
     hipError_t e = hipFuncSetAttribute((const void*)k_fused_large<N2, MODE>,
@@ -258,7 +272,7 @@ static hipError_t launch_large_mode(const FusedArgs& a, int grid, hipStream_t st
     if (e != hipSuccess) return e;
     attr_done = true;
   }
-  hipLaunchKernelGGL((k_fused_large<N2, MODE>), dim3(grid), dim3(1024), smem, st, a);
+  hipLaunchKernelGGL((k_fused_large<N2, MODE>), dim3(grid), dim3(T), smem, st, a);
   return hipGetLastError();
 }
 
@@ -274,15 +288,27 @@ extern "C" {
 uint64_t spx_large_slot_bytes(uint32_t nfft) {
   using namespace spx;
   switch (nfft) {
-    case 1u << 15: return LargeCfg<128>::SLOT_BYTES;
-    case 1u << 16: return LargeCfg<256>::SLOT_BYTES;
-    case 1u << 17: return LargeCfg<512>::SLOT_BYTES;
-    case 1u << 18: return LargeCfg<1024>::SLOT_BYTES;
+    case 1u << 15: return cmax(LargeCfg<128, 1024>::SLOT_BYTES, LargeCfg<128, LargeShape<128>::T>::SLOT_BYTES);
+    case 1u << 16: return cmax(LargeCfg<256, 1024>::SLOT_BYTES, LargeCfg<256, LargeShape<256>::T>::SLOT_BYTES);
+    case 1u << 17: return cmax(LargeCfg<512, 1024>::SLOT_BYTES, LargeCfg<512, LargeShape<512>::T>::SLOT_BYTES);
+    case 1u << 18: return cmax(LargeCfg<1024, 1024>::SLOT_BYTES, LargeCfg<1024, LargeShape<1024>::T>::SLOT_BYTES);
     default: return 0;
   }
 }
 
 int spx_fused_is_large(uint32_t nfft) { return nfft >= (1u << 15) ? 1 : 0; }
+
+// persistent workgroups per CU of the fused large path for length nfft
+int spx_large_groups_per_cu(uint32_t nfft) {
+  using namespace spx;
+  switch (nfft) {
+    case 1u << 15: return 1024 / LargeShape<128>::T;
+    case 1u << 16: return 1024 / LargeShape<256>::T;
+    case 1u << 17: return 1024 / LargeShape<512>::T;
+    case 1u << 18: return 1024 / LargeShape<1024>::T;
+    default: return 1;
+  }
+}
 
 // concatenated per-pass twiddle table length (float2 entries) of plan (L, V)
 int spx_twiddle_table_len(uint32_t L, int V) {

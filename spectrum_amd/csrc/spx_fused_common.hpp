@@ -227,16 +227,16 @@ __device__ __forceinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t 
   }
 }
 
-template <int N2_>
+template <int N2_, int T_ = 1024>
 struct LargeCfg {
   static constexpr int N1 = 256;                // phase-2 row length (radix 16 x 16)
   static constexpr int N2 = N2_;                // phase-1 column length
   static constexpr int N = N1 * N2;
-  static constexpr int T = 1024;
+  static constexpr int T = T_;                  // workgroup size (1024, or 512 for two workgroups per CU)
   static constexpr int PPT = 16;
-  static constexpr int TILE = T * PPT;          // 16384 points per tile
+  static constexpr int TILE = T * PPT;          // points per tile
   static constexpr int CB = TILE / N2;          // phase-1 columns per tile
-  static constexpr int RB = TILE / N1;          // phase-2 rows per tile (64)
+  static constexpr int RB = TILE / N1;          // phase-2 rows per tile
   static constexpr int NT1 = N1 / CB;           // phase-1 tiles
   static constexpr int NT2 = N2 / RB;           // phase-2 tiles
   using LAY1 = LayInter<N2, CB>;
@@ -249,16 +249,27 @@ struct LargeCfg {
   static constexpr uint64_t EDGE_M_BYTES = (uint64_t)NT2 * 4 * N1 * 4;
   static constexpr uint64_t EDGE_X_BYTES = (uint64_t)NT2 * 2 * N1 * 8;
   static constexpr uint64_t SLOT_BYTES = Z_BYTES + CAND_BYTES + EDGE_M_BYTES + EDGE_X_BYTES;
-  static constexpr int NBK = 1024;              // selection buckets
-  static constexpr int BSH = ilog2(N) - 10;     // bucket = k >> BSH
-  static_assert(NT1 >= 1 && NT2 >= 1 && CB >= 16, "tile shape");
+  static constexpr int NBK = T;                 // selection buckets, one per thread
+  static constexpr int BSH = ilog2(N) - ilog2(T);  // bucket = k >> BSH
+  static_assert(NT1 >= 1 && NT2 >= 1 && CB >= 8 && RB >= 4, "tile shape");
   static_assert(Plan<N1>::RL == 16, "rows end on a radix-16 pass");
 };
 
+// TWL: the per-pass twiddle tables of the 256-point rows and of the column
+// length are copied per channel into LDS (padded rows) instead of being read
+// from global memory.
+constexpr int kTwRows = Plan<256>::TWSIZE_LDS;
 template <int N2>
+constexpr int tw_cols_lds() {
+  return Plan<N2, 0>::TWSIZE_LDS > Plan<N2, 1>::TWSIZE_LDS ? Plan<N2, 0>::TWSIZE_LDS : Plan<N2, 1>::TWSIZE_LDS;
+}
+
+template <int N2, int T = 1024, bool TWL = true>
 struct LargeShared {
-  using C = LargeCfg<N2>;
+  using C = LargeCfg<N2, T>;
   float2 tile[C::LDS];
+  float2 twa[TWL ? kTwRows : 1];
+  float2 twb[TWL ? tw_cols_lds<N2>() : 1];
   BlockRed red;
   int hist[C::NBK];
   uint32_t ncand;
@@ -267,14 +278,15 @@ struct LargeShared {
 };
 
 // One channel of length N = 256*N2 through the three phases, run by a whole
-// 1024-thread workgroup.  ich indexes the input (a.in + ich*n_point), och the
+// workgroup of TW threads.  ich indexes the input (a.in + ich*n_point), och the
 // outputs; slot is this workgroup's scratch slot.  V picks the radix order of
 // the N2-point column transform (Plan<N2, V>; tw_b must hold its tables).
 // MODE 0: dense outputs only (spectrum / |X| / |X|^2 / mag+phase); MODE 1: peaks only.
-template <int N2, int MODE, int V>
+template <int N2, int MODE, int V, int TW = 1024, bool TWL = true>
 __device__ __forceinline__ void large_channel(const FusedArgs& a, uint64_t ich, uint64_t och, unsigned char* slot,
-                                              LargeShared<N2>& sh) {
-  using C = LargeCfg<N2>;
+                                              LargeShared<N2, TW, TWL>& sh) {
+  constexpr bool kLargeTwLds = TWL;
+  using C = LargeCfg<N2, TW>;
   constexpr int N = C::N, N1 = C::N1, T = C::T, PPT = C::PPT, CB = C::CB, RB = C::RB;
   constexpr int NT1 = C::NT1, NT2 = C::NT2;
   constexpr int R01 = Plan<N2, V>::R0, RL1 = Plan<N2, V>::RL;
@@ -293,6 +305,17 @@ __device__ __forceinline__ void large_channel(const FusedArgs& a, uint64_t ich, 
   constexpr bool peaks = (MODE == 1);
   const bool full_band = (a.s0 == 0 && a.s1 == (uint32_t)N);
 
+  const float2* twb_src = a.tw_b;  // column tables: LDS copy or global
+  const float2* twa_src = a.tw_a;  // row tables
+  if constexpr (kLargeTwLds) {
+    static_assert(Plan<N2, V>::TWSIZE_LDS <= tw_cols_lds<N2>() && Plan<256>::TWSIZE_LDS <= kTwRows, "LDS tables");
+    __syncthreads();  // the previous channel may still read them
+    copy_tw_to_lds<N2, V, T>(a.tw_b, sh.twb, opaque_v(tid0));
+    copy_tw_to_lds<256, 0, T>(a.tw_a, sh.twa, opaque_v(tid0));
+    __syncthreads();
+    twb_src = sh.twb;
+    twa_src = sh.twa;
+  }
   {
     int tid = opaque_v(tid0);
     FusedArgs ai = a;
@@ -322,7 +345,8 @@ __device__ __forceinline__ void large_channel(const FusedArgs& a, uint64_t ich, 
       }
 #pragma unroll
       for (int r = 0; r < 16; ++r) energy = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, energy));
-      fft_tile<LAY1, N2, CB, T, PPT, V>(lds, v, ai.tw_b, tid);
+      if constexpr (kLargeTwLds) fft_tile<LAY1, N2, CB, T, PPT, V, true>(lds, v, twb_src, tid);
+      else fft_tile<LAY1, N2, CB, T, PPT, V>(lds, v, ai.tw_b, tid);
       // v[b*RL1 + r] = Y[column s][k2 = jj' + r*N2/RL1] for j = tid + b*T
 #pragma unroll
       for (int b = 0; b < PPT / RL1; ++b) {
@@ -371,7 +395,8 @@ __device__ __forceinline__ void large_channel(const FusedArgs& a, uint64_t ich, 
 #pragma unroll
         for (int r = 0; r < 16; ++r) v[r] = gp(zr)[r * (N1 / 16)];
       }
-      fft_tile<LAY2, N1, RB, T, PPT>(lds, v, ai.tw_a, tid);
+      if constexpr (kLargeTwLds) fft_tile<LAY2, N1, RB, T, PPT, 0, true>(lds, v, twa_src, tid);
+      else fft_tile<LAY2, N1, RB, T, PPT>(lds, v, ai.tw_a, tid);
       // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]  ->  bin k2 + N2*k1
       const int k2 = t * RB + s_row;
       float m[16];
