@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -226,9 +227,47 @@ int prepare(spx_ctx* c, FusedArgs& a, int* grid) {
   return SPX_OK;
 }
 
+// Channels per launch group of the split long-channel peak search: bounds the
+// workspace (Z, candidate lists, edge rows) to about 2 GiB.
+uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch) {
+  const char* env = std::getenv("SPX_FS_CHUNK");
+  uint32_t ch = env ? (uint32_t)std::strtoul(env, nullptr, 10) : 0;
+  if (ch == 0) {
+    const uint64_t per = spx_fs_workspace_bytes(nfft, 1);
+    ch = (uint32_t)std::max<uint64_t>(64, (2ull << 30) / std::max<uint64_t>(per, 1));
+  }
+  return std::min(ch, batch);
+}
+
+// Threshold / local-maximum search of long channels: the three-launch split
+// path (spx_fourstep.hip) for 2^15 and 2^16, the single persistent launch
+// (k_fused_large) above, as measured on MI355X.  SPX_LARGE_PATH=fused|split
+// overrides the choice (both give the same bins; used to cross-check them).
+bool use_fourstep(const FusedArgs& a) {
+  if (!spx_fused_is_large(a.nfft) || a.peak_mode == 0) return false;
+  const char* env = std::getenv("SPX_LARGE_PATH");
+  if (env && std::strcmp(env, "fused") == 0) return false;
+  if (env && std::strcmp(env, "split") == 0) return true;
+  return a.nfft <= (1u << 16);
+}
+
 int run_fused(spx_ctx* c, FusedArgs& a) {
   if (a.batch == 0) return SPX_OK;
   int grid = 0, rc;
+  if (use_fourstep(a)) {
+    const void* t;
+    if ((rc = get_table(c, TW, 256, 0, &t))) return rc;
+    a.tw_a = (const float2*)t;
+    if ((rc = get_table(c, TW, a.nfft / 256, 0, &t))) return rc;
+    a.tw_b = (const float2*)t;
+    if ((rc = get_table(c, TW4, a.nfft, 0, &t))) return rc;
+    a.tw_4 = (const float2*)t;
+    const uint32_t chunk = fourstep_chunk(a.nfft, a.batch);
+    if ((rc = ensure_scratch(c, spx_fs_workspace_bytes(a.nfft, chunk)))) return rc;
+    SPX_HIP(c, hipSetDevice(c->device));
+    SPX_HIP(c, spx_launch_fourstep(&a, c->scratch, chunk, c->stream));
+    return SPX_OK;
+  }
   if ((rc = prepare(c, a, &grid))) return rc;
   SPX_HIP(c, hipSetDevice(c->device));
   SPX_HIP(c, spx_launch_fused(&a, grid, c->stream));

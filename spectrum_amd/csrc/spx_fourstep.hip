@@ -1,0 +1,464 @@
+// Threshold peak search of long channels (N = 256*N2 >= 32768) as three
+// launches over a chunk of channels, many small workgroups per CU:
+//   k_fs_cols   column transforms (N2 points) times W_N^(n1*k2) -> Z[k2][n1],
+//               per-workgroup input energy (Parseval)
+//   k_fs_rows   256-point row transforms, |X|, per-block sums, best bin and
+//               the local maxima that can clear the threshold; a maximum on
+//               a block's first or last row is kept with its outer neighbour
+//               pending (NaN), and those two |X| rows go to an edge buffer
+//   k_fs_finish one workgroup per channel: resolves pending neighbours,
+//               statistics, threshold, first K survivors in ascending order
+// Results equal those of the single-launch path (k_fused_large) bin for bin;
+// sums are accumulated in another order, so mean/std differ in the last bits.
+// The kernel boundaries order every exchange between workgroups (no flags,
+// no waits), and several resident workgroups per CU overlap one workgroup's
+// loads with another's arithmetic.
+#include "spx_fused_common.hpp"
+
+namespace spx {
+
+struct FsRowPart {
+  double s1, s2;      // sum |X|, sum |X|^2 over the block's rows (in band)
+  float m;            // largest |X| of the block, -1 if none
+  uint32_t k;         // its bin
+  float xr, xi, ml, mr;  // X and the neighbours' |X| (NaN: outside the block)
+};
+
+struct FsWork {
+  float2* Z;          // [nch][N]
+  Cand* cand;         // [nch][N/2 + 2]
+  uint32_t* ncand;    // [nch]
+  double* pe;         // [nch][NT1] input energy per column block
+  FsRowPart* part;    // [nch][NT2]
+  float* edge;        // [nch][NT2][2][256]  |X| of block rows 0 and RB-1
+  uint32_t ch0, nch;  // chunk: channels ch0 .. ch0+nch-1 of the launch
+};
+
+template <int N2, int TC>
+struct FsCfg {
+  static constexpr int N1 = 256, N = N1 * N2, PPT = 16;
+  static constexpr int CB = TC * PPT / N2;     // columns per k_fs_cols workgroup
+  static constexpr int NT1 = N1 / CB;
+  static constexpr int TR = 256;               // k_fs_rows workgroup (its launch bound)
+  static constexpr int RB = TR * PPT / N1;     // rows per k_fs_rows workgroup (16)
+  static constexpr int NT2 = N2 / RB;
+  static constexpr int TF = 256;               // k_fs_finish workgroup (its launch bound)
+  static constexpr int BSH = ilog2(N) - ilog2(TF);
+  static constexpr uint64_t CAND_CAP = N / 2 + 2;
+  using LAY1 = LayInter<N2, CB>;
+  using LAY2 = LayContig<N1, RB>;
+  static_assert(CB >= 8 && NT1 >= 1 && RB >= 4 && NT2 >= 1, "tile shape");
+  static_assert(Plan<N2, 0>::R0 == 16, "column plan starts with radix 16");
+};
+
+template <int T>
+__device__ __forceinline__ double block_sum_d(double x, double* red) {
+  constexpr int NW = T / 64;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  x = wave_sum(x);
+  if (lane == 0) red[w] = x;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) s += red[i];
+  return s;
+}
+
+template <int N2, int TC>
+__device__ __forceinline__ double fs_energy(const FsWork& w, uint32_t ci) {
+  using C = FsCfg<N2, TC>;
+  double P = 0.0;
+  for (int i = 0; i < C::NT1; ++i) P += gp(w.pe)[(uint64_t)ci * C::NT1 + i];
+  return P;
+}
+
+// 256-point transforms of 16 interleaved columns (thread t: column s = t%16,
+// jj = t/16; in: v[r] = x[s][jj + 16 r]; out: v[r] = X[s][jj + 16 r]), with
+// the LDS exchange done in halves through a float tile of 5104 entries.
+// Element e = 16 a + b of column s is at 320 a + 16 b + s + 16 (a & 3) (rows
+// of 320 floats, so the skew never reaches the next row): the pass-0 writes
+// (a = jj, b = r) and the pass-1 reads (a = r, b = jj) of a wave both fall on
+// 64 distinct banks.
+__device__ __forceinline__ void fft256_cols16_halfx(float* lf, float2 (&v)[16], const float2* tw, int tid) {
+  using PL = Plan<256, 0>;
+  static_assert(PL::NP == 2 && PL::radix(0) == 16 && PL::radix(1) == 16, "plan");
+  const int s = tid & 15, jj = tid >> 4;
+  Dft<16>::run(v);
+  float* wb = lf + 320 * jj + s + 16 * (jj & 3);
+  const float* rb = lf + 16 * jj + s;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) wb[16 * r] = v[r].x;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r].x = rb[320 * r + 16 * (r & 3)];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) wb[16 * r] = v[r].y;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r].y = rb[320 * r + 16 * (r & 3)];
+  float2 w[16];
+  load_tw_row<16, false>(tw + PL::twoff(1) + (uint32_t)(jj * 16), w);
+#pragma unroll
+  for (int r = 1; r < 16; ++r) v[r] = c_mul(v[r], w[r]);
+  Dft<16>::run(v);
+}
+
+// ---------------------------------------------------------------------------
+template <int N2, int TC>
+__global__ void __launch_bounds__(TC) k_fs_cols(FusedArgs a, FsWork w) {
+  using C = FsCfg<N2, TC>;
+  constexpr int N1 = C::N1, CB = C::CB, NT1 = C::NT1, PPT = C::PPT;
+  constexpr int RL1 = Plan<N2, 0>::RL;
+  using LAY1 = typename C::LAY1;
+  constexpr bool HALFX = (N2 == 256 && CB == 16);
+  __shared__ __align__(16) float2 tile[HALFX ? 5104 / 2 : LAY1::SIZE];
+  __shared__ double red[TC / 64];
+  const int tid = threadIdx.x;
+  const uint32_t ci = blockIdx.x / NT1, t = blockIdx.x % NT1;
+  const uint64_t ich = (uint64_t)w.ch0 + ci;
+  float2 v[PPT];
+  {
+    const int c = tid % CB, jj = tid / CB;  // jj in [0, N2/16)
+    const int n1 = t * CB + c;
+    load_run<16>(a, ich, (uint32_t)(n1 + N1 * jj), (uint32_t)(N1 * (N2 / 16)), v);
+  }
+  float e = 0.f;
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) e = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, e));
+  if constexpr (HALFX) fft256_cols16_halfx(reinterpret_cast<float*>(tile), v, a.tw_b, tid);
+  else fft_tile<LAY1, N2, CB, TC, PPT, 0, false>(tile, v, a.tw_b, tid);
+  float2* Z = w.Z + (uint64_t)ci * C::N;
+#pragma unroll
+  for (int b = 0; b < PPT / RL1; ++b) {
+    int s, jb;
+    LAY1::template map<RL1>(tid + b * TC, s, jb);
+    const uint32_t col = t * CB + s;
+#pragma unroll
+    for (int r = 0; r < RL1; ++r) {
+      const uint32_t el = (uint32_t)(jb + r * (N2 / RL1)) * N1 + col;
+      gp(Z)[el] = c_mul(v[b * RL1 + r], gp(a.tw_4)[el]);
+    }
+  }
+  const double E = block_sum_d<TC>((double)e, red);
+  if (tid == 0) {
+    gp(w.pe)[(uint64_t)ci * NT1 + t] = E;
+    if (t == 0) gp(w.ncand)[ci] = 0u;
+  }
+}
+
+// 256-point transforms of the RB rows of a LayContig<256, RB> tile, 16
+// points per thread, with the one LDS exchange between the two radix-16
+// passes done in halves (real parts, then imaginary parts): the tile needs 4
+// bytes per point.  In: v[r] = x[s][jj + 16 r] for s = tid/16, jj = tid%16.
+// Out: v[r] = X[s][jj + 16 r].  tw: the twiddle tables of Plan<256, 0>.
+template <int RB, int T>
+__device__ __forceinline__ void fft256_rows_halfx(float* lf, float2 (&v)[16], const float2* tw, int tid) {
+  using LAY = LayContig<256, RB>;
+  using PL = Plan<256, 0>;
+  static_assert(PL::NP == 2 && PL::radix(0) == 16 && PL::radix(1) == 16 && T * 16 == 256 * RB, "plan");
+  Dft<16>::run(v);  // pass 0: sub-transform length 1, no twiddles
+  int s, jj;
+  LAY::template map<16>(tid, s, jj);
+  float* wb = lf + LAY::idx(s, 16 * jj);  // pass-0 outputs y[s][16 jj + r]
+  const float* rbase = lf + LAY::idx(s, jj);  // pass-1 inputs y[s][jj + 16 r], stride 17
+#pragma unroll
+  for (int r = 0; r < 16; ++r) wb[r] = v[r].x;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r].x = rbase[17 * r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) wb[r] = v[r].y;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r].y = rbase[17 * r];
+  // pass 1: sub-transform length 16, twiddles W_256^(r (jj % 16))
+  float2 w[16];
+  load_tw_row<16, false>(tw + PL::twoff(1) + (uint32_t)(jj * 16), w);
+#pragma unroll
+  for (int r = 1; r < 16; ++r) v[r] = c_mul(v[r], w[r]);
+  Dft<16>::run(v);
+}
+
+// ---------------------------------------------------------------------------
+template <int N2, int TC>
+__global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
+  using C = FsCfg<N2, TC>;
+  constexpr int N1 = C::N1, RB = C::RB, NT2 = C::NT2, PPT = C::PPT, T = C::TR;
+  using LAY2 = typename C::LAY2;
+  __shared__ __align__(16) float ldsf[LAY2::SIZE];
+  __shared__ BlockRed red;
+  const int tid = threadIdx.x;
+  const uint32_t ci = blockIdx.x / NT2, t = blockIdx.x % NT2;
+  const int s_row = tid / (N1 / 16), jj_row = tid % (N1 / 16);
+  float2 v[PPT];
+  {
+    const float2* zr = w.Z + (uint64_t)ci * C::N + (uint32_t)((t * RB + s_row) * N1 + jj_row);
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) v[r] = gp(zr)[r * (N1 / 16)];
+  }
+  const bool full_band = (a.s0 == 0 && a.s1 == (uint32_t)C::N);
+  float lb = 0.f;
+  if (a.peak_mode == 1 && full_band && a.nsigma >= 0.f)
+    lb = (float)(fmin(1.0, (double)a.nsigma) * sqrt(fs_energy<N2, TC>(w, ci)) * 0.999);
+  fft256_rows_halfx<RB, T>(ldsf, v, a.tw_a, tid);
+  // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]  ->  bin k2 + N2*k1 (ascending in r)
+  const uint32_t k2 = t * RB + s_row;
+  float m[PPT];
+  float fs1 = 0.f, fs2 = 0.f;
+  float bm = -1.f;  // largest in-band |X| of this thread (first r on ties: lowest bin)
+  int rb = -1;
+  if (full_band) {
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) {
+      m[r] = mag_of(v[r]);
+      fs1 += m[r];
+      fs2 = fmaf(m[r], m[r], fs2);
+      const bool u = m[r] > bm;
+      bm = u ? m[r] : bm;
+      rb = u ? r : rb;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) {
+      m[r] = mag_of(v[r]);
+      if (in_range(k2 + (uint32_t)N2 * (uint32_t)(jj_row + 16 * r), a)) {
+        fs1 += m[r];
+        fs2 = fmaf(m[r], m[r], fs2);
+        const bool u = m[r] > bm;
+        bm = u ? m[r] : bm;
+        rb = u ? r : rb;
+      }
+    }
+  }
+  __syncthreads();  // LDS free again
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) ldsf[LAY2::idx(s_row, jj_row + 16 * r)] = m[r];
+  if (s_row == 0 || s_row == RB - 1) {
+    float* er = w.edge + (((uint64_t)ci * NT2 + t) * 2 + (s_row == 0 ? 0 : 1)) * N1;
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) gp(er)[jj_row + 16 * r] = m[r];
+  }
+  __syncthreads();
+  // neighbours k -/+ 1 are rows s_row -/+ 1 of the same column; outside the
+  // block they are left pending (NaN) for the finish
+  const bool top = (s_row == 0), bot = (s_row == RB - 1);
+  const int rl = top ? s_row : s_row - 1, rr = bot ? s_row : s_row + 1;
+  const float qnan = __builtin_nanf("");
+  if (bm >= lb) {  // some bin of this thread may clear the threshold
+    Cand* cand = w.cand + (uint64_t)ci * C::CAND_CAP;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) {
+      const int k1 = jj_row + 16 * r;
+      const float yl = ldsf[LAY2::idx(rl, k1)];
+      const float yr = ldsf[LAY2::idx(rr, k1)];
+      const bool ok = full_band || in_range(k2 + (uint32_t)N2 * (uint32_t)k1, a);
+      if (ok && (top || m[r] > yl) && (bot || m[r] > yr) && m[r] >= lb) mask |= 1u << r;
+    }
+    while (mask) {
+      const int r = __builtin_ctz(mask);
+      mask &= mask - 1;
+      const int k1 = jj_row + 16 * r;
+      float2 x = v[0];
+      float mc = m[0];
+#pragma unroll
+      for (int q = 1; q < PPT; ++q) {
+        x = (q == r) ? v[q] : x;
+        mc = (q == r) ? m[q] : mc;
+      }
+      push_cand(cand, gp(w.ncand) + ci, (uint32_t)C::CAND_CAP, k2 + (uint32_t)N2 * (uint32_t)k1, x,
+                top ? qnan : ldsf[LAY2::idx(rl, k1)], mc, bot ? qnan : ldsf[LAY2::idx(rr, k1)]);
+    }
+  }
+  const uint32_t bk = rb >= 0 ? k2 + (uint32_t)N2 * (uint32_t)(jj_row + 16 * rb) : 0xffffffffu;
+  double s1 = fs1, s2 = fs2;
+  float gm = bm;
+  uint32_t gk = bk;
+  block_stats<T>(s1, s2, gm, gk, red);
+  FsRowPart* pp = w.part + (uint64_t)ci * NT2 + t;
+  if (tid == 0) {
+    gp(&pp->s1)[0] = s1;
+    gp(&pp->s2)[0] = s2;
+    gp(&pp->m)[0] = gm;
+    gp(&pp->k)[0] = gk;
+  }
+  if (gk != 0xffffffffu && bk == gk && bm == gm) {
+    float2 x = v[0];
+#pragma unroll
+    for (int q = 1; q < PPT; ++q) x = (q == rb) ? v[q] : x;
+    const int k1 = jj_row + 16 * rb;
+    gp(&pp->xr)[0] = x.x;
+    gp(&pp->xi)[0] = x.y;
+    gp(&pp->ml)[0] = top ? qnan : ldsf[LAY2::idx(rl, k1)];
+    gp(&pp->mr)[0] = bot ? qnan : ldsf[LAY2::idx(rr, k1)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int N2, int TC>
+struct FsFinishShared {
+  int hist[FsCfg<N2, TC>::TF];
+  uint2 gat[4096];
+  BlockRed red;
+  uint32_t ngather;
+  int bstar;
+};
+
+// |X| of the outer neighbour of bin k (side -1: k-1 from the previous block's
+// last row; side +1: k+1 from the next block's first row; both wrap)
+template <int N2, int TC>
+__device__ __forceinline__ float fs_outer(const float* edge, uint32_t k, int side) {
+  using C = FsCfg<N2, TC>;
+  constexpr int N1 = C::N1, RB = C::RB, NT2 = C::NT2;
+  const uint32_t k2 = k & (N2 - 1), k1 = k / N2;
+  if (side < 0) {
+    if (k2 > 0) return gp(edge)[((k2 / RB - 1) * 2 + 1) * N1 + k1];
+    return gp(edge)[((NT2 - 1) * 2 + 1) * N1 + (k1 + N1 - 1) % N1];  // k-1 wraps to row N2-1, column k1-1
+  }
+  if (k2 < (uint32_t)N2 - 1) return gp(edge)[((k2 / RB + 1) * 2 + 0) * N1 + k1];
+  return gp(edge)[0 * N1 + (k1 + 1) % N1];  // k+1 wraps to row 0, column k1+1
+}
+
+template <int N2, int TC>
+__device__ __forceinline__ void fs_finish_body(const FusedArgs& a, const FsWork& w, uint32_t ci, int* hist, uint2* gat,
+                                               int& s_bstar, uint32_t& s_ngather, BlockRed& red) {
+  using C = FsCfg<N2, TC>;
+  constexpr int NT2 = C::NT2, T = C::TF;
+  const int tid = threadIdx.x;
+  const uint64_t och = (uint64_t)w.ch0 + ci;
+  const double P = fs_energy<N2, TC>(w, ci);
+  const Cand* cand = w.cand + (uint64_t)ci * C::CAND_CAP;
+  const float* edge = w.edge + (uint64_t)ci * NT2 * 2 * C::N1;
+  const uint32_t nc = min(gp(w.ncand)[ci], (uint32_t)C::CAND_CAP);
+
+  // the row blocks' sums and best bins (fixed order: thread t takes block t)
+  float bm = -1.f;
+  uint32_t bk = 0xffffffffu;
+  float2 bx = make_float2(0.f, 0.f);
+  float bl = 0.f, br = 0.f;
+  double s1 = 0.0, s2 = 0.0;
+  for (int t = tid; t < NT2; t += T) {
+    const FsRowPart* pp = w.part + (uint64_t)ci * NT2 + t;
+    s1 += gp(&pp->s1)[0];
+    s2 += gp(&pp->s2)[0];
+    const float pm = gp(&pp->m)[0];
+    const uint32_t pk = gp(&pp->k)[0];
+    if (pk != 0xffffffffu && (pm > bm || (pm == bm && pk < bk))) {
+      bm = pm; bk = pk;
+      bx = make_float2(gp(&pp->xr)[0], gp(&pp->xi)[0]);
+      bl = gp(&pp->ml)[0];
+      br = gp(&pp->mr)[0];
+    }
+  }
+  float gm = bm;
+  uint32_t gk = bk;
+  block_stats<T>(s1, s2, gm, gk, red);
+  const double cnt = (double)(a.s1 - a.s0);
+  float mu, sd;
+  const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
+  if (a.gmax && gk != 0xffffffffu && bk == gk && bm == gm) {
+    if (__builtin_isnan(bl)) bl = fs_outer<N2, TC>(edge, bk, -1);
+    if (__builtin_isnan(br)) br = fs_outer<N2, TC>(edge, bk, 1);
+    write_record(a.gmax + och, bk, bx, bl, bm, br, a);
+  }
+  // a maximum on a block's first or last row has its outer neighbour pending
+  // (NaN): look it up, and drop the candidate (mc = NaN) if it is no maximum
+  auto resolve = [edge](Cand& c) {
+    const bool pl = __builtin_isnan(c.ml), pr = __builtin_isnan(c.mr);
+    if (pl || pr) {
+      const float nb = fs_outer<N2, TC>(edge, c.k, pl ? -1 : 1);
+      if (!(c.mc > nb)) c.mc = __builtin_nanf("");
+      if (pl) c.ml = nb;
+      else c.mr = nb;
+    }
+  };
+  select_first_k<T, C::BSH, 4096>(a, cand, nc, thr, mu, sd, P, och, hist, gat, s_bstar, s_ngather, red, resolve);
+}
+
+
+template <int N2, int TC>
+__global__ void __launch_bounds__(256) k_fs_finish(FusedArgs a, FsWork w) {
+  __shared__ FsFinishShared<N2, TC> sh;
+  fs_finish_body<N2, TC>(a, w, blockIdx.x, sh.hist, sh.gat, sh.bstar, sh.ngather, sh.red);
+}
+
+template <int N2, int TC>
+static hipError_t launch_fs(const FusedArgs& a, const FsWork& w0, uint32_t chunk, hipStream_t st) {
+  using C = FsCfg<N2, TC>;
+  for (uint32_t c0 = 0; c0 < a.batch; c0 += chunk) {
+    FsWork w = w0;
+    w.ch0 = c0;
+    w.nch = min(chunk, a.batch - c0);
+    hipLaunchKernelGGL((k_fs_cols<N2, TC>), dim3(w.nch * C::NT1), dim3(TC), 0, st, a, w);
+    hipLaunchKernelGGL((k_fs_rows<N2, TC>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);
+    hipLaunchKernelGGL((k_fs_finish<N2, TC>), dim3(w.nch), dim3(C::TF), 0, st, a, w);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+template <int N2>
+struct FsShape { static constexpr int TC = N2 >= 1024 ? 512 : 256; };
+
+template <int N2>
+static uint64_t fs_channel_bytes() {
+  using C = FsCfg<N2, FsShape<N2>::TC>;
+  auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
+  return r((uint64_t)C::N * 8) + r(C::CAND_CAP * sizeof(Cand)) + 256 + r((uint64_t)C::NT1 * 8) +
+         r((uint64_t)C::NT2 * sizeof(FsRowPart)) + r((uint64_t)C::NT2 * 2 * 256 * 4);
+}
+
+template <int N2>
+static hipError_t launch_fs_n(const FusedArgs& a, void* ws, uint32_t chunk, hipStream_t st) {
+  using C = FsCfg<N2, FsShape<N2>::TC>;
+  auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
+  unsigned char* p = reinterpret_cast<unsigned char*>(ws);
+  FsWork w;
+  w.Z = reinterpret_cast<float2*>(p);            p += r((uint64_t)chunk * C::N * 8);
+  w.cand = reinterpret_cast<Cand*>(p);           p += r((uint64_t)chunk * C::CAND_CAP * sizeof(Cand));
+  w.ncand = reinterpret_cast<uint32_t*>(p);      p += r((uint64_t)chunk * 4);
+  w.pe = reinterpret_cast<double*>(p);           p += r((uint64_t)chunk * C::NT1 * 8);
+  w.part = reinterpret_cast<FsRowPart*>(p);      p += r((uint64_t)chunk * C::NT2 * sizeof(FsRowPart));
+  w.edge = reinterpret_cast<float*>(p);
+  w.ch0 = 0;
+  w.nch = 0;
+  return launch_fs<N2, FsShape<N2>::TC>(a, w, chunk, st);
+}
+
+}  // namespace spx
+
+extern "C" {
+
+// Workspace of the split long-channel peak search for `chunk` channels of
+// length nfft (0 when nfft is not 2^15 .. 2^18).
+uint64_t spx_fs_workspace_bytes(uint32_t nfft, uint32_t chunk) {
+  using namespace spx;
+  uint64_t per = 0;
+  switch (nfft) {
+    case 1u << 15: per = fs_channel_bytes<128>(); break;
+    case 1u << 16: per = fs_channel_bytes<256>(); break;
+    case 1u << 17: per = fs_channel_bytes<512>(); break;
+    case 1u << 18: per = fs_channel_bytes<1024>(); break;
+    default: return 0;
+  }
+  return per * (uint64_t)chunk + 6 * 256;
+}
+
+// Runs the split path over a.batch channels in chunks of `chunk` channels,
+// reusing the workspace `ws` (spx_fs_workspace_bytes(nfft, chunk) bytes).
+hipError_t spx_launch_fourstep(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st) {
+  using namespace spx;
+  if (chunk == 0) return hipErrorInvalidValue;
+  switch (a->nfft) {
+    case 1u << 15: return launch_fs_n<128>(*a, ws, chunk, st);
+    case 1u << 16: return launch_fs_n<256>(*a, ws, chunk, st);
+    case 1u << 17: return launch_fs_n<512>(*a, ws, chunk, st);
+    case 1u << 18: return launch_fs_n<1024>(*a, ws, chunk, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // extern "C"

@@ -227,6 +227,104 @@ __device__ __forceinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t 
   }
 }
 
+// Selection of the first K threshold survivors in ascending bin order from a
+// candidate list (local maxima that may clear the threshold), plus the counts
+// and stats outputs of channel och.  One bucket per thread (T buckets of
+// 2^BSH bins) is histogrammed and scanned to find the bucket holding the K-th
+// survivor; survivors up to that bucket are gathered into gat (GMAX entries of
+// LDS) and bitonic-sorted by bin.  hist: T ints of LDS.  fix(cd) is applied
+// to every candidate read (it may complete it, or set mc = NaN to drop it).
+// Every thread of the T-thread workgroup calls it.
+struct CandAsIs {
+  __device__ __forceinline__ void operator()(Cand&) const {}
+};
+
+template <int T, int BSH, int GMAX, class FIX = CandAsIs>
+__device__ __forceinline__ void select_first_k(const FusedArgs& a, const Cand* cand, uint32_t ncand, float thr,
+                                               float mu, float sd, double P, uint64_t och, int* hist, uint2* gat,
+                                               int& s_bstar, uint32_t& s_ngather, BlockRed& red, FIX fix = FIX()) {
+  const int tid0 = threadIdx.x;
+  for (int i = opaque_v(tid0); i < T; i += T) hist[i] = 0;
+  __syncthreads();
+  for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
+    Cand cd[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cd[q] = gp(cand)[min(i0 + q * T, ncand - 1)];
+      fix(cd[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (i0 + q * T < ncand && cd[q].mc > thr) atomicAdd(&hist[cd[q].k >> BSH], 1);
+  }
+  __syncthreads();
+  // exclusive scan over buckets (one bucket per thread)
+  int hv = hist[tid0];
+  int total;
+  int excl = block_excl_scan<T>(hv, total, red);
+  const int kcap = (int)a.kcap;
+  if (tid0 == 0) { s_bstar = T - 1; s_ngather = 0; }
+  __syncthreads();
+  if (total > kcap && excl < kcap && excl + hv >= kcap) s_bstar = tid0;
+  __syncthreads();
+  const int bstar = s_bstar;
+  // gather (key = bin, value = candidate index) of survivors in buckets <= bstar
+  for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
+    Cand cd[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cd[q] = gp(cand)[min(i0 + q * T, ncand - 1)];
+      fix(cd[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t i = i0 + q * T;
+      if (i < ncand && cd[q].mc > thr && (int)(cd[q].k >> BSH) <= bstar) {
+        const uint32_t p = atomicAdd(&s_ngather, 1u);
+        if (p < (uint32_t)GMAX) gat[p] = make_uint2(cd[q].k, i);
+      }
+    }
+  }
+  __syncthreads();
+  const int ng = min((int)s_ngather, GMAX);
+  int np2 = 1;
+  while (np2 < ng) np2 <<= 1;
+  for (int i = ng + opaque_v(tid0); i < np2; i += T) gat[i] = make_uint2(0xffffffffu, 0u);
+  __syncthreads();
+  // bitonic sort by key
+  for (int size = 2; size <= np2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = opaque_v(tid0); i < np2 / 2; i += T) {
+        const int lo = 2 * i - (i & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        uint2 x = gat[lo], y = gat[hi];
+        if ((x.x > y.x) == up) { gat[lo] = y; gat[hi] = x; }
+      }
+      __syncthreads();
+    }
+  }
+  const int nout = min(total, kcap);
+  MaxValue* out = a.peaks + (uint64_t)och * a.kcap;
+  for (int i = opaque_v(tid0); i < nout && i < ng; i += T) {
+    Cand cd = gp(cand)[gat[i].y];
+    fix(cd);
+    write_record(out + i, cd.k, make_float2(cd.re, cd.im), cd.ml, cd.mc, cd.mr, a);
+  }
+  if (tid0 == 0) {
+    if (a.counts) {
+      gp(a.counts)[2 * och + 0] = (uint32_t)nout;
+      gp(a.counts)[2 * och + 1] = (uint32_t)total;
+    }
+    if (a.stats) {
+      gp(a.stats)[4 * och + 0] = mu;
+      gp(a.stats)[4 * och + 1] = sd;
+      gp(a.stats)[4 * och + 2] = thr;
+      gp(a.stats)[4 * och + 3] = (float)P;
+    }
+  }
+}
+
 template <int N2_, int T_ = 1024>
 struct LargeCfg {
   static constexpr int N1 = 256;                // phase-2 row length (radix 16 x 16)
@@ -530,80 +628,9 @@ __device__ __forceinline__ void large_channel(const FusedArgs& a, uint64_t ich, 
       if (a.gmax && gk != 0xffffffffu && bk == gk && bm == gm) write_record(a.gmax + och, bk, bx, bl, bm, br, a);
 
       const uint32_t ncand = min(sh.ncand, (uint32_t)C::CAND_CAP);
-      for (int i = opaque_v(tid0); i < C::NBK; i += T) sh.hist[i] = 0;
-      __syncthreads();
-      for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
-        Cand cd[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) cd[q] = gp(cand)[min(i0 + q * T, ncand - 1)];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (i0 + q * T < ncand && cd[q].mc > thr) atomicAdd(&sh.hist[cd[q].k >> C::BSH], 1);
-      }
-      __syncthreads();
-      // exclusive scan over buckets (one bucket per thread)
-      int hv = sh.hist[tid0];
-      int total;
-      int excl = block_excl_scan<T>(hv, total, sh.red);
-      const int kcap = (int)a.kcap;
-      if (tid0 == 0) { sh.bstar = C::NBK - 1; sh.ngather = 0; }
-      __syncthreads();
-      if (total > kcap && excl < kcap && excl + hv >= kcap) sh.bstar = tid0;
-      __syncthreads();
-      const int bstar = sh.bstar;
-      // gather (key = bin, value = candidate index) of survivors in buckets <= bstar
-      uint2* gat = reinterpret_cast<uint2*>(lds);
-      constexpr int GMAX = C::LDS > 4096 ? 4096 : C::LDS;
-      for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
-        Cand cd[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) cd[q] = gp(cand)[min(i0 + q * T, ncand - 1)];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const uint32_t i = i0 + q * T;
-          if (i < ncand && cd[q].mc > thr && (int)(cd[q].k >> C::BSH) <= bstar) {
-            const uint32_t p = atomicAdd(&sh.ngather, 1u);
-            if (p < (uint32_t)GMAX) gat[p] = make_uint2(cd[q].k, i);
-          }
-        }
-      }
-      __syncthreads();
-      const int ng = min((int)sh.ngather, GMAX);
-      int np2 = 1;
-      while (np2 < ng) np2 <<= 1;
-      for (int i = ng + opaque_v(tid0); i < np2; i += T) gat[i] = make_uint2(0xffffffffu, 0u);
-      __syncthreads();
-      // bitonic sort by key
-      for (int size = 2; size <= np2; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-          for (int i = opaque_v(tid0); i < np2 / 2; i += T) {
-            const int lo = 2 * i - (i & (stride - 1));
-            const int hi = lo + stride;
-            const bool up = ((lo & size) == 0);
-            uint2 x = gat[lo], y = gat[hi];
-            if ((x.x > y.x) == up) { gat[lo] = y; gat[hi] = x; }
-          }
-          __syncthreads();
-        }
-      }
-      const int nout = min(total, kcap);
-      MaxValue* out = a.peaks + (uint64_t)och * a.kcap;
-      for (int i = opaque_v(tid0); i < nout && i < ng; i += T) {
-        const Cand cd = gp(cand)[gat[i].y];
-        write_record(out + i, cd.k, make_float2(cd.re, cd.im), cd.ml, cd.mc, cd.mr, a);
-      }
-      if (tid0 == 0) {
-        if (a.counts) {
-          gp(a.counts)[2 * och + 0] = (uint32_t)nout;
-          gp(a.counts)[2 * och + 1] = (uint32_t)total;
-        }
-        if (a.stats) {
-          gp(a.stats)[4 * och + 0] = mu;
-          gp(a.stats)[4 * och + 1] = sd;
-          gp(a.stats)[4 * och + 2] = thr;
-          gp(a.stats)[4 * och + 3] = (float)P;
-        }
-      }
+      select_first_k<T, C::BSH, (C::LDS > 4096 ? 4096 : C::LDS)>(a, cand, ncand, thr, mu, sd, P, och, sh.hist,
+                                                                reinterpret_cast<uint2*>(lds), sh.bstar,
+                                                                sh.ngather, sh.red);
     }
     __syncthreads();
     }
