@@ -239,16 +239,15 @@ uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch) {
   return std::min(ch, batch);
 }
 
-// Threshold / local-maximum search of long channels: the three-launch split
-// path (spx_fourstep.hip) for 2^15 and 2^16, the single persistent launch
-// (k_fused_large) above, as measured on MI355X.  SPX_LARGE_PATH=fused|split
-// overrides the choice (both give the same bins; used to cross-check them).
+// Threshold / local-maximum search of long channels runs on the three-launch
+// split path (spx_fourstep.hip), faster than the single persistent launch
+// (k_fused_large) at every length 2^15 .. 2^18 on MI355X.
+// SPX_LARGE_PATH=fused selects the single launch instead (both give the same
+// bins; the tests cross-check them).
 bool use_fourstep(const FusedArgs& a) {
   if (!spx_fused_is_large(a.nfft) || a.peak_mode == 0) return false;
   const char* env = std::getenv("SPX_LARGE_PATH");
-  if (env && std::strcmp(env, "fused") == 0) return false;
-  if (env && std::strcmp(env, "split") == 0) return true;
-  return a.nfft <= (1u << 16);
+  return !(env && std::strcmp(env, "fused") == 0);
 }
 
 int run_fused(spx_ctx* c, FusedArgs& a) {

@@ -26,8 +26,8 @@ struct FsRowPart {
 
 struct FsWork {
   float2* Z;          // [nch][N]
-  Cand* cand;         // [nch][N/2 + 2]
-  uint32_t* ncand;    // [nch]
+  Cand* cand;         // [nch][N/2 + 2]: the high list from the front, the low list from the back
+  uint32_t* ncand;    // [nch][2] lengths of the high and the low list
   double* pe;         // [nch][NT1] input energy per column block
   FsRowPart* part;    // [nch][NT2]
   float* edge;        // [nch][NT2][2][256]  |X| of block rows 0 and RB-1
@@ -42,7 +42,7 @@ struct FsCfg {
   static constexpr int TR = 256;               // k_fs_rows workgroup (its launch bound)
   static constexpr int RB = TR * PPT / N1;     // rows per k_fs_rows workgroup (16)
   static constexpr int NT2 = N2 / RB;
-  static constexpr int TF = 256;               // k_fs_finish workgroup (its launch bound)
+  static constexpr int TF = 64;                // k_fs_finish workgroup: one wave (its launch bound)
   static constexpr int BSH = ilog2(N) - ilog2(TF);
   static constexpr uint64_t CAND_CAP = N / 2 + 2;
   using LAY1 = LayInter<N2, CB>;
@@ -50,6 +50,23 @@ struct FsCfg {
   static_assert(CB >= 8 && NT1 >= 1 && RB >= 4 && NT2 >= 1, "tile shape");
   static_assert(Plan<N2, 0>::R0 == 16, "column plan starts with radix 16");
 };
+
+// Candidate pre-filter of a channel, identical in the row and finish
+// kernels.  By Parseval mean |X|^2 = P is known before any row transform, and
+// mean + k std >= min(1, k) sqrt(P), so maxima below lb (0.999 of that) can
+// never clear the threshold.  Candidates at or above hi = 3 lb go to the high
+// list: when the threshold turns out >= hi, the low list is not read at all.
+struct FsBounds {
+  float lb, hi;
+};
+__device__ __forceinline__ FsBounds fs_bounds(const FusedArgs& a, double P, bool full_band) {
+  FsBounds b{0.f, 0.f};
+  if (a.peak_mode == 1 && full_band && a.nsigma >= 0.f) {
+    b.lb = (float)(fmin(1.0, (double)a.nsigma) * sqrt(P) * 0.999);
+    b.hi = 3.0f * b.lb;
+  }
+  return b;
+}
 
 template <int T>
 __device__ __forceinline__ double block_sum_d(double x, double* red) {
@@ -143,7 +160,10 @@ __global__ void __launch_bounds__(TC) k_fs_cols(FusedArgs a, FsWork w) {
   const double E = block_sum_d<TC>((double)e, red);
   if (tid == 0) {
     gp(w.pe)[(uint64_t)ci * NT1 + t] = E;
-    if (t == 0) gp(w.ncand)[ci] = 0u;
+    if (t == 0) {
+      gp(w.ncand)[2 * ci + 0] = 0u;
+      gp(w.ncand)[2 * ci + 1] = 0u;
+    }
   }
 }
 
@@ -199,9 +219,8 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
     for (int r = 0; r < PPT; ++r) v[r] = gp(zr)[r * (N1 / 16)];
   }
   const bool full_band = (a.s0 == 0 && a.s1 == (uint32_t)C::N);
-  float lb = 0.f;
-  if (a.peak_mode == 1 && full_band && a.nsigma >= 0.f)
-    lb = (float)(fmin(1.0, (double)a.nsigma) * sqrt(fs_energy<N2, TC>(w, ci)) * 0.999);
+  const FsBounds bnd = fs_bounds(a, fs_energy<N2, TC>(w, ci), full_band);
+  const float lb = bnd.lb;
   fft256_rows_halfx<RB, T>(ldsf, v, a.tw_a, tid);
   // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]  ->  bin k2 + N2*k1 (ascending in r)
   const uint32_t k2 = t * RB + s_row;
@@ -248,29 +267,48 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
   const float qnan = __builtin_nanf("");
   if (bm >= lb) {  // some bin of this thread may clear the threshold
     Cand* cand = w.cand + (uint64_t)ci * C::CAND_CAP;
-    uint32_t mask = 0;
+    uint32_t mhigh = 0, mlow = 0;
 #pragma unroll
     for (int r = 0; r < PPT; ++r) {
       const int k1 = jj_row + 16 * r;
       const float yl = ldsf[LAY2::idx(rl, k1)];
       const float yr = ldsf[LAY2::idx(rr, k1)];
       const bool ok = full_band || in_range(k2 + (uint32_t)N2 * (uint32_t)k1, a);
-      if (ok && (top || m[r] > yl) && (bot || m[r] > yr) && m[r] >= lb) mask |= 1u << r;
-    }
-    while (mask) {
-      const int r = __builtin_ctz(mask);
-      mask &= mask - 1;
-      const int k1 = jj_row + 16 * r;
-      float2 x = v[0];
-      float mc = m[0];
-#pragma unroll
-      for (int q = 1; q < PPT; ++q) {
-        x = (q == r) ? v[q] : x;
-        mc = (q == r) ? m[q] : mc;
+      if (ok && (top || m[r] > yl) && (bot || m[r] > yr) && m[r] >= lb) {
+        if (m[r] >= bnd.hi) mhigh |= 1u << r;
+        else mlow |= 1u << r;
       }
-      push_cand(cand, gp(w.ncand) + ci, (uint32_t)C::CAND_CAP, k2 + (uint32_t)N2 * (uint32_t)k1, x,
-                top ? qnan : ldsf[LAY2::idx(rl, k1)], mc, bot ? qnan : ldsf[LAY2::idx(rr, k1)]);
     }
+    // the two lists are filled in separate loops, each with one counter
+    // address for the whole wave, so that the increments of a wave are
+    // combined into one atomic
+    auto push = [&](uint32_t mask, uint32_t* ctr, bool high) {
+      while (mask) {
+        const int r = __builtin_ctz(mask);
+        mask &= mask - 1;
+        const int k1 = jj_row + 16 * r;
+        float2 x = v[0];
+        float mc = m[0];
+#pragma unroll
+        for (int q = 1; q < PPT; ++q) {
+          x = (q == r) ? v[q] : x;
+          mc = (q == r) ? m[q] : mc;
+        }
+        const uint32_t p = atomicAdd(ctr, 1u);
+        if (p < (uint32_t)C::CAND_CAP) {
+          Cand cd;
+          cd.k = k2 + (uint32_t)N2 * (uint32_t)k1;
+          cd.re = x.x;
+          cd.im = x.y;
+          cd.ml = top ? qnan : ldsf[LAY2::idx(rl, k1)];
+          cd.mc = mc;
+          cd.mr = bot ? qnan : ldsf[LAY2::idx(rr, k1)];
+          gp(cand)[high ? p : (uint32_t)C::CAND_CAP - 1 - p] = cd;
+        }
+      }
+    };
+    push(mhigh, gp(w.ncand) + 2 * ci + 0, true);
+    push(mlow, gp(w.ncand) + 2 * ci + 1, false);
   }
   const uint32_t bk = rb >= 0 ? k2 + (uint32_t)N2 * (uint32_t)(jj_row + 16 * rb) : 0xffffffffu;
   double s1 = fs1, s2 = fs2;
@@ -297,15 +335,6 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
 }
 
 // ---------------------------------------------------------------------------
-template <int N2, int TC>
-struct FsFinishShared {
-  int hist[FsCfg<N2, TC>::TF];
-  uint2 gat[4096];
-  BlockRed red;
-  uint32_t ngather;
-  int bstar;
-};
-
 // |X| of the outer neighbour of bin k (side -1: k-1 from the previous block's
 // last row; side +1: k+1 from the next block's first row; both wrap)
 template <int N2, int TC>
@@ -322,66 +351,117 @@ __device__ __forceinline__ float fs_outer(const float* edge, uint32_t k, int sid
 }
 
 template <int N2, int TC>
-__device__ __forceinline__ void fs_finish_body(const FusedArgs& a, const FsWork& w, uint32_t ci, int* hist, uint2* gat,
-                                               int& s_bstar, uint32_t& s_ngather, BlockRed& red) {
+struct FsFinishShared {
+  int hist[FsCfg<N2, TC>::TF];
+  uint2 gat[4096];
+  BlockRed red;
+  uint32_t ngather;
+  int bstar;
+};
+
+// The finish of one channel, one wave (TF = 64 threads) per channel: sums and
+// best bin of the row blocks (one block per lane), threshold, then the first
+// K survivors of the candidate list in ascending bin order.  A maximum on a
+// block's first or last row has its outer neighbour pending (NaN): it is
+// looked up here, and the candidate dropped (mc = NaN) when it is no maximum.
+// Up to 64 candidates (one per lane) are selected with ballots, without
+// barriers or LDS; longer lists go through the shared selection stage.
+template <int N2, int TC>
+__global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
   using C = FsCfg<N2, TC>;
-  constexpr int NT2 = C::NT2, T = C::TF;
-  const int tid = threadIdx.x;
+  constexpr int NT1 = C::NT1, NT2 = C::NT2, T = C::TF;
+  static_assert(T == 64 && NT1 <= 64 && NT2 <= 64 && C::CAND_CAP >= 64, "one value per lane");
+  __shared__ FsFinishShared<N2, TC> sh;
+  const int lane = threadIdx.x;
+  const uint32_t ci = blockIdx.x;
   const uint64_t och = (uint64_t)w.ch0 + ci;
-  const double P = fs_energy<N2, TC>(w, ci);
   const Cand* cand = w.cand + (uint64_t)ci * C::CAND_CAP;
   const float* edge = w.edge + (uint64_t)ci * NT2 * 2 * C::N1;
-  const uint32_t nc = min(gp(w.ncand)[ci], (uint32_t)C::CAND_CAP);
-
-  // the row blocks' sums and best bins (fixed order: thread t takes block t)
-  float bm = -1.f;
+  // every load of the channel at once: its high-list candidate (speculative),
+  // list lengths, energy and row-block part
+  Cand c = gp(cand)[lane];
+  const uint32_t nh = min(gp(w.ncand)[2 * ci + 0], (uint32_t)C::CAND_CAP);
+  const uint32_t nl = min(gp(w.ncand)[2 * ci + 1], (uint32_t)C::CAND_CAP - nh);
+  const double P = fs_energy<N2, TC>(w, ci);
+  double s1 = 0.0, s2 = 0.0;
+  float bm = -1.f, bl = 0.f, br = 0.f;
   uint32_t bk = 0xffffffffu;
   float2 bx = make_float2(0.f, 0.f);
-  float bl = 0.f, br = 0.f;
-  double s1 = 0.0, s2 = 0.0;
-  for (int t = tid; t < NT2; t += T) {
-    const FsRowPart* pp = w.part + (uint64_t)ci * NT2 + t;
-    s1 += gp(&pp->s1)[0];
-    s2 += gp(&pp->s2)[0];
-    const float pm = gp(&pp->m)[0];
-    const uint32_t pk = gp(&pp->k)[0];
-    if (pk != 0xffffffffu && (pm > bm || (pm == bm && pk < bk))) {
-      bm = pm; bk = pk;
-      bx = make_float2(gp(&pp->xr)[0], gp(&pp->xi)[0]);
-      bl = gp(&pp->ml)[0];
-      br = gp(&pp->mr)[0];
-    }
+  if (lane < NT2) {
+    const FsRowPart* pp = w.part + (uint64_t)ci * NT2 + lane;
+    s1 = gp(&pp->s1)[0];
+    s2 = gp(&pp->s2)[0];
+    bm = gp(&pp->m)[0];
+    bk = gp(&pp->k)[0];
+    bx = make_float2(gp(&pp->xr)[0], gp(&pp->xi)[0]);
+    bl = gp(&pp->ml)[0];
+    br = gp(&pp->mr)[0];
+    if (bk == 0xffffffffu) bm = -1.f;
   }
+  auto resolve = [edge](Cand& q) {
+    const bool pl = __builtin_isnan(q.ml), pr = __builtin_isnan(q.mr);
+    if (pl || pr) {
+      const float nb = fs_outer<N2, TC>(edge, q.k, pl ? -1 : 1);
+      if (!(q.mc > nb)) q.mc = __builtin_nanf("");
+      if (pl) q.ml = nb;
+      else q.mr = nb;
+    }
+  };
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
   float gm = bm;
   uint32_t gk = bk;
-  block_stats<T>(s1, s2, gm, gk, red);
+  wave_argmax(gm, gk);
   const double cnt = (double)(a.s1 - a.s0);
   float mu, sd;
   const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
+  // the candidate list to select from: the high list, and the low list
+  // (entries CAND_CAP-1, CAND_CAP-2, ...) unless the threshold is above it
+  const bool full_band = (a.s0 == 0 && a.s1 == (uint32_t)C::N);
+  const FsBounds bnd = fs_bounds(a, P, full_band);
+  const uint32_t nc = (thr >= bnd.hi && bnd.hi > 0.f) ? nh : nh + nl;
+  auto at = [cand, nh](uint32_t i) { return gp(cand)[i < nh ? i : (uint32_t)C::CAND_CAP - 1 - (i - nh)]; };
+  if ((uint32_t)lane >= nh && (uint32_t)lane < nc) c = at(lane);
+  if ((uint32_t)lane < nc) resolve(c);
   if (a.gmax && gk != 0xffffffffu && bk == gk && bm == gm) {
     if (__builtin_isnan(bl)) bl = fs_outer<N2, TC>(edge, bk, -1);
     if (__builtin_isnan(br)) br = fs_outer<N2, TC>(edge, bk, 1);
     write_record(a.gmax + och, bk, bx, bl, bm, br, a);
   }
-  // a maximum on a block's first or last row has its outer neighbour pending
-  // (NaN): look it up, and drop the candidate (mc = NaN) if it is no maximum
-  auto resolve = [edge](Cand& c) {
-    const bool pl = __builtin_isnan(c.ml), pr = __builtin_isnan(c.mr);
-    if (pl || pr) {
-      const float nb = fs_outer<N2, TC>(edge, c.k, pl ? -1 : 1);
-      if (!(c.mc > nb)) c.mc = __builtin_nanf("");
-      if (pl) c.ml = nb;
-      else c.mr = nb;
+  if (nc <= 64u) {
+    const bool surv = (uint32_t)lane < nc && c.mc > thr;
+    uint64_t smask = __ballot(surv);
+    const int total = __popcll(smask);
+    // rank = survivors with a lower bin (bins are distinct)
+    int rank = 0;
+    while (smask) {
+      const int j = __builtin_ctzll(smask);
+      smask &= smask - 1;
+      const uint32_t kj = __builtin_amdgcn_readlane(c.k, j);
+      rank += (kj < c.k) ? 1 : 0;
     }
-  };
-  select_first_k<T, C::BSH, 4096>(a, cand, nc, thr, mu, sd, P, och, hist, gat, s_bstar, s_ngather, red, resolve);
-}
-
-
-template <int N2, int TC>
-__global__ void __launch_bounds__(256) k_fs_finish(FusedArgs a, FsWork w) {
-  __shared__ FsFinishShared<N2, TC> sh;
-  fs_finish_body<N2, TC>(a, w, blockIdx.x, sh.hist, sh.gat, sh.bstar, sh.ngather, sh.red);
+    const int kcap = (int)a.kcap;
+    if (surv && rank < kcap)
+      write_record(a.peaks + och * a.kcap + rank, c.k, make_float2(c.re, c.im), c.ml, c.mc, c.mr, a);
+    if (lane == 0) {
+      if (a.counts) {
+        gp(a.counts)[2 * och + 0] = (uint32_t)min(total, kcap);
+        gp(a.counts)[2 * och + 1] = (uint32_t)total;
+      }
+      if (a.stats) {
+        gp(a.stats)[4 * och + 0] = mu;
+        gp(a.stats)[4 * och + 1] = sd;
+        gp(a.stats)[4 * och + 2] = thr;
+        gp(a.stats)[4 * och + 3] = (float)P;
+      }
+    }
+  } else {
+    select_first_k<T, C::BSH, 4096>(a, [at, resolve](uint32_t i) {
+      Cand q = at(i);
+      resolve(q);
+      return q;
+    }, nc, thr, mu, sd, P, och, sh.hist, sh.gat, sh.bstar, sh.ngather, sh.red);
+  }
 }
 
 template <int N2, int TC>
@@ -419,7 +499,7 @@ static hipError_t launch_fs_n(const FusedArgs& a, void* ws, uint32_t chunk, hipS
   FsWork w;
   w.Z = reinterpret_cast<float2*>(p);            p += r((uint64_t)chunk * C::N * 8);
   w.cand = reinterpret_cast<Cand*>(p);           p += r((uint64_t)chunk * C::CAND_CAP * sizeof(Cand));
-  w.ncand = reinterpret_cast<uint32_t*>(p);      p += r((uint64_t)chunk * 4);
+  w.ncand = reinterpret_cast<uint32_t*>(p);      p += r((uint64_t)chunk * 8);
   w.pe = reinterpret_cast<double*>(p);           p += r((uint64_t)chunk * C::NT1 * 8);
   w.part = reinterpret_cast<FsRowPart*>(p);      p += r((uint64_t)chunk * C::NT2 * sizeof(FsRowPart));
   w.edge = reinterpret_cast<float*>(p);

@@ -232,27 +232,26 @@ __device__ __forceinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t 
 // and stats outputs of channel och.  One bucket per thread (T buckets of
 // 2^BSH bins) is histogrammed and scanned to find the bucket holding the K-th
 // survivor; survivors up to that bucket are gathered into gat (GMAX entries of
-// LDS) and bitonic-sorted by bin.  hist: T ints of LDS.  fix(cd) is applied
-// to every candidate read (it may complete it, or set mc = NaN to drop it).
-// Every thread of the T-thread workgroup calls it.
-struct CandAsIs {
-  __device__ __forceinline__ void operator()(Cand&) const {}
+// LDS) and bitonic-sorted by bin.  hist: T ints of LDS.  cand(i) returns
+// candidate i (a candidate with mc = NaN is no survivor).  Every thread of the
+// T-thread workgroup calls it.
+// candidate i of a list in global memory, as stored
+struct CandGlobal {
+  const Cand* c;
+  __device__ __forceinline__ Cand operator()(uint32_t i) const { return gp(c)[i]; }
 };
 
-template <int T, int BSH, int GMAX, class FIX = CandAsIs>
-__device__ __forceinline__ void select_first_k(const FusedArgs& a, const Cand* cand, uint32_t ncand, float thr,
-                                               float mu, float sd, double P, uint64_t och, int* hist, uint2* gat,
-                                               int& s_bstar, uint32_t& s_ngather, BlockRed& red, FIX fix = FIX()) {
+template <int T, int BSH, int GMAX, class SRC>
+__device__ __forceinline__ void select_first_k(const FusedArgs& a, SRC cand, uint32_t ncand, float thr, float mu,
+                                               float sd, double P, uint64_t och, int* hist, uint2* gat, int& s_bstar,
+                                               uint32_t& s_ngather, BlockRed& red) {
   const int tid0 = threadIdx.x;
   for (int i = opaque_v(tid0); i < T; i += T) hist[i] = 0;
   __syncthreads();
   for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
     Cand cd[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      cd[q] = gp(cand)[min(i0 + q * T, ncand - 1)];
-      fix(cd[q]);
-    }
+    for (int q = 0; q < 4; ++q) cd[q] = cand(min(i0 + q * T, ncand - 1));
 #pragma unroll
     for (int q = 0; q < 4; ++q)
       if (i0 + q * T < ncand && cd[q].mc > thr) atomicAdd(&hist[cd[q].k >> BSH], 1);
@@ -272,10 +271,7 @@ __device__ __forceinline__ void select_first_k(const FusedArgs& a, const Cand* c
   for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
     Cand cd[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      cd[q] = gp(cand)[min(i0 + q * T, ncand - 1)];
-      fix(cd[q]);
-    }
+    for (int q = 0; q < 4; ++q) cd[q] = cand(min(i0 + q * T, ncand - 1));
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const uint32_t i = i0 + q * T;
@@ -307,8 +303,7 @@ __device__ __forceinline__ void select_first_k(const FusedArgs& a, const Cand* c
   const int nout = min(total, kcap);
   MaxValue* out = a.peaks + (uint64_t)och * a.kcap;
   for (int i = opaque_v(tid0); i < nout && i < ng; i += T) {
-    Cand cd = gp(cand)[gat[i].y];
-    fix(cd);
+    const Cand cd = cand(gat[i].y);
     write_record(out + i, cd.k, make_float2(cd.re, cd.im), cd.ml, cd.mc, cd.mr, a);
   }
   if (tid0 == 0) {
@@ -628,7 +623,8 @@ __device__ __forceinline__ void large_channel(const FusedArgs& a, uint64_t ich, 
       if (a.gmax && gk != 0xffffffffu && bk == gk && bm == gm) write_record(a.gmax + och, bk, bx, bl, bm, br, a);
 
       const uint32_t ncand = min(sh.ncand, (uint32_t)C::CAND_CAP);
-      select_first_k<T, C::BSH, (C::LDS > 4096 ? 4096 : C::LDS)>(a, cand, ncand, thr, mu, sd, P, och, sh.hist,
+      select_first_k<T, C::BSH, (C::LDS > 4096 ? 4096 : C::LDS)>(a, CandGlobal{cand}, ncand, thr, mu, sd, P, och,
+                                                                sh.hist,
                                                                 reinterpret_cast<uint2*>(lds), sh.bstar,
                                                                 sh.ngather, sh.red);
     }
