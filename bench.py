@@ -3,11 +3,14 @@
 Workload (the metric's configuration): 1024 channels x 65536-point complex64,
 1-8 tones per channel at 0-20 dB SNR, Hann window, threshold mean + 6 std of
 |X|, first 64 local maxima in ascending bin order, parabolic refinement,
-global-maximum record.  One step = one launch over the whole batch, input
-already resident in HBM.
+global-maximum record.  One step = one call of the library over the whole
+batch (three launches: column transforms, row transforms with |X| and local
+maxima, per-channel finish), input already resident in HBM.
 
-Prints one JSON line: value = channels/s, roofline = algorithmic bytes /
-average kernel time vs 8 TB/s, cpu_baseline = the reference's CPU path (its
+Prints one JSON line: value = channels/s, roofline = algorithmic bytes of the
+call / its average time (hipEvents on the launch stream) vs 8 TB/s, with the
+HBM traffic of the three kernels from the committed rocprofv3 counter
+summary, cpu_baseline = the reference's CPU path (its
 pocketfft CpuFFT, compiled from the reference sources into oracle/_ref) plus
 the oracle's C peak rules, timed on a bounded sample on this host (one
 thread, plus a multi-core variant).  Extra objects on the same line (skipped
@@ -272,9 +275,9 @@ def main():
     alg_bytes = B * (8 * N + K * 32 + 8 + 32)
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "fused_large_n65536_b1024_traffic.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    tf = os.path.join(ROOT, "profiles", "fourstep_n65536_b1024_traffic.json")
+    if (B, N, K) == (1024, 65536, 64) and os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_step")
     if args.calibrate_copy:
         hip = C.CDLL("libamdhip64.so")
         hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
@@ -300,7 +303,8 @@ def main():
                                "parabolic, global max", "batch": B, "nfft": N, "max_peaks": K,
                    "peaks_found_mean": float(ct[:, 1].mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernels": "k_fs_cols + k_fs_rows + k_fs_finish (one call)"},
         "wall_s_timed": wall,
     }
     if not args.no_cpu:
