@@ -197,17 +197,19 @@ int spx_fft_mixed(spx_ctx* ctx, const void* in, void* out_c64, uint32_t batch, u
                   uint32_t dtype, uint32_t compute);
 int spx_fused_peaks_mixed(spx_ctx* ctx, const spx_fused_desc* desc, uint32_t compute);
 
-/* ---- uneven-length batch: one persistent launch, captured in a hipGraph ----
+/* ---- uneven-length batch: one hipGraph per plan ----
  * No counterpart in the reference, which builds one hipFFT plan per (nFFT,
  * beam_count) and runs lengths one after another
  * (src/fft_func/src/spectrum_processor_rocm.cpp:1129-1150).  Channels of
  * lengths 1024 .. 131072 (powers of two, any mix, any order) are searched with
- * the SPX_PEAKS_THRESHOLD rules of spx_fused_peaks (whole band, nfft = length)
- * in one launch of a persistent kernel over a cost-sorted work list.  The
- * radix order per length comes from an autotuner cache (JSON, written by
- * spx_batch_autotune); without one every length uses the default order.
- * A plan binds its buffers: execute re-runs the captured graph on whatever
- * the input buffer holds at that time. */
+ * the SPX_PEAKS_THRESHOLD rules of spx_fused_peaks (whole band, nfft = length):
+ * lengths up to 16384 in one launch of a persistent kernel over a cost-sorted
+ * work list, longer lengths by the split four-step launches of spx_fused_peaks
+ * reading each channel at its own offset; the plan captures all of it in one
+ * hipGraph.  The radix order per length up to 16384 comes from an autotuner
+ * cache (JSON, written by spx_batch_autotune); without one every length uses
+ * the default order.  A plan binds its buffers: execute re-runs the captured
+ * graph on whatever the input buffer holds at that time. */
 typedef struct spx_batch_plan spx_batch_plan;
 
 #define SPX_BATCH_NO_GRAPH 1u   /* launch the kernels directly on every execute */

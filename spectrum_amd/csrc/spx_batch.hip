@@ -1,20 +1,18 @@
-// Persistent uneven-length batch: every channel of a batch whose lengths range
-// over 2^10 .. 2^17 runs through the fused window -> FFT -> |X| -> threshold
-// peak search in ONE launch (and, through the host plan, one hipGraph node).
+// Persistent uneven-length batch: every channel of a batch whose length is
+// 2^10 .. 2^14 runs through the fused window -> FFT -> |X| -> threshold peak
+// search in ONE launch (the host plan captures it in a hipGraph together with
+// the split-path launches of the longer lengths).
 //
-// The host builds a work list sorted by decreasing cost (four-step channels of
-// the longest length first, LDS-resident tiles of the shortest last: a
-// longest-processing-time-first schedule) and the grid is one 1024-thread
-// workgroup per CU.  Each workgroup takes the next item from a global cursor
-// (one relaxed atomic per item, no other cross-workgroup traffic):
-//   * lengths 1024 .. 16384: a tile of G = 16384/L whole channels in LDS,
-//     the same Stockham passes as the single-channel kernel but over G
-//     sequences, then per-channel (segmented) statistics, peak flags and
-//     scan, so every channel's records equal the single-channel kernel's;
-//   * lengths 32768 .. 131072: the four-step channel body of the fused large
-//     kernel, in this workgroup's scratch slot.
-// The radix order of each length (Plan<L, V>) is chosen per bucket by the
-// host autotuner and carried in the item word.
+// The host builds a work list sorted by decreasing cost (tiles of the longest
+// length first: a longest-processing-time-first schedule) and the grid is one
+// 1024-thread workgroup per CU.  Each workgroup takes the next item from a
+// global cursor (one relaxed atomic per item, no other cross-workgroup
+// traffic); an item is a tile of G = 16384/L whole channels in LDS: the same
+// Stockham passes as the single-channel kernel but over G sequences, then
+// per-channel (segmented) statistics, peak flags and scan, so every channel's
+// records equal the single-channel kernel's.  The radix order of each length
+// (Plan<L, V>) is chosen per bucket by the host autotuner and carried in the
+// item word.
 #include "spx_fused_common.hpp"
 
 namespace spx {
@@ -198,53 +196,21 @@ __device__ __forceinline__ void batch_small_tile(const BatchArgs& b, const uint4
   __syncthreads();  // LDS and red are reused by the next item
 }
 
-template <int N2, int V>
-__device__ __forceinline__ void batch_large_item(const BatchArgs& b, const uint4 it, unsigned char* slot,
-                                                 unsigned char* smem) {
-  constexpr uint32_t N = 256u * N2;
-  const uint32_t bucket = it.x & 0xffu;
-  const uint32_t ch = gp(b.perm)[it.y];
-  FusedArgs a;
-  a.in = chan_base(b, ch);
-  a.in_scale = nullptr;
-  a.batch = 1;
-  a.n_point = N;
-  a.nfft = N;
-  a.dtype = b.dtype;
-  a.window = b.bk[bucket].window;
-  a.fs = b.fs;
-  a.spec_out = nullptr;
-  a.mag_out = nullptr;
-  a.mag_kind = 0;
-  a.mag_scale = 1.0f;
-  a.peak_mode = 1;
-  a.nsigma = b.nsigma;
-  a.kcap = b.kcap;
-  a.s0 = 0;
-  a.s1 = N;
-  a.refine = b.refine;
-  a.peaks = b.peaks;
-  a.counts = b.counts;
-  a.gmax = b.gmax;
-  a.stats = b.stats;
-  a.tw_a = b.tw256;
-  a.tw_b = b.bk[bucket].tw;
-  a.tw_4 = b.bk[bucket].tw4;
-  a.scratch = b.scratch;
-  a.slot_bytes = b.slot_bytes;
-  a.mlp = nullptr;
-  large_channel<N2, 1, V, 1024, true>(a, 0, ch, slot, *reinterpret_cast<LargeShared<N2, 1024, true>*>(smem));
-}
-
 __global__ void k_batch_reset(uint32_t* counter) {
   if (threadIdx.x == 0) gp(counter)[0] = 0u;
 }
 
+// LDS of a tile: every length's LayContig<L, 16384/L> has the same size
+struct BatchShared {
+  float2 tile[LayContig<1024, 16>::SIZE];
+  BlockRed red;
+};
+static_assert(LayContig<16384, 1>::SIZE == LayContig<1024, 16>::SIZE, "one tile size for every length");
+
 __global__ void __launch_bounds__(1024) k_batch(BatchArgs b) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ uint32_t item_s;
-  LargeShared<512>& sh = *reinterpret_cast<LargeShared<512>*>(smem_raw);
-  unsigned char* slot = reinterpret_cast<unsigned char*>(b.scratch) + (uint64_t)blockIdx.x * b.slot_bytes;
+  BatchShared& sh = *reinterpret_cast<BatchShared*>(smem_raw);
   for (;;) {
     if (threadIdx.x == 0) item_s = atomicAdd(b.counter, 1u);
     __syncthreads();
@@ -265,11 +231,6 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs b) {
       case 0x103: batch_small_tile<8192, 1>(b, it, sh.tile, sh.red); break;
       case 0x004: batch_small_tile<16384, 0>(b, it, sh.tile, sh.red); break;
       case 0x104: batch_small_tile<16384, 1>(b, it, sh.tile, sh.red); break;
-      case 0x005: batch_large_item<128, 0>(b, it, slot, smem_raw); break;
-      case 0x105: batch_large_item<128, 1>(b, it, slot, smem_raw); break;
-      case 0x006: batch_large_item<256, 0>(b, it, slot, smem_raw); break;
-      case 0x007: batch_large_item<512, 0>(b, it, slot, smem_raw); break;
-      case 0x107: batch_large_item<512, 1>(b, it, slot, smem_raw); break;
       default: break;  // the host validates every item word
     }
   }
@@ -279,15 +240,8 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs b) {
 
 extern "C" {
 
-// Dynamic LDS of the batch kernel: the largest per-item layout (each item
-// reinterprets the buffer with its own layout; nothing carries over).
-static size_t batch_smem() {
-  using namespace spx;
-  size_t m = sizeof(LargeShared<512>);
-  if (sizeof(LargeShared<256>) > m) m = sizeof(LargeShared<256>);
-  if (sizeof(LargeShared<128>) > m) m = sizeof(LargeShared<128>);
-  return m;
-}
+// Dynamic LDS of the batch kernel (one tile).
+static size_t batch_smem() { return sizeof(spx::BatchShared); }
 
 // Allows the batch kernel its dynamic LDS; call once per process before the
 // first launch and outside any stream capture.

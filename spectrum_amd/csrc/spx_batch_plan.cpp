@@ -1,5 +1,8 @@
-// Host side of the uneven-length batch: work-list planning, the radix-order
-// autotuner and its JSON cache, and hipGraph capture of the persistent launch.
+// Host side of the uneven-length batch: work-list planning for the persistent
+// LDS-tile launch (lengths 2^10 .. 2^14), the split-path launch sequences of
+// the longer lengths (2^15 .. 2^17, one per length, channel indirection
+// through the plan's permutation and offsets), the radix-order autotuner and
+// its JSON cache, and hipGraph capture of the whole sequence.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +18,13 @@
 
 using namespace spx_host;
 
+// one length of the batch on the split path
+struct SplitGroup {
+  FusedArgs fa;      // batch = count, in = the packed input, tables of the length
+  uint32_t first;    // its channels are perm[first .. first + count)
+  uint32_t chunk;    // channels per launch sequence
+};
+
 struct spx_batch_plan {
   spx_ctx* c = nullptr;
   BatchArgs args{};
@@ -25,6 +35,7 @@ struct spx_batch_plan {
   void* d_counter = nullptr;
   void* d_scratch = nullptr;
   uint64_t scratch_bytes = 0;
+  std::vector<SplitGroup> split;
   bool graph = false;
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
@@ -49,7 +60,8 @@ uint32_t tuned_len(int bucket) {
   return bucket >= SPX_BATCH_FIRST_LARGE ? L / 256 : L;
 }
 
-bool has_variants(int bucket) { return (log2u(tuned_len(bucket)) % 4) != 0; }
+// the split path of the long lengths has one radix order (radix 16 first)
+bool has_variants(int bucket) { return bucket < SPX_BATCH_FIRST_LARGE && (log2u(tuned_len(bucket)) % 4) != 0; }
 
 // Cache format (written by spx_batch_autotune):
 // {"device": "...", "cu": N, "lengths": [{"length": L, "order": V, "us": [t0, t1], "channels": n}, ...]}
@@ -101,6 +113,18 @@ void release(spx_batch_plan* p) {
   delete p;
 }
 
+// The plan's launches in stream order: the split-path sequence of every long
+// length (the longest work), then the persistent LDS-tile launch.
+hipError_t launch_all(spx_batch_plan* p, hipStream_t st) {
+  for (const SplitGroup& g : p->split) {
+    hipError_t e = spx_launch_fourstep_ex(&g.fa, p->d_scratch, g.chunk, (const uint32_t*)p->d_perm + g.first,
+                                          (const uint64_t*)p->d_offsets, st);
+    if (e != hipSuccess) return e;
+  }
+  if (p->n_items) return spx_launch_batch(&p->args, p->grid, st);
+  return hipSuccess;
+}
+
 // Builds a plan; bucket_only >= 0 restricts it to one length and forces
 // `force_variant` there (the autotuner's measurement plans).
 int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_variant, bool graph, spx_batch_plan** out) {
@@ -118,8 +142,8 @@ int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_varian
     if (bucket_only >= 0 && b != bucket_only) continue;
     by[b].push_back(ch);
   }
-  // work list, longest-processing-time first: four-step channels from the
-  // longest length down, then LDS tiles from the longest length down
+  // long lengths: one split-path group each (their channels first in perm);
+  // short lengths: work list of LDS tiles, longest-processing-time first
   std::vector<uint32_t> perm;
   std::vector<uint32_t> items;  // 4 words per item
   for (int b = SPX_BATCH_BUCKETS - 1; b >= 0; --b) {
@@ -127,17 +151,27 @@ int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_varian
     p->chans_per_bucket[b] = (uint32_t)v.size();
     if (v.empty()) continue;
     const uint32_t L = 1u << (SPX_BATCH_LOG_MIN + b);
-    const uint32_t G = b >= SPX_BATCH_FIRST_LARGE ? 1u : 16384u / L;
-    const uint32_t word = (uint32_t)b | ((uint32_t)p->variant[b] << 8);
-    for (size_t i = 0; i < v.size(); i += G) {
-      const uint32_t cnt = (uint32_t)std::min<size_t>(G, v.size() - i);
-      items.insert(items.end(), {word, (uint32_t)(perm.size() + i), cnt, 0u});
-      p->items_per_bucket[b]++;
+    if (b >= SPX_BATCH_FIRST_LARGE) {
+      SplitGroup g;
+      std::memset(&g.fa, 0, sizeof(g.fa));
+      g.fa.batch = (uint32_t)v.size();
+      g.first = (uint32_t)perm.size();
+      g.chunk = fourstep_chunk(L, g.fa.batch);
+      p->split.push_back(g);
+      p->items_per_bucket[b] = (g.fa.batch + g.chunk - 1) / g.chunk;
+    } else {
+      const uint32_t G = 16384u / L;
+      const uint32_t word = (uint32_t)b | ((uint32_t)p->variant[b] << 8);
+      for (size_t i = 0; i < v.size(); i += G) {
+        const uint32_t cnt = (uint32_t)std::min<size_t>(G, v.size() - i);
+        items.insert(items.end(), {word, (uint32_t)(perm.size() + i), cnt, 0u});
+        p->items_per_bucket[b]++;
+      }
     }
     perm.insert(perm.end(), v.begin(), v.end());
   }
   p->n_items = (uint32_t)(items.size() / 4);
-  if (p->n_items == 0) {
+  if (p->n_items == 0 && p->split.empty()) {
     release(p);
     return fail(c, SPX_EINVAL, "spx_batch: no channels in the requested length");
   }
@@ -149,11 +183,12 @@ int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_varian
   hipError_t e;
   if ((e = hipSetDevice(c->device)) != hipSuccess) return fail_hip(e, "hipSetDevice");
   const size_t off_bytes = (size_t)d->n_chan * sizeof(uint64_t);
-  if ((e = hipMalloc(&p->d_items, items.size() * 4)) != hipSuccess) return fail_hip(e, "hipMalloc(items)");
+  if ((e = hipMalloc(&p->d_items, std::max<size_t>(items.size(), 4) * 4)) != hipSuccess)
+    return fail_hip(e, "hipMalloc(items)");
   if ((e = hipMalloc(&p->d_perm, perm.size() * 4)) != hipSuccess) return fail_hip(e, "hipMalloc(perm)");
   if ((e = hipMalloc(&p->d_offsets, off_bytes)) != hipSuccess) return fail_hip(e, "hipMalloc(offsets)");
   if ((e = hipMalloc(&p->d_counter, 256)) != hipSuccess) return fail_hip(e, "hipMalloc(counter)");
-  if ((e = hipMemcpy(p->d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+  if (!items.empty() && (e = hipMemcpy(p->d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
     return fail_hip(e, "upload items");
   if ((e = hipMemcpy(p->d_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
     return fail_hip(e, "upload perm");
@@ -177,8 +212,7 @@ int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_varian
   a.gmax = d->gmax_out;
   a.stats = d->stats_out;
   const void* t = nullptr;
-  uint64_t slot = 0;
-  for (int b = 0; b < SPX_BATCH_BUCKETS; ++b) {
+  for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b) {
     if (by[b].empty()) continue;
     const uint32_t L = 1u << (SPX_BATCH_LOG_MIN + b);
     if ((rc = get_table(c, TW, tuned_len(b), (uint32_t)p->variant[b], &t))) { release(p); return rc; }
@@ -186,34 +220,52 @@ int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_varian
     const float* w = nullptr;
     if ((rc = window_ptr(c, d->window, L, &w))) { release(p); return rc; }
     a.bk[b].window = w;
-    if (b >= SPX_BATCH_FIRST_LARGE) {
-      if ((rc = get_table(c, TW4, L, 0, &t))) { release(p); return rc; }
-      a.bk[b].tw4 = (const float2*)t;
-      slot = std::max<uint64_t>(slot, spx_large_slot_bytes(L));
-      if (!a.tw256) {
-        if ((rc = get_table(c, TW, 256, 0, &t))) { release(p); return rc; }
-        a.tw256 = (const float2*)t;
-      }
-    }
+  }
+  uint64_t ws = 0;
+  for (SplitGroup& g : p->split) {
+    const uint32_t L = d->lengths[perm[g.first]];
+    FusedArgs& fa = g.fa;
+    fa.in = d->input;
+    fa.n_point = L;
+    fa.nfft = L;
+    fa.dtype = d->dtype;
+    if ((rc = window_ptr(c, d->window, L, &fa.window))) { release(p); return rc; }
+    fa.fs = d->sample_rate;
+    fa.mag_scale = 1.0f;
+    fa.peak_mode = 1;
+    fa.nsigma = d->nsigma;
+    fa.kcap = d->max_peaks;
+    fa.s0 = 0;
+    fa.s1 = L;
+    fa.refine = d->refine;
+    fa.peaks = d->peaks_out;
+    fa.counts = d->counts_out;
+    fa.gmax = d->gmax_out;
+    fa.stats = d->stats_out;
+    if ((rc = get_table(c, TW, 256, 0, &t))) { release(p); return rc; }
+    fa.tw_a = (const float2*)t;
+    if ((rc = get_table(c, TW, L / 256, 0, &t))) { release(p); return rc; }
+    fa.tw_b = (const float2*)t;
+    if ((rc = get_table(c, TW4, L, 0, &t))) { release(p); return rc; }
+    fa.tw_4 = (const float2*)t;
+    ws = std::max<uint64_t>(ws, spx_fs_workspace_bytes(L, g.chunk));
   }
   if ((e = spx_batch_kernel_init()) != hipSuccess) return fail_hip(e, "spx_batch_kernel_init");
   p->grid = c->cu_count > 0 ? c->cu_count : 256;
   if ((uint32_t)p->grid > p->n_items) p->grid = (int)p->n_items;
-  a.slot_bytes = slot;
-  if (slot) {
-    p->scratch_bytes = slot * (uint64_t)p->grid;
+  if (ws) {
+    p->scratch_bytes = ws;
     if ((e = hipMalloc(&p->d_scratch, p->scratch_bytes)) != hipSuccess) {
       release(p);
-      return fail(c, SPX_ENOMEM, "spx_batch: scratch allocation failed");
+      return fail(c, SPX_ENOMEM, "spx_batch: workspace allocation failed");
     }
-    a.scratch = p->d_scratch;
   }
   if (graph) {
     if ((e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return fail_hip(e, "hipStreamBeginCapture");
-    hipError_t le = spx_launch_batch(&a, p->grid, c->stream);
+    hipError_t le = launch_all(p, c->stream);
     e = hipStreamEndCapture(c->stream, &p->g);
-    if (le != hipSuccess) return fail_hip(le, "spx_launch_batch (capture)");
+    if (le != hipSuccess) return fail_hip(le, "batch launches (capture)");
     if (e != hipSuccess) return fail_hip(e, "hipStreamEndCapture");
     if ((e = hipGraphInstantiate(&p->ge, p->g, nullptr, nullptr, 0)) != hipSuccess)
       return fail_hip(e, "hipGraphInstantiate");
@@ -229,7 +281,7 @@ int run(spx_batch_plan* p) {
   if (p->graph) {
     SPX_HIP(c, hipGraphLaunch(p->ge, c->stream));
   } else {
-    SPX_HIP(c, spx_launch_batch(&p->args, p->grid, c->stream));
+    SPX_HIP(c, launch_all(p, c->stream));
   }
   return SPX_OK;
 }
@@ -265,7 +317,7 @@ int spx_batch_plan_info(spx_batch_plan* p, char* buf, size_t len) {
     if (!p->chans_per_bucket[b]) continue;
     o << (first ? "" : ", ") << "{\"length\": " << (1u << (SPX_BATCH_LOG_MIN + b)) << ", \"order\": " << p->variant[b]
       << ", \"channels\": " << p->chans_per_bucket[b] << ", \"items\": " << p->items_per_bucket[b]
-      << ", \"path\": \"" << (b >= SPX_BATCH_FIRST_LARGE ? "four-step" : "lds-tile") << "\"}";
+      << ", \"path\": \"" << (b >= SPX_BATCH_FIRST_LARGE ? "split" : "lds-tile") << "\"}";
     first = false;
   }
   o << "]}";

@@ -32,6 +32,10 @@ struct FsWork {
   FsRowPart* part;    // [nch][NT2]
   float* edge;        // [nch][NT2][2][256]  |X| of block rows 0 and RB-1
   uint32_t ch0, nch;  // chunk: channels ch0 .. ch0+nch-1 of the launch
+  // optional indirection (uneven batches): channel i of the launch is channel
+  // chan[i] of the caller, whose input starts inoff[chan[i]] elements into a.in
+  const uint32_t* chan;
+  const uint64_t* inoff;
 };
 
 template <int N2, int TC>
@@ -133,12 +137,18 @@ __global__ void __launch_bounds__(TC) k_fs_cols(FusedArgs a, FsWork w) {
   __shared__ double red[TC / 64];
   const int tid = threadIdx.x;
   const uint32_t ci = blockIdx.x / NT1, t = blockIdx.x % NT1;
-  const uint64_t ich = (uint64_t)w.ch0 + ci;
+  uint64_t ich = (uint64_t)w.ch0 + ci;
+  FusedArgs ai = a;
+  if (w.chan != nullptr) {
+    const uint64_t es = a.dtype == SPX_DTYPE_C64 ? 8 : 4;
+    ai.in = reinterpret_cast<const char*>(a.in) + gp(w.inoff)[gp(w.chan)[ich]] * es;
+    ich = 0;
+  }
   float2 v[PPT];
   {
     const int c = tid % CB, jj = tid / CB;  // jj in [0, N2/16)
     const int n1 = t * CB + c;
-    load_run<16>(a, ich, (uint32_t)(n1 + N1 * jj), (uint32_t)(N1 * (N2 / 16)), v);
+    load_run<16>(ai, ich, (uint32_t)(n1 + N1 * jj), (uint32_t)(N1 * (N2 / 16)), v);
   }
   float e = 0.f;
 #pragma unroll
@@ -374,7 +384,7 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
   __shared__ FsFinishShared<N2, TC> sh;
   const int lane = threadIdx.x;
   const uint32_t ci = blockIdx.x;
-  const uint64_t och = (uint64_t)w.ch0 + ci;
+  const uint64_t och = w.chan != nullptr ? (uint64_t)gp(w.chan)[w.ch0 + ci] : (uint64_t)w.ch0 + ci;
   const Cand* cand = w.cand + (uint64_t)ci * C::CAND_CAP;
   const float* edge = w.edge + (uint64_t)ci * NT2 * 2 * C::N1;
   // every load of the channel at once: its high-list candidate (speculative),
@@ -492,7 +502,8 @@ static uint64_t fs_channel_bytes() {
 }
 
 template <int N2>
-static hipError_t launch_fs_n(const FusedArgs& a, void* ws, uint32_t chunk, hipStream_t st) {
+static hipError_t launch_fs_n(const FusedArgs& a, void* ws, uint32_t chunk, const uint32_t* chan,
+                              const uint64_t* inoff, hipStream_t st) {
   using C = FsCfg<N2, FsShape<N2>::TC>;
   auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
   unsigned char* p = reinterpret_cast<unsigned char*>(ws);
@@ -505,6 +516,8 @@ static hipError_t launch_fs_n(const FusedArgs& a, void* ws, uint32_t chunk, hipS
   w.edge = reinterpret_cast<float*>(p);
   w.ch0 = 0;
   w.nch = 0;
+  w.chan = chan;
+  w.inoff = inoff;
   return launch_fs<N2, FsShape<N2>::TC>(a, w, chunk, st);
 }
 
@@ -529,16 +542,24 @@ uint64_t spx_fs_workspace_bytes(uint32_t nfft, uint32_t chunk) {
 
 // Runs the split path over a.batch channels in chunks of `chunk` channels,
 // reusing the workspace `ws` (spx_fs_workspace_bytes(nfft, chunk) bytes).
-hipError_t spx_launch_fourstep(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st) {
+// chan / inoff (device arrays, or both null): channel i of the launch is the
+// caller's channel chan[i], its input at a.in + inoff[chan[i]] elements, its
+// outputs at index chan[i].
+hipError_t spx_launch_fourstep_ex(const FusedArgs* a, void* ws, uint32_t chunk, const uint32_t* chan,
+                                  const uint64_t* inoff, hipStream_t st) {
   using namespace spx;
-  if (chunk == 0) return hipErrorInvalidValue;
+  if (chunk == 0 || (chan == nullptr) != (inoff == nullptr)) return hipErrorInvalidValue;
   switch (a->nfft) {
-    case 1u << 15: return launch_fs_n<128>(*a, ws, chunk, st);
-    case 1u << 16: return launch_fs_n<256>(*a, ws, chunk, st);
-    case 1u << 17: return launch_fs_n<512>(*a, ws, chunk, st);
-    case 1u << 18: return launch_fs_n<1024>(*a, ws, chunk, st);
+    case 1u << 15: return launch_fs_n<128>(*a, ws, chunk, chan, inoff, st);
+    case 1u << 16: return launch_fs_n<256>(*a, ws, chunk, chan, inoff, st);
+    case 1u << 17: return launch_fs_n<512>(*a, ws, chunk, chan, inoff, st);
+    case 1u << 18: return launch_fs_n<1024>(*a, ws, chunk, chan, inoff, st);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t spx_launch_fourstep(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st) {
+  return spx_launch_fourstep_ex(a, ws, chunk, nullptr, nullptr, st);
 }
 
 }  // extern "C"

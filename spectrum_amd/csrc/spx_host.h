@@ -37,6 +37,8 @@ int hip_fail(spx_ctx* c, hipError_t e, const char* what);
 // once per context.
 int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out);
 int window_ptr(spx_ctx* c, uint32_t window, uint32_t n_point, const float** w);
+// Channels per launch group of the split long-channel path (bounds its workspace).
+uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch);
 
 }  // namespace spx_host
 

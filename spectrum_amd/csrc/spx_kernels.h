@@ -44,18 +44,17 @@ struct FusedArgs {
   float wc0, wc1, wc2;
 };
 
-// Uneven-length batch (one persistent launch over a work list).
-// Bucket b holds length L = 2^(10+b): b 0..4 run as tiles of 16384/L channels
-// (whole channels in LDS), b 5..7 one channel per item on the four-step path.
+// Uneven-length batch.  Bucket b holds length L = 2^(10+b): b 0..4 run in
+// one persistent launch over a work list of tiles of 16384/L channels (whole
+// channels in LDS); b 5..7 run on the split four-step path, one launch
+// sequence per length with channel indirection (spx_launch_fourstep_ex).
 #define SPX_BATCH_BUCKETS 8
 #define SPX_BATCH_LOG_MIN 10
 #define SPX_BATCH_FIRST_LARGE 5
 
 struct BatchBucket {
-  const float2* tw;        // small: tables of plan (L, V); large: tables of the column length N2 = L/256 under V
+  const float2* tw;        // tables of plan (L, V)
   const float* window;     // [L] or null
-  const float2* tw4;       // large only: W_L^(n1*k2) at [k2*256 + n1]
-  uint64_t pad;
 };
 
 struct BatchArgs {
@@ -64,16 +63,13 @@ struct BatchArgs {
   const uint32_t* perm;    // channel ids in work-list order
   const uint4* items;      // {bucket | variant << 8, first index into perm, channel count, 0}
   uint32_t* counter;       // work-list cursor (zeroed by spx_launch_batch before the batch kernel)
-  const float2* tw256;     // large path: tables of the 256-point row transform
-  BatchBucket bk[SPX_BATCH_BUCKETS];
+  BatchBucket bk[SPX_BATCH_FIRST_LARGE];
   uint32_t n_items, dtype, kcap, refine;
   float fs, nsigma;
   spx_max_value* peaks;    // [n_chan][kcap]
   uint32_t* counts;        // [n_chan][2]
   spx_max_value* gmax;     // [n_chan] or null
   float* stats;            // [n_chan][4] or null
-  void* scratch;           // grid * slot_bytes (large items)
-  uint64_t slot_bytes;
 };
 
 extern "C" {
@@ -98,6 +94,8 @@ int spx_fused_is_large(uint32_t nfft);
 int spx_large_groups_per_cu(uint32_t nfft);
 uint64_t spx_fs_workspace_bytes(uint32_t nfft, uint32_t chunk);
 hipError_t spx_launch_fourstep(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st);
+hipError_t spx_launch_fourstep_ex(const FusedArgs* a, void* ws, uint32_t chunk, const uint32_t* chan,
+                                  const uint64_t* inoff, hipStream_t st);
 int spx_twiddle_table_len(uint32_t L, int V);
 hipError_t spx_launch_mag_phase(const void* in, void* out, uint64_t total, hipStream_t st);
 hipError_t spx_launch_magnitude(const void* in, float* out, uint64_t total, float inv_n, int squared,
