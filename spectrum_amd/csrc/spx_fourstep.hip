@@ -100,6 +100,7 @@ __device__ __forceinline__ double fs_energy(const FsWork& w, uint32_t ci) {
 // of 320 floats, so the skew never reaches the next row): the pass-0 writes
 // (a = jj, b = r) and the pass-1 reads (a = r, b = jj) of a wave both fall on
 // 64 distinct banks.
+template <bool TWL>
 __device__ __forceinline__ void fft256_cols16_halfx(float* lf, float2 (&v)[16], const float2* tw, int tid) {
   using PL = Plan<256, 0>;
   static_assert(PL::NP == 2 && PL::radix(0) == 16 && PL::radix(1) == 16, "plan");
@@ -119,7 +120,8 @@ __device__ __forceinline__ void fft256_cols16_halfx(float* lf, float2 (&v)[16], 
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r].y = rb[320 * r + 16 * (r & 3)];
   float2 w[16];
-  load_tw_row<16, false>(tw + PL::twoff(1) + (uint32_t)(jj * 16), w);
+  if constexpr (TWL) load_tw_row<16, true>(tw + PL::twoff_lds(1) + jj * 18, w);
+  else load_tw_row<16, false>(tw + PL::twoff(1) + (uint32_t)(jj * 16), w);
 #pragma unroll
   for (int r = 1; r < 16; ++r) v[r] = c_mul(v[r], w[r]);
   Dft<16>::run(v);
@@ -134,8 +136,10 @@ __global__ void __launch_bounds__(TC) k_fs_cols(FusedArgs a, FsWork w) {
   using LAY1 = typename C::LAY1;
   constexpr bool HALFX = (N2 == 256 && CB == 16);
   __shared__ __align__(16) float2 tile[HALFX ? 5104 / 2 : LAY1::SIZE];
+  __shared__ __align__(16) float2 twl[HALFX ? Plan<256, 0>::TWSIZE_LDS : 1];
   __shared__ double red[TC / 64];
   const int tid = threadIdx.x;
+  if constexpr (HALFX) copy_tw_to_lds<256, 0, TC>(a.tw_b, twl, tid);  // read after the exchange's barriers
   const uint32_t ci = blockIdx.x / NT1, t = blockIdx.x % NT1;
   uint64_t ich = (uint64_t)w.ch0 + ci;
   FusedArgs ai = a;
@@ -153,7 +157,7 @@ __global__ void __launch_bounds__(TC) k_fs_cols(FusedArgs a, FsWork w) {
   float e = 0.f;
 #pragma unroll
   for (int r = 0; r < PPT; ++r) e = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, e));
-  if constexpr (HALFX) fft256_cols16_halfx(reinterpret_cast<float*>(tile), v, a.tw_b, tid);
+  if constexpr (HALFX) fft256_cols16_halfx<true>(reinterpret_cast<float*>(tile), v, twl, tid);
   else fft_tile<LAY1, N2, CB, TC, PPT, 0, false>(tile, v, a.tw_b, tid);
   float2* Z = w.Z + (uint64_t)ci * C::N;
 #pragma unroll
@@ -177,12 +181,25 @@ __global__ void __launch_bounds__(TC) k_fs_cols(FusedArgs a, FsWork w) {
   }
 }
 
+// Orders a wave's LDS writes before its later LDS reads (a wave's LDS
+// operations complete in order; this only stops the compiler from moving
+// them), for exchanges that stay inside one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // 256-point transforms of the RB rows of a LayContig<256, RB> tile, 16
 // points per thread, with the one LDS exchange between the two radix-16
 // passes done in halves (real parts, then imaginary parts): the tile needs 4
 // bytes per point.  In: v[r] = x[s][jj + 16 r] for s = tid/16, jj = tid%16.
-// Out: v[r] = X[s][jj + 16 r].  tw: the twiddle tables of Plan<256, 0>.
-template <int RB, int T>
+// Out: v[r] = X[s][jj + 16 r].  tw: the twiddle tables of Plan<256, 0>, in
+// global memory or (TWL) the padded LDS copy of copy_tw_to_lds.
+// A row is 16 consecutive lanes, so every exchange stays inside one wave
+// (rows 4w .. 4w+3 of the tile belong to wave w) and needs no workgroup
+// barrier.
+template <int RB, int T, bool TWL>
 __device__ __forceinline__ void fft256_rows_halfx(float* lf, float2 (&v)[16], const float2* tw, int tid) {
   using LAY = LayContig<256, RB>;
   using PL = Plan<256, 0>;
@@ -194,18 +211,19 @@ __device__ __forceinline__ void fft256_rows_halfx(float* lf, float2 (&v)[16], co
   const float* rbase = lf + LAY::idx(s, jj);  // pass-1 inputs y[s][jj + 16 r], stride 17
 #pragma unroll
   for (int r = 0; r < 16; ++r) wb[r] = v[r].x;
-  __syncthreads();
+  wave_lds_sync();
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r].x = rbase[17 * r];
-  __syncthreads();
+  wave_lds_sync();
 #pragma unroll
   for (int r = 0; r < 16; ++r) wb[r] = v[r].y;
-  __syncthreads();
+  wave_lds_sync();
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r].y = rbase[17 * r];
   // pass 1: sub-transform length 16, twiddles W_256^(r (jj % 16))
   float2 w[16];
-  load_tw_row<16, false>(tw + PL::twoff(1) + (uint32_t)(jj * 16), w);
+  if constexpr (TWL) load_tw_row<16, true>(tw + PL::twoff_lds(1) + jj * 18, w);
+  else load_tw_row<16, false>(tw + PL::twoff(1) + (uint32_t)(jj * 16), w);
 #pragma unroll
   for (int r = 1; r < 16; ++r) v[r] = c_mul(v[r], w[r]);
   Dft<16>::run(v);
@@ -218,6 +236,7 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
   constexpr int N1 = C::N1, RB = C::RB, NT2 = C::NT2, PPT = C::PPT, T = C::TR;
   using LAY2 = typename C::LAY2;
   __shared__ __align__(16) float ldsf[LAY2::SIZE];
+  __shared__ __align__(16) float2 twl[Plan<256, 0>::TWSIZE_LDS];
   __shared__ BlockRed red;
   const int tid = threadIdx.x;
   const uint32_t ci = blockIdx.x / NT2, t = blockIdx.x % NT2;
@@ -231,7 +250,9 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
   const bool full_band = (a.s0 == 0 && a.s1 == (uint32_t)C::N);
   const FsBounds bnd = fs_bounds(a, fs_energy<N2, TC>(w, ci), full_band);
   const float lb = bnd.lb;
-  fft256_rows_halfx<RB, T>(ldsf, v, a.tw_a, tid);
+  copy_tw_to_lds<256, 0, T>(a.tw_a, twl, tid);
+  __syncthreads();
+  fft256_rows_halfx<RB, T, true>(ldsf, v, twl, tid);
   // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]  ->  bin k2 + N2*k1 (ascending in r)
   const uint32_t k2 = t * RB + s_row;
   float m[PPT];
@@ -261,7 +282,7 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
       }
     }
   }
-  __syncthreads();  // LDS free again
+  wave_lds_sync();  // the wave's rows of the tile are free again
 #pragma unroll
   for (int r = 0; r < PPT; ++r) ldsf[LAY2::idx(s_row, jj_row + 16 * r)] = m[r];
   if (s_row == 0 || s_row == RB - 1) {
@@ -324,7 +345,7 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
   double s1 = fs1, s2 = fs2;
   float gm = bm;
   uint32_t gk = bk;
-  block_stats<T>(s1, s2, gm, gk, red);
+  block_stats<T, false>(s1, s2, gm, gk, red);
   FsRowPart* pp = w.part + (uint64_t)ci * NT2 + t;
   if (tid == 0) {
     gp(&pp->s1)[0] = s1;

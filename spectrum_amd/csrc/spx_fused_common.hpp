@@ -157,8 +157,9 @@ __device__ __forceinline__ float threshold_of(const FusedArgs& a, double s1, dou
 }
 
 // Block-wide sum of two doubles and argmax (lowest index wins ties).
-// Result valid in all threads on return.
-template <int T>
+// Result valid in all threads on return.  REUSE: end with a barrier so that
+// `red` may be written again (not needed when it is the last use).
+template <int T, bool REUSE = true>
 __device__ __forceinline__ void block_stats(double& s1, double& s2, float& bm, uint32_t& bk, BlockRed& red) {
   constexpr int NW = (T + 63) / 64;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -185,7 +186,7 @@ __device__ __forceinline__ void block_stats(double& s1, double& s2, float& bm, u
       bk = red.k[i];
     }
   }
-  __syncthreads();
+  if constexpr (REUSE) __syncthreads();
 }
 
 // Block exclusive scan of per-thread counts; returns the offset, total in `total`.
