@@ -10,18 +10,20 @@
 //    |X|, mean/std, local maxima, the block scan and the records are produced
 //    from the LDS copy of the spectrum; nothing but records leaves the chip.
 //
-//  * large  (N = 256 * N2, N >= 32768): persistent grid, one workgroup per
-//    CU, channels strided over workgroups.  Phase 1 runs N2-point column FFTs
-//    (16384/N2 columns per tile, interleaved LDS layout) and writes the
-//    twiddled intermediate Z[k2][n1] to a per-workgroup scratch slot; phase 2
-//    runs 256-point row FFTs; local maxima of interior rows are decided from
-//    the LDS magnitude tile, tile-boundary rows from a small edge buffer.
-//    Only local maxima that can clear the threshold are kept as candidates:
-//    by Parseval mean|X|^2 = sum|x_w|^2 =: P, and mean + k*std >=
-//    min(1,k)*sqrt(P), so candidates below that bound are dropped before the
-//    mean and std are known.  Phase 3 filters candidates by the final
-//    threshold and selects the first K in ascending bin order (bucket
-//    histogram + bitonic sort of the boundary bucket).
+//  * large  (N = 256 * N2, N >= 32768), the single-launch form: persistent
+//    grid, one workgroup per CU, channels strided over workgroups.  Phase 1
+//    runs N2-point column FFTs (16384/N2 columns per tile, interleaved LDS
+//    layout) and writes the twiddled intermediate Z[k2][n1] to a
+//    per-workgroup scratch slot; phase 2 runs 256-point row FFTs; local
+//    maxima of interior rows are decided from the LDS magnitude tile,
+//    tile-boundary rows from a small edge buffer.  Only local maxima that can
+//    clear the threshold are kept as candidates: by Parseval mean|X|^2 =
+//    sum|x_w|^2 =: P, and mean + k*std >= min(1,k)*sqrt(P), so candidates
+//    below that bound are dropped before the mean and std are known.  Phase 3
+//    filters candidates by the final threshold and selects the first K in
+//    ascending bin order (select_first_k).  It serves the dense outputs at
+//    these lengths; the peak search runs on the faster three-launch form in
+//    spx_fourstep.hip, which gives the same bins.
 //
 // Peak rules (peak_mode 1, the fused north-star path; no reference kernel):
 //   m[k] = |X[k]| (FP32), range [s0,s1) (default the whole band, circular
