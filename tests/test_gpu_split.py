@@ -107,3 +107,31 @@ def test_split_degenerate(ctx, nfft):
     assert ct[0, 1] == 0 and ct[2, 1] == 0
     assert ct[1, 1] == 1 and pk[1, 0]["index"] == 0 and gm[1]["index"] == 0
     assert_same((pk, ct, gm, st), run(ctx, x, nfft, "fused", K=8, window=0))
+
+
+@pytest.mark.parametrize("nfft,chunk", [(1 << 16, 3), (1 << 15, 5), (1 << 17, 2)])
+def test_split_chunks_equal_one_launch_group(ctx, nfft, chunk):
+    """A batch split into several launch groups (SPX_FS_CHUNK; the workspace
+    bound does this for large batches), including a last group shorter than
+    the column kernel's four channels per workgroup, gives the same bytes as
+    one group."""
+    B = 11
+    print(f"[case] split chunks nfft={nfft} chunk={chunk}", flush=True)
+    x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED0050 + nfft, B, nfft)
+    one = run(ctx, x, nfft, "split", K=32)
+    old = os.environ.get("SPX_FS_CHUNK")
+    os.environ["SPX_FS_CHUNK"] = str(chunk)
+    try:
+        many = run(ctx, x, nfft, "split", K=32)
+    finally:
+        if old is None:
+            del os.environ["SPX_FS_CHUNK"]
+        else:
+            os.environ["SPX_FS_CHUNK"] = old
+    # records beyond counts[c][0] are not written (the buffers are not cleared)
+    (pa, ca, ga, sa), (pb, cb, gb, sb) = one, many
+    assert np.array_equal(ca, cb)
+    for c in range(B):
+        n = ca[c, 0]
+        assert pa[c, :n].tobytes() == pb[c, :n].tobytes(), (c, pa[c, :n], pb[c, :n])
+    assert ga.tobytes() == gb.tobytes() and sa.tobytes() == sb.tobytes()
