@@ -127,34 +127,44 @@ __device__ __forceinline__ void fft256_cols16_halfx(float* lf, float2 (&v)[16], 
   Dft<16>::run(v);
 }
 
-// Column transforms of 65536-point channels (N2 = 256, 16 columns per
-// workgroup) for kFsColsChannels channels of the same column block: the
-// window values and the four-step twiddles W_N^(n1 k2) of the block are read
-// once into registers and serve every channel (they are the same for all
-// channels, and re-reading them per channel costs as much vector-memory
-// bandwidth as the input itself).  Same arithmetic, in the same order, as
-// the one-channel form.
+// Column transforms of kFsColsChannels channels of the same column block
+// (CB columns of N2 rows) per workgroup: the window values and the four-step
+// twiddles W_N^(n1 k2) of the block are read once into registers and serve
+// every channel (they are the same for all channels, and re-reading them per
+// channel put as many bytes through the vector-memory pipeline as the input
+// itself).  The arithmetic of a channel does not depend on its neighbours.
+// N2 = 256 uses the half-width exchange fft256_cols16_halfx with its
+// pass-1 twiddles in LDS (twl); other lengths the generic column tile.
 constexpr int kFsColsChannels = 4;
 
-template <int DT, int TC>
+template <int N2, int DT, int TC>
 __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& w, float2* tile, const float2* twl,
                                               double* red) {
-  using C = FsCfg<256, TC>;
+  using C = FsCfg<N2, TC>;
   constexpr int N1 = C::N1, CB = C::CB, NT1 = C::NT1, PPT = C::PPT, MC = kFsColsChannels;
-  static_assert(CB == 16 && TC == 256, "shape of fft256_cols16_halfx");
+  constexpr int RL1 = Plan<N2, 0>::RL;
+  constexpr bool HALFX = (N2 == 256 && CB == 16);
+  using LAY1 = typename C::LAY1;
   constexpr uint32_t ES = DT == SPX_DTYPE_C64 ? 8 : 4;
   const int tid = threadIdx.x;
   const uint32_t cg = blockIdx.x / NT1, t = blockIdx.x % NT1;
-  const int c = tid % CB, jj = tid / CB;
-  const uint32_t n1 = t * CB + c;
-  const uint32_t n0 = n1 + N1 * jj, stride = N1 * 16;
+  const int c = tid % CB, jj = tid / CB;  // jj in [0, N2/16)
+  const uint32_t n0 = t * CB + c + N1 * jj, stride = N1 * (N2 / 16);
   const uint32_t last = a.n_point - 1;
   float wv[PPT];
 #pragma unroll
   for (int r = 0; r < PPT; ++r) wv[r] = a.window != nullptr ? gp(a.window)[min(n0 + r * stride, last)] : 1.0f;
+  // output positions of v[b*RL1 + r]: Z element (jb + r*N2/RL1)*256 + column
+  uint32_t zoff[PPT / RL1];
   float2 t4[PPT];
 #pragma unroll
-  for (int r = 0; r < PPT; ++r) t4[r] = gp(a.tw_4)[(uint32_t)(jj + 16 * r) * N1 + n1];
+  for (int b = 0; b < PPT / RL1; ++b) {
+    int s, jb;
+    LAY1::template map<RL1>(tid + b * TC, s, jb);
+    zoff[b] = (uint32_t)jb * N1 + t * CB + s;
+#pragma unroll
+    for (int r = 0; r < RL1; ++r) t4[b * RL1 + r] = gp(a.tw_4)[zoff[b] + (uint32_t)(r * (N2 / RL1)) * N1];
+  }
 #pragma unroll 1
   for (int m = 0; m < MC; ++m) {
     const uint32_t ci = cg * MC + m;
@@ -179,12 +189,16 @@ __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& 
 #pragma unroll
     for (int r = 0; r < PPT; ++r) e = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, e));
     if (m > 0) __syncthreads();  // the previous channel's exchange reads are done
-    // laundered thread id and table pointer: the pass-1 twiddle rows are
-    // re-read per channel instead of being kept live across the loop
-    fft256_cols16_halfx<true>(reinterpret_cast<float*>(tile), v, opaque(twl), opaque_v(tid));
+    // laundered thread id and table pointers: the transform's twiddle rows
+    // are re-read per channel instead of being kept live across the loop
+    if constexpr (HALFX) fft256_cols16_halfx<true>(reinterpret_cast<float*>(tile), v, opaque(twl), opaque_v(tid));
+    else fft_tile<LAY1, N2, CB, TC, PPT, 0, false>(tile, v, opaque(a.tw_b), opaque_v(tid));
     float2* Z = w.Z + (uint64_t)ci * C::N;
 #pragma unroll
-    for (int r = 0; r < PPT; ++r) gp(Z)[(uint32_t)(jj + 16 * r) * N1 + n1] = c_mul(v[r], t4[r]);
+    for (int b = 0; b < PPT / RL1; ++b)
+#pragma unroll
+      for (int r = 0; r < RL1; ++r)
+        gp(Z)[zoff[b] + (uint32_t)(r * (N2 / RL1)) * N1] = c_mul(v[b * RL1 + r], t4[b * RL1 + r]);
     const double E = block_sum_d<TC>((double)e, red);
     if (tid == 0) {
       gp(w.pe)[(uint64_t)ci * NT1 + t] = E;
@@ -200,60 +214,15 @@ __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& 
 template <int N2, int TC>
 __global__ void __launch_bounds__(TC) k_fs_cols(FusedArgs a, FsWork w) {
   using C = FsCfg<N2, TC>;
-  constexpr int N1 = C::N1, CB = C::CB, NT1 = C::NT1, PPT = C::PPT;
-  constexpr int RL1 = Plan<N2, 0>::RL;
   using LAY1 = typename C::LAY1;
-  constexpr bool HALFX = (N2 == 256 && CB == 16);
+  constexpr bool HALFX = (N2 == 256 && C::CB == 16);
   __shared__ __align__(16) float2 tile[HALFX ? 5104 / 2 : LAY1::SIZE];
   __shared__ __align__(16) float2 twl[HALFX ? Plan<256, 0>::TWSIZE_LDS : 1];
   __shared__ double red[TC / 64];
-  const int tid = threadIdx.x;
-  if constexpr (HALFX) {
-    copy_tw_to_lds<256, 0, TC>(a.tw_b, twl, tid);  // read after the exchange's barriers
-    if (a.dtype == SPX_DTYPE_C64) fs_cols_multi<SPX_DTYPE_C64, TC>(a, w, tile, twl, red);
-    else if (a.dtype == SPX_DTYPE_C32F16) fs_cols_multi<SPX_DTYPE_C32F16, TC>(a, w, tile, twl, red);
-    else fs_cols_multi<SPX_DTYPE_C32BF16, TC>(a, w, tile, twl, red);
-    return;
-  }
-  const uint32_t ci = blockIdx.x / NT1, t = blockIdx.x % NT1;
-  uint64_t ich = (uint64_t)w.ch0 + ci;
-  FusedArgs ai = a;
-  if (w.chan != nullptr) {
-    const uint64_t es = a.dtype == SPX_DTYPE_C64 ? 8 : 4;
-    ai.in = reinterpret_cast<const char*>(a.in) + gp(w.inoff)[gp(w.chan)[ich]] * es;
-    ich = 0;
-  }
-  float2 v[PPT];
-  {
-    const int c = tid % CB, jj = tid / CB;  // jj in [0, N2/16)
-    const int n1 = t * CB + c;
-    load_run<16>(ai, ich, (uint32_t)(n1 + N1 * jj), (uint32_t)(N1 * (N2 / 16)), v);
-  }
-  float e = 0.f;
-#pragma unroll
-  for (int r = 0; r < PPT; ++r) e = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, e));
-  if constexpr (HALFX) fft256_cols16_halfx<true>(reinterpret_cast<float*>(tile), v, twl, tid);
-  else fft_tile<LAY1, N2, CB, TC, PPT, 0, false>(tile, v, a.tw_b, tid);
-  float2* Z = w.Z + (uint64_t)ci * C::N;
-#pragma unroll
-  for (int b = 0; b < PPT / RL1; ++b) {
-    int s, jb;
-    LAY1::template map<RL1>(tid + b * TC, s, jb);
-    const uint32_t col = t * CB + s;
-#pragma unroll
-    for (int r = 0; r < RL1; ++r) {
-      const uint32_t el = (uint32_t)(jb + r * (N2 / RL1)) * N1 + col;
-      gp(Z)[el] = c_mul(v[b * RL1 + r], gp(a.tw_4)[el]);
-    }
-  }
-  const double E = block_sum_d<TC>((double)e, red);
-  if (tid == 0) {
-    gp(w.pe)[(uint64_t)ci * NT1 + t] = E;
-    if (t == 0) {
-      gp(w.ncand)[2 * ci + 0] = 0u;
-      gp(w.ncand)[2 * ci + 1] = 0u;
-    }
-  }
+  if constexpr (HALFX) copy_tw_to_lds<256, 0, TC>(a.tw_b, twl, threadIdx.x);  // read after the exchange's barriers
+  if (a.dtype == SPX_DTYPE_C64) fs_cols_multi<N2, SPX_DTYPE_C64, TC>(a, w, tile, twl, red);
+  else if (a.dtype == SPX_DTYPE_C32F16) fs_cols_multi<N2, SPX_DTYPE_C32F16, TC>(a, w, tile, twl, red);
+  else fs_cols_multi<N2, SPX_DTYPE_C32BF16, TC>(a, w, tile, twl, red);
 }
 
 // Orders a wave's LDS writes before its later LDS reads (a wave's LDS
@@ -577,7 +546,7 @@ static hipError_t launch_fs(const FusedArgs& a, const FsWork& w0, uint32_t chunk
     FsWork w = w0;
     w.ch0 = c0;
     w.nch = min(chunk, a.batch - c0);
-    const uint32_t groups = (N2 == 256 && C::CB == 16) ? (w.nch + kFsColsChannels - 1) / kFsColsChannels : w.nch;
+    const uint32_t groups = (w.nch + kFsColsChannels - 1) / kFsColsChannels;
     hipLaunchKernelGGL((k_fs_cols<N2, TC>), dim3(groups * C::NT1), dim3(TC), 0, st, a, w);
     hipLaunchKernelGGL((k_fs_rows<N2, TC>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);
     hipLaunchKernelGGL((k_fs_finish<N2, TC>), dim3(w.nch), dim3(C::TF), 0, st, a, w);
