@@ -239,13 +239,13 @@ uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch) {
   return std::min(ch, batch);
 }
 
-// Threshold / local-maximum search of long channels runs on the three-launch
-// split path (spx_fourstep.hip), faster than the single persistent launch
-// (k_fused_large) at every length 2^15 .. 2^18 on MI355X.
-// SPX_LARGE_PATH=fused selects the single launch instead (both give the same
-// bins; the tests cross-check them).
+// Long channels run on the split path (spx_fourstep.hip): three launches for
+// the threshold / local-maximum search, two for the dense outputs, faster
+// than the single persistent launch (k_fused_large) at every length 2^15 ..
+// 2^18 on MI355X.  SPX_LARGE_PATH=fused selects the single launch instead
+// (same bins and spectra to rounding; the tests cross-check them).
 bool use_fourstep(const FusedArgs& a) {
-  if (!spx_fused_is_large(a.nfft) || a.peak_mode == 0) return false;
+  if (!spx_fused_is_large(a.nfft)) return false;
   const char* env = std::getenv("SPX_LARGE_PATH");
   return !(env && std::strcmp(env, "fused") == 0);
 }
@@ -261,9 +261,16 @@ int run_fused(spx_ctx* c, FusedArgs& a) {
     a.tw_b = (const float2*)t;
     if ((rc = get_table(c, TW4, a.nfft, 0, &t))) return rc;
     a.tw_4 = (const float2*)t;
+    SPX_HIP(c, hipSetDevice(c->device));
+    if (a.peak_mode == 0) {
+      const uint64_t per = spx_fs_dense_workspace_bytes(a.nfft, 1);
+      const uint32_t chunk = (uint32_t)std::min<uint64_t>(a.batch, std::max<uint64_t>(64, (2ull << 30) / per));
+      if ((rc = ensure_scratch(c, spx_fs_dense_workspace_bytes(a.nfft, chunk)))) return rc;
+      SPX_HIP(c, spx_launch_fourstep_dense(&a, c->scratch, chunk, c->stream));
+      return SPX_OK;
+    }
     const uint32_t chunk = fourstep_chunk(a.nfft, a.batch);
     if ((rc = ensure_scratch(c, spx_fs_workspace_bytes(a.nfft, chunk)))) return rc;
-    SPX_HIP(c, hipSetDevice(c->device));
     SPX_HIP(c, spx_launch_fourstep(&a, c->scratch, chunk, c->stream));
     return SPX_OK;
   }

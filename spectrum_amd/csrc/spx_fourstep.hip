@@ -15,6 +15,8 @@
 // loads with another's arithmetic.
 #include "spx_fused_common.hpp"
 
+#include <cstring>
+
 namespace spx {
 
 struct FsRowPart {
@@ -539,6 +541,92 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
   }
 }
 
+template <int N2>
+struct FsShape { static constexpr int TC = N2 >= 1024 ? 512 : 256; };
+
+// Dense outputs of long channels (spectrum, |X|, |X|^2 or {|X|, arg X}) after
+// k_fs_cols: the row transforms of k_fs_rows, then a transpose through the
+// float tile (real parts, then imaginary parts) so that 16 lanes write 16
+// consecutive bins k2 .. k2+15 of one k1 (bin k2 + N2 k1).  Element (s, k1)
+// of the block sits at 17 k1 + s.
+template <int N2, int TC>
+__global__ void __launch_bounds__(256) k_fs_rows_dense(FusedArgs a, FsWork w) {
+  using C = FsCfg<N2, TC>;
+  constexpr int N1 = C::N1, RB = C::RB, NT2 = C::NT2, PPT = C::PPT, T = C::TR;
+  using LAY2 = typename C::LAY2;
+  static_assert(RB == 16 && 17 * (N1 - 1) + RB <= LAY2::SIZE, "transpose fits the tile");
+  __shared__ __align__(16) float ldsf[LAY2::SIZE];
+  __shared__ __align__(16) float2 twl[Plan<256, 0>::TWSIZE_LDS];
+  const int tid = threadIdx.x;
+  const uint32_t ci = blockIdx.x / NT2, t = blockIdx.x % NT2;
+  const uint64_t och = (uint64_t)w.ch0 + ci;
+  const int s_row = tid / (N1 / 16), jj_row = tid % (N1 / 16);
+  float2 v[PPT];
+  {
+    const float2* zr = w.Z + (uint64_t)ci * C::N + (uint32_t)((t * RB + s_row) * N1 + jj_row);
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) v[r] = gp(zr)[r * (N1 / 16)];
+  }
+  copy_tw_to_lds<256, 0, T>(a.tw_a, twl, tid);
+  __syncthreads();
+  fft256_rows_halfx<RB, T, true>(ldsf, v, twl, tid);
+  // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]; after the transpose thread
+  // tid holds X[k2 = t*RB + tid%16][k1 = tid/16 + 16 q], q < 16
+  const int s_out = tid % 16, k1_out = tid / 16;
+  float2 x[PPT];
+  __syncthreads();  // every wave is past its transform's exchange
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) ldsf[17 * (jj_row + 16 * r) + s_row] = v[r].x;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) x[q].x = ldsf[17 * (k1_out + 16 * q) + s_out];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) ldsf[17 * (jj_row + 16 * r) + s_row] = v[r].y;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) x[q].y = ldsf[17 * (k1_out + 16 * q) + s_out];
+  const uint64_t cbase = och * (uint64_t)C::N + (uint64_t)(t * RB + s_out);
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) store_dense(a, cbase + (uint64_t)N2 * (uint32_t)(k1_out + 16 * q), x[q]);
+}
+
+template <int N2, int TC>
+static hipError_t launch_fs_dense(const FusedArgs& a, const FsWork& w0, uint32_t chunk, hipStream_t st) {
+  using C = FsCfg<N2, TC>;
+  for (uint32_t c0 = 0; c0 < a.batch; c0 += chunk) {
+    FsWork w = w0;
+    w.ch0 = c0;
+    w.nch = min(chunk, a.batch - c0);
+    const uint32_t groups = (w.nch + kFsColsChannels - 1) / kFsColsChannels;
+    hipLaunchKernelGGL((k_fs_cols<N2, TC>), dim3(groups * C::NT1), dim3(TC), 0, st, a, w);
+    hipLaunchKernelGGL((k_fs_rows_dense<N2, TC>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+template <int N2>
+static uint64_t fs_dense_channel_bytes() {
+  using C = FsCfg<N2, FsShape<N2>::TC>;
+  auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
+  return r((uint64_t)C::N * 8) + 256 + r((uint64_t)C::NT1 * 8);
+}
+
+template <int N2>
+static hipError_t launch_fs_dense_n(const FusedArgs& a, void* ws, uint32_t chunk, hipStream_t st) {
+  using C = FsCfg<N2, FsShape<N2>::TC>;
+  auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
+  unsigned char* p = reinterpret_cast<unsigned char*>(ws);
+  FsWork w;
+  std::memset(&w, 0, sizeof(w));
+  w.Z = reinterpret_cast<float2*>(p);            p += r((uint64_t)chunk * C::N * 8);
+  w.ncand = reinterpret_cast<uint32_t*>(p);      p += r((uint64_t)chunk * 8);
+  w.pe = reinterpret_cast<double*>(p);
+  return launch_fs_dense<N2, FsShape<N2>::TC>(a, w, chunk, st);
+}
+
 template <int N2, int TC>
 static hipError_t launch_fs(const FusedArgs& a, const FsWork& w0, uint32_t chunk, hipStream_t st) {
   using C = FsCfg<N2, TC>;
@@ -556,8 +644,6 @@ static hipError_t launch_fs(const FusedArgs& a, const FsWork& w0, uint32_t chunk
   return hipSuccess;
 }
 
-template <int N2>
-struct FsShape { static constexpr int TC = N2 >= 1024 ? 512 : 256; };
 
 template <int N2>
 static uint64_t fs_channel_bytes() {
@@ -626,6 +712,32 @@ hipError_t spx_launch_fourstep_ex(const FusedArgs* a, void* ws, uint32_t chunk, 
 
 hipError_t spx_launch_fourstep(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st) {
   return spx_launch_fourstep_ex(a, ws, chunk, nullptr, nullptr, st);
+}
+
+// Dense outputs (a.spec_out / a.mag_out) of long channels on the split path.
+uint64_t spx_fs_dense_workspace_bytes(uint32_t nfft, uint32_t chunk) {
+  using namespace spx;
+  uint64_t per = 0;
+  switch (nfft) {
+    case 1u << 15: per = fs_dense_channel_bytes<128>(); break;
+    case 1u << 16: per = fs_dense_channel_bytes<256>(); break;
+    case 1u << 17: per = fs_dense_channel_bytes<512>(); break;
+    case 1u << 18: per = fs_dense_channel_bytes<1024>(); break;
+    default: return 0;
+  }
+  return per * (uint64_t)chunk + 3 * 256;
+}
+
+hipError_t spx_launch_fourstep_dense(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st) {
+  using namespace spx;
+  if (chunk == 0) return hipErrorInvalidValue;
+  switch (a->nfft) {
+    case 1u << 15: return launch_fs_dense_n<128>(*a, ws, chunk, st);
+    case 1u << 16: return launch_fs_dense_n<256>(*a, ws, chunk, st);
+    case 1u << 17: return launch_fs_dense_n<512>(*a, ws, chunk, st);
+    case 1u << 18: return launch_fs_dense_n<1024>(*a, ws, chunk, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // extern "C"

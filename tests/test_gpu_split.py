@@ -135,3 +135,39 @@ def test_split_chunks_equal_one_launch_group(ctx, nfft, chunk):
         n = ca[c, 0]
         assert pa[c, :n].tobytes() == pb[c, :n].tobytes(), (c, pa[c, :n], pb[c, :n])
     assert ga.tobytes() == gb.tobytes() and sa.tobytes() == sb.tobytes()
+
+
+@pytest.mark.parametrize("nfft", LENGTHS)
+def test_split_dense_outputs_equal_single_launch(ctx, nfft):
+    """Dense spectrum, |X| and {|X|, arg X} of the split path against the
+    single persistent launch, with zero padding and a window."""
+    B, n_point = 3, nfft - 123
+    print(f"[case] split dense nfft={nfft}", flush=True)
+    rng = np.random.default_rng(nfft + 7)
+    x = (rng.standard_normal((B, n_point)) + 1j * rng.standard_normal((B, n_point))).astype(np.complex64)
+    d_in = ctx.to_device(x)
+    outs = {}
+    for path in ("split", "fused"):
+        old = os.environ.get("SPX_LARGE_PATH")
+        os.environ["SPX_LARGE_PATH"] = path
+        try:
+            spec, mag, mp = ctx.alloc(B * nfft * 8), ctx.alloc(B * nfft * 4), ctx.alloc(B * nfft * 8)
+            ctx.fft_c2c(d_in, spec, B, n_point, nfft, 1)
+            ctx.fft_magnitude(d_in, mag, B, n_point, nfft, 1, squared=False, scale=1.0)
+            ctx.fft_mag_phase(d_in, mp, B, n_point, nfft, 1)
+            ctx.sync()
+            outs[path] = (spec.download(np.complex64, (B, nfft)), mag.download(np.float32, (B, nfft)),
+                          mp.download(np.float32, (B, nfft, 2)))
+        finally:
+            if old is None:
+                del os.environ["SPX_LARGE_PATH"]
+            else:
+                os.environ["SPX_LARGE_PATH"] = old
+    (sa, ma, pa), (sb, mb, pb) = outs["split"], outs["fused"]
+    scale = float(np.abs(sb).max())
+    assert np.max(np.abs(sa - sb)) <= 2e-6 * scale
+    assert np.max(np.abs(ma - mb)) <= 2e-6 * scale
+    assert np.max(np.abs(pa[..., 0] - pb[..., 0])) <= 2e-6 * scale
+    big = pb[..., 0] > 1e-3 * scale
+    dphi = np.angle(np.exp(1j * (pa[..., 1] - pb[..., 1])))
+    assert np.max(np.abs(dphi[big])) < 1e-4
