@@ -132,6 +132,62 @@ def _timer_ms(ctx, fn, reps=10, warm=3):
     return float(np.median(ts))
 
 
+def extra_host(ctx, x, K=64, reps=5):
+    """PCIe-inclusive wall time of the headline call with the input in host
+    memory and the results returned to host memory (reported beside the
+    device-resident value, never as it): the streamed host entry point
+    (spx_fused_peaks_host) against the plain sequence upload -> search ->
+    download; the same for the complex spectrum (spx_fft_c2c_host); the pinned
+    DMA rate bounds both."""
+    B, N = x.shape
+
+    def wall(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3
+
+    d = ctx.alloc(x.nbytes)
+    pk, ct, g = ctx.alloc(B * K * 32), ctx.alloc(B * 8), ctx.alloc(B * 32)
+
+    def plain_peaks():
+        d.upload(x)
+        ctx.fused_peaks(d, B, N, N, pk, ct, window=1, fs=1.0, nsigma=6.0, max_peaks=K, gmax=g)
+        ctx.sync()
+        pk.download(np.uint8, (B * K * 32,))
+        ct.download(np.uint32, (B, 2))
+        g.download(np.uint8, (B * 32,))
+
+    def streamed_peaks(src):
+        ctx.fused_peaks_host(src, N, window=1, fs=1.0, nsigma=6.0, max_peaks=K, with_stats=False)
+
+    out = {"plain_ms": wall(plain_peaks), "streamed_ms": wall(lambda: streamed_peaks(x))}
+    xp = ctx.pinned(x.shape, np.complex64)
+    xp.array[...] = x
+    out["pinned_input_ms"] = wall(lambda: streamed_peaks(xp.array))
+    out["pinned_h2d_GBps"] = x.nbytes / (wall(lambda: ctx.check(
+        ctx.lib.spx_memcpy_h2d(ctx.h, d.ptr, xp.ptr, x.nbytes), "spx_memcpy_h2d")) * 1e-3) / 1e9
+    xp.free()
+    out["streamed_channels_per_s"] = B / (out["streamed_ms"] * 1e-3)
+    spec = ctx.alloc(x.nbytes)
+    host_spec = np.empty_like(x)
+
+    def plain_c2c():
+        d.upload(x)
+        ctx.fft_c2c(d, spec, B, N, N, 0)
+        ctx.sync()
+        spec.download(np.complex64, x.shape)
+
+    out["c2c_plain_ms"] = wall(plain_c2c)
+    out["c2c_streamed_ms"] = wall(lambda: ctx.fft_c2c_host(x, N, out=host_spec))
+    for b in (d, pk, ct, g, spec):
+        b.free()
+    return out
+
+
 def extra_uneven(ctx, K=64):
     from spectrum_amd import capi
 
@@ -316,6 +372,11 @@ def main():
         except Exception as e:  # the single-thread baseline stands on its own
             out["cpu_baseline"]["multicore"] = {"error": repr(e)}
     if not args.no_extras:
+        print("[bench] host_stream ...", file=sys.stderr, flush=True)
+        try:
+            out["host_stream"] = extra_host(ctx, x, K)
+        except Exception as e:
+            out["host_stream"] = {"error": repr(e)}
         del d_in, x
         for name, fn in (("uneven", extra_uneven), ("mixed_mlp", extra_mixed_mlp), ("refiner", extra_refiner)):
             print(f"[bench] {name} ...", file=sys.stderr, flush=True)

@@ -150,6 +150,34 @@ typedef struct spx_fused_desc {
 
 int spx_fused_peaks(spx_ctx* ctx, const spx_fused_desc* desc);
 
+/* ---- host-resident data ------------------------------------------------------
+ * The same operations on HOST buffers (every pointer, including all of the
+ * descriptor's, is a host address; pageable or pinned).  The batch streams
+ * through the GPU in chunks of channels: the upload of one chunk, the kernels
+ * of the previous one and the download of the one before overlap on three
+ * streams, with pinned staging (skipped for buffers that are already pinned).
+ * Results equal the device entry points' bit for bit; the call returns once
+ * they are in host memory.  Replace the host-vector paths of
+ * FFTProcessorROCm::ProcessComplex / ProcessMagPhase
+ * (fft_processor_rocm.cpp:306-570) and SpectrumProcessorROCm::ProcessFromCPU /
+ * FindAllMaximaFromCPU (spectrum_processor_rocm.cpp:263-307, 444-543), which
+ * upload, compute and download one after the other. */
+int spx_fft_c2c_host(spx_ctx* ctx, const void* in, void* out, uint32_t batch, uint32_t n_point, uint32_t nfft,
+                     uint32_t window);
+int spx_fft_mag_phase_host(spx_ctx* ctx, const void* in, void* out_float2, uint32_t batch, uint32_t n_point,
+                           uint32_t nfft, uint32_t window);
+int spx_fft_magnitude_host(spx_ctx* ctx, const void* in, float* out, uint32_t batch, uint32_t n_point,
+                           uint32_t nfft, uint32_t window, int squared, float scale);
+int spx_fused_peaks_host(spx_ctx* ctx, const spx_fused_desc* desc);
+/* Bytes per transfer chunk of the *_host calls (default 32 MiB; the larger of
+ * a chunk's input and output).  The context keeps two chunks of pinned and
+ * two of device memory per direction. */
+int spx_set_host_chunk(spx_ctx* ctx, uint64_t bytes);
+/* Pinned (page-locked) host memory for the *_host calls' buffers: transfers
+ * from and to it need no staging copy. */
+int spx_host_alloc(spx_ctx* ctx, uint64_t bytes, void** out);
+int spx_host_free(spx_ctx* ctx, void* ptr);
+
 /* ---- learned sub-bin refiner ---------------------------------------------
  * No counterpart in the reference (its refinement is parabolic only,
  * include/spectrum/kernels/fft_kernel_sources_rocm.hpp:228-239).  A

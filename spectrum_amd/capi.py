@@ -81,6 +81,13 @@ SIGNATURES = [
     ("spx_peaks_top", _I, [_P, _P, _P, _U32, _U32, _U32, _F, _I]),
     ("spx_all_maxima", _I, [_P, _P, _P, _P, _U32, _U32, _F, _U32, _U32, _U32]),
     ("spx_fused_peaks", _I, [_P, C.POINTER(SpxFusedDesc)]),
+    ("spx_fft_c2c_host", _I, [_P, _P, _P, _U32, _U32, _U32, _U32]),
+    ("spx_fft_mag_phase_host", _I, [_P, _P, _P, _U32, _U32, _U32, _U32]),
+    ("spx_fft_magnitude_host", _I, [_P, _P, _P, _U32, _U32, _U32, _U32, _I, _F]),
+    ("spx_fused_peaks_host", _I, [_P, C.POINTER(SpxFusedDesc)]),
+    ("spx_set_host_chunk", _I, [_P, _U64]),
+    ("spx_host_alloc", _I, [_P, _U64, C.POINTER(C.c_void_p)]),
+    ("spx_host_free", _I, [_P, _P]),
     ("spx_mlp_param_count", _I, []),
     ("spx_mlp_deployed_size", _I, []),
     ("spx_mlp_init", _I, [_P, _P, _U64]),
@@ -245,6 +252,72 @@ class Context:
         d.gmax_out, d.stats_out = _ptr(gmax), _ptr(stats)
         self.check(self.lib.spx_fused_peaks(self.h, C.byref(d)), "spx_fused_peaks")
 
+    # -- host-resident data (numpy in, numpy out; synchronous, streamed in chunks)
+    def set_host_chunk(self, nbytes: int):
+        self.check(self.lib.spx_set_host_chunk(self.h, int(nbytes)), "spx_set_host_chunk")
+
+    def pinned(self, shape, dtype) -> "PinnedArray":
+        """A numpy array in pinned host memory (no staging copy in the *_host calls)."""
+        return PinnedArray(self, shape, dtype)
+
+    @staticmethod
+    def _host_in(x: np.ndarray, dtype) -> np.ndarray:
+        x = np.ascontiguousarray(x)
+        if x.dtype != dtype:
+            raise TypeError(f"expected {np.dtype(dtype)} input, got {x.dtype}")
+        return x
+
+    def fft_c2c_host(self, x: np.ndarray, nfft: int, window: int = 0, out: Optional[np.ndarray] = None):
+        """[batch][n_point] complex64 host array -> [batch][nfft] complex64 spectra."""
+        x = self._host_in(x, np.complex64)
+        B, n = x.shape
+        out = np.empty((B, nfft), np.complex64) if out is None else out
+        self.check(self.lib.spx_fft_c2c_host(self.h, x.ctypes.data, out.ctypes.data, B, n, nfft, window),
+                   "spx_fft_c2c_host")
+        return out
+
+    def fft_mag_phase_host(self, x: np.ndarray, nfft: int, window: int = 0):
+        """-> [batch][nfft][2] float32 {|X|, arg X}."""
+        x = self._host_in(x, np.complex64)
+        B, n = x.shape
+        out = np.empty((B, nfft, 2), np.float32)
+        self.check(self.lib.spx_fft_mag_phase_host(self.h, x.ctypes.data, out.ctypes.data, B, n, nfft, window),
+                   "spx_fft_mag_phase_host")
+        return out
+
+    def fft_magnitude_host(self, x: np.ndarray, nfft: int, window: int = 0, squared=False, scale=1.0):
+        x = self._host_in(x, np.complex64)
+        B, n = x.shape
+        out = np.empty((B, nfft), np.float32)
+        self.check(self.lib.spx_fft_magnitude_host(self.h, x.ctypes.data, out.ctypes.data, B, n, nfft, window,
+                                                   int(bool(squared)), float(scale)), "spx_fft_magnitude_host")
+        return out
+
+    def fused_peaks_host(self, x: np.ndarray, nfft: int, *, window=1, fs=1.0, peak_mode=PEAKS_THRESHOLD,
+                         nsigma=6.0, max_peaks=32, s0=0, s1=0, refine=REFINE_PARABOLIC, with_gmax=True,
+                         with_stats=True, dtype=DTYPE_C64, in_scale: Optional[np.ndarray] = None):
+        """Host signal [batch][n_point] (complex64, or [batch][n_point][2] float16 / uint16-bf16 for the
+        fp16 / bf16 dtypes) -> (records [batch][max_peaks], counts [batch][2], gmax [batch] or None,
+        stats [batch][4] or None), all numpy."""
+        x = np.ascontiguousarray(x)
+        B, n = x.shape[0], x.shape[1]
+        pk = np.zeros((B, max_peaks), MAXVALUE_DTYPE)
+        ct = np.zeros((B, 2), np.uint32)
+        gm = np.zeros(B, MAXVALUE_DTYPE) if with_gmax else None
+        st = np.zeros((B, 4), np.float32) if with_stats else None
+        sc = None if in_scale is None else np.ascontiguousarray(in_scale, dtype=np.float32)
+        d = SpxFusedDesc()
+        d.input = x.ctypes.data
+        d.in_scale = None if sc is None else sc.ctypes.data
+        d.batch, d.n_point, d.nfft = B, n, nfft
+        d.window, d.dtype, d.sample_rate = window, dtype, float(fs)
+        d.peak_mode, d.nsigma, d.max_peaks = peak_mode, float(nsigma), max_peaks
+        d.search_start, d.search_end, d.refine = s0, s1, refine
+        d.peaks_out, d.counts_out = pk.ctypes.data, ct.ctypes.data
+        d.gmax_out = None if gm is None else gm.ctypes.data
+        d.stats_out = None if st is None else st.ctypes.data
+        self.check(self.lib.spx_fused_peaks_host(self.h, C.byref(d)), "spx_fused_peaks_host")
+        return pk, ct, gm, st
 
     # -- learned sub-bin refiner
     def set_refiner(self, deployed):
@@ -349,6 +422,32 @@ class BatchPlan:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """Pinned host memory from spx_host_alloc, viewed as a numpy array (.array);
+    freed with the object (keep it alive while .array is in use)."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        p = C.c_void_p()
+        ctx.check(ctx.lib.spx_host_alloc(ctx.h, nbytes, C.byref(p)), "spx_host_alloc")
+        self.ptr = p.value
+        buf = (C.c_char * max(nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.ctx.lib.spx_host_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

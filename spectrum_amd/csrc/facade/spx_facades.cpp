@@ -152,6 +152,50 @@ void FFTProcessorROCm::CalculateNFFT(const FFTProcessorParams& p) {
 
 void FFTProcessorROCm::Reserve(void** buf, size_t* cap, size_t bytes) { reserve(buf, cap, bytes); }
 
+namespace {
+
+std::vector<FFTComplexResult> pack_complex(const std::vector<std::complex<float>>& all, size_t B, uint32_t nfft,
+                                           float fs) {
+  std::vector<FFTComplexResult> res(B);
+  for (size_t b = 0; b < B; ++b) {
+    res[b].beam_id = (uint32_t)b;
+    res[b].nFFT = nfft;
+    res[b].sample_rate = fs;
+    res[b].spectrum.assign(all.begin() + b * nfft, all.begin() + (b + 1) * nfft);
+  }
+  return res;
+}
+
+// {|X|, arg X} interleaved -> per-beam magnitude / phase (/ frequency axis)
+std::vector<FFTMagPhaseResult> pack_magphase(const std::vector<float>& inter, size_t B, uint32_t nfft,
+                                             const FFTProcessorParams& params) {
+  const bool freq = params.output_mode == FFTOutputMode::MAGNITUDE_PHASE_FREQ;
+  std::vector<FFTMagPhaseResult> res(B);
+  for (size_t b = 0; b < B; ++b) {
+    auto& r = res[b];
+    r.beam_id = (uint32_t)b;
+    r.nFFT = nfft;
+    r.sample_rate = params.sample_rate;
+    r.magnitude.resize(nfft);
+    r.phase.resize(nfft);
+    const float* p = inter.data() + 2 * b * nfft;
+    for (uint32_t k = 0; k < nfft; ++k) {
+      r.magnitude[k] = p[2 * k];
+      r.phase[k] = p[2 * k + 1];
+    }
+    if (freq) {
+      r.frequency.resize(nfft);
+      for (uint32_t k = 0; k < nfft; ++k) r.frequency[k] = (float)k * params.sample_rate / (float)nfft;
+    }
+  }
+  return res;
+}
+
+}  // namespace
+
+// Host vector in, host results out: streamed in chunks (spx_fft_c2c_host), so
+// the upload, the transforms and the download overlap; their total is
+// reported as the FFT time and the upload / download times are 0.
 std::vector<FFTComplexResult> FFTProcessorROCm::ProcessComplex(const std::vector<std::complex<float>>& data,
                                                                const FFTProcessorParams& params,
                                                                ROCmProfEvents* prof_events) {
@@ -161,12 +205,16 @@ std::vector<FFTComplexResult> FFTProcessorROCm::ProcessComplex(const std::vector
     throw std::invalid_argument("ProcessComplex: data size (" + std::to_string(data.size()) +
                                 ") != beam_count * n_point (" + std::to_string(total) + ")");
   spx_ctx* c = (spx_ctx*)ctx_;
+  const size_t B = params.beam_count;
   auto t0 = Clock::now();
-  Reserve(&in_, &in_cap_, total * 8);
-  hip_check(hipMemcpyAsync(in_, data.data(), total * 8, hipMemcpyHostToDevice, stream_of(c)), "upload");
-  spx_check(c, spx_synchronize(c), "upload");
-  prof_.upload_time_ms = ms_since(t0);
-  auto r = ProcessComplex(in_, params, total * 8);
+  std::vector<std::complex<float>> all(B * nFFT_);
+  spx_check(c, spx_fft_c2c_host(c, data.data(), all.data(), (uint32_t)B, params.n_point, nFFT_, SPX_WINDOW_NONE),
+            "spx_fft_c2c_host");
+  prof_.upload_time_ms = 0.0;
+  prof_.fft_time_ms = ms_since(t0);
+  prof_.download_time_ms = 0.0;
+  auto r = pack_complex(all, B, nFFT_, params.sample_rate);
+  prof_.post_processing_time_ms = 0.0;
   prof_.total_time_ms = ms_since(t0);
   if (prof_events) {
     prof_events->push_back({"Upload", {prof_.upload_time_ms}});
@@ -191,16 +239,10 @@ std::vector<FFTComplexResult> FFTProcessorROCm::ProcessComplex(void* gpu_data, c
   spx_check(c, spx_synchronize(c), "spx_fft_c2c");
   prof_.fft_time_ms = ms_since(t0);
   t0 = Clock::now();
-  std::vector<FFTComplexResult> res(B);
   std::vector<std::complex<float>> all(B * nFFT_);
   hip_check(hipMemcpyAsync(all.data(), out_, B * nFFT_ * 8, hipMemcpyDeviceToHost, stream_of(c)), "download");
   spx_check(c, spx_synchronize(c), "download");
-  for (size_t b = 0; b < B; ++b) {
-    res[b].beam_id = (uint32_t)b;
-    res[b].nFFT = nFFT_;
-    res[b].sample_rate = params.sample_rate;
-    res[b].spectrum.assign(all.begin() + b * nFFT_, all.begin() + (b + 1) * nFFT_);
-  }
+  auto res = pack_complex(all, B, nFFT_, params.sample_rate);
   prof_.download_time_ms = ms_since(t0);
   prof_.post_processing_time_ms = 0.0;
   return res;
@@ -214,12 +256,15 @@ std::vector<FFTMagPhaseResult> FFTProcessorROCm::ProcessMagPhase(const std::vect
   if (params.beam_count == 0 || data.size() != total)
     throw std::invalid_argument("ProcessMagPhase: data size != beam_count * n_point");
   spx_ctx* c = (spx_ctx*)ctx_;
+  const size_t B = params.beam_count;
   auto t0 = Clock::now();
-  Reserve(&in_, &in_cap_, total * 8);
-  hip_check(hipMemcpyAsync(in_, data.data(), total * 8, hipMemcpyHostToDevice, stream_of(c)), "upload");
-  spx_check(c, spx_synchronize(c), "upload");
-  prof_.upload_time_ms = ms_since(t0);
-  auto r = ProcessMagPhase(in_, params, total * 8);
+  std::vector<float> inter(B * nFFT_ * 2);
+  spx_check(c, spx_fft_mag_phase_host(c, data.data(), inter.data(), (uint32_t)B, params.n_point, nFFT_,
+                                      SPX_WINDOW_NONE), "spx_fft_mag_phase_host");
+  prof_.upload_time_ms = 0.0;
+  prof_.fft_time_ms = ms_since(t0);
+  prof_.download_time_ms = 0.0;
+  auto r = pack_magphase(inter, B, nFFT_, params);
   prof_.total_time_ms = ms_since(t0);
   if (prof_events) {
     prof_events->push_back({"Upload", {prof_.upload_time_ms}});
@@ -248,25 +293,7 @@ std::vector<FFTMagPhaseResult> FFTProcessorROCm::ProcessMagPhase(void* gpu_data,
   std::vector<float> inter(B * nFFT_ * 2);
   hip_check(hipMemcpyAsync(inter.data(), out_, inter.size() * 4, hipMemcpyDeviceToHost, stream_of(c)), "download");
   spx_check(c, spx_synchronize(c), "download");
-  const bool freq = params.output_mode == FFTOutputMode::MAGNITUDE_PHASE_FREQ;
-  std::vector<FFTMagPhaseResult> res(B);
-  for (size_t b = 0; b < B; ++b) {
-    auto& r = res[b];
-    r.beam_id = (uint32_t)b;
-    r.nFFT = nFFT_;
-    r.sample_rate = params.sample_rate;
-    r.magnitude.resize(nFFT_);
-    r.phase.resize(nFFT_);
-    const float* p = inter.data() + 2 * b * nFFT_;
-    for (uint32_t k = 0; k < nFFT_; ++k) {
-      r.magnitude[k] = p[2 * k];
-      r.phase[k] = p[2 * k + 1];
-    }
-    if (freq) {
-      r.frequency.resize(nFFT_);
-      for (uint32_t k = 0; k < nFFT_; ++k) r.frequency[k] = (float)k * params.sample_rate / (float)nFFT_;
-    }
-  }
+  auto res = pack_magphase(inter, B, nFFT_, params);
   prof_.download_time_ms = ms_since(t0);
   return res;
 }
@@ -458,6 +485,20 @@ uint32_t capacity(uint32_t s0, uint32_t s1, size_t max_per_beam) {
 }
 
 // Build the result from device records [B][cap] and per-beam found counts.
+// Per-beam host results from the records [B][cap] and the uncapped counts.
+void fill_cpu_beams(AllMaximaResult& res, const std::vector<MaxValue>& all, const std::vector<uint32_t>& found,
+                    uint32_t cap) {
+  const uint32_t B = (uint32_t)found.size();
+  res.beams.resize(B);
+  for (uint32_t b = 0; b < B; ++b) {
+    const uint32_t n = std::min(found[b], cap);  // truncation keeps the lowest bins
+    res.beams[b].antenna_id = b;
+    res.beams[b].num_maxima = n;
+    res.beams[b].maxima.assign(all.begin() + (size_t)b * cap, all.begin() + (size_t)b * cap + n);
+    res.total_maxima += n;
+  }
+}
+
 AllMaximaResult finish_all_maxima(spx_ctx* c, const void* rec_dev, const std::vector<uint32_t>& found, uint32_t cap,
                                   OutputDestination dest) {
   const uint32_t B = (uint32_t)found.size();
@@ -470,14 +511,7 @@ AllMaximaResult finish_all_maxima(spx_ctx* c, const void* rec_dev, const std::ve
                                stream_of(c)), "download");
       spx_check(c, spx_synchronize(c), "download");
     }
-    res.beams.resize(B);
-    for (uint32_t b = 0; b < B; ++b) {
-      const uint32_t n = std::min(found[b], cap);  // truncation keeps the lowest bins
-      res.beams[b].antenna_id = b;
-      res.beams[b].num_maxima = n;
-      res.beams[b].maxima.assign(all.begin() + (size_t)b * cap, all.begin() + (size_t)b * cap + n);
-      res.total_maxima += n;
-    }
+    fill_cpu_beams(res, all, found, cap);
   } else {
     for (uint32_t f : found) res.total_maxima += f;
   }
@@ -692,6 +726,45 @@ AllMaximaResult SpectrumProcessorROCm::FindAllMaximaFromCPU(const std::vector<st
   if (data.size() != total) throw std::invalid_argument("FindAllMaximaFromCPU: data size != antenna_count * n_point");
   spx_ctx* c = (spx_ctx*)ctx_;
   auto t0 = Clock::now();
+  if (dest == OutputDestination::CPU) {
+    // host results only: streamed in chunks (spx_fused_peaks_host), upload and
+    // search overlapping
+    const uint32_t N = params_.nFFT;
+    normalise_range(N, s0, s1);
+    const uint32_t cap = capacity(s0, s1, params_.max_maxima_per_beam);
+    const size_t beams = params_.antenna_count;
+    std::vector<MaxValue> all(beams * cap);
+    std::vector<uint32_t> ct(beams * 2);
+    spx_fused_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.input = data.data();
+    d.batch = (uint32_t)beams;
+    d.n_point = params_.n_point;
+    d.nfft = N;
+    d.window = SPX_WINDOW_NONE;
+    d.dtype = SPX_DTYPE_C64;
+    d.sample_rate = params_.sample_rate;
+    d.peak_mode = SPX_PEAKS_ALL_MAXIMA;
+    d.max_peaks = cap;
+    d.search_start = s0;
+    d.search_end = s1;
+    d.refine = SPX_REFINE_NONE;
+    d.peaks_out = cap ? (spx_max_value*)all.data() : nullptr;
+    d.counts_out = ct.data();
+    spx_check(c, spx_fused_peaks_host(c, &d), "spx_fused_peaks_host");
+    prof_.upload_time_ms = 0.0;
+    prof_.fft_time_ms = ms_since(t0);
+    std::vector<uint32_t> found(beams);
+    for (size_t b = 0; b < beams; ++b) found[b] = ct[2 * b + 1];
+    AllMaximaResult r;
+    r.destination = dest;
+    fill_cpu_beams(r, all, found, cap);
+    if (prof_events) {
+      prof_events->push_back({"Upload", {prof_.upload_time_ms}});
+      prof_events->push_back({"FusedAllMaxima", {prof_.fft_time_ms}});
+    }
+    return r;
+  }
   Reserve(&in_, &in_cap_, total * 8);
   hip_check(hipMemcpyAsync(in_, data.data(), total * 8, hipMemcpyHostToDevice, stream_of(c)), "upload");
   spx_check(c, spx_synchronize(c), "upload");

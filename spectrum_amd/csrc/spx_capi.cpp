@@ -350,6 +350,7 @@ int spx_destroy(spx_ctx* c) {
   for (auto& kv : c->tables) (void)hipFree(kv.second);
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->mlp_partial) (void)hipFree(c->mlp_partial);
+  host_stage_free(c);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return SPX_OK;

@@ -10,6 +10,8 @@
 
 #include "spx_kernels.h"
 
+struct HostStage;  // pinned and device double buffers of the host-streaming entry points
+
 struct spx_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
@@ -24,6 +26,8 @@ struct spx_ctx {
   const void* refiner = nullptr;  // deployed learned-refiner block (caller-owned device memory)
   void* mlp_partial = nullptr;    // per-workgroup gradient partials
   uint64_t mlp_partial_bytes = 0;
+  HostStage* hstage = nullptr;             // created by the first *_host call
+  uint64_t host_chunk_bytes = 32ull << 20;  // per-chunk transfer size of the *_host calls
 };
 
 namespace spx_host {
@@ -37,6 +41,8 @@ int hip_fail(spx_ctx* c, hipError_t e, const char* what);
 // once per context.
 int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out);
 int window_ptr(spx_ctx* c, uint32_t window, uint32_t n_point, const float** w);
+// Releases the host-streaming buffers, streams and events.
+void host_stage_free(spx_ctx* c);
 // Channels per launch group of the split long-channel path (bounds its workspace).
 uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch);
 
