@@ -186,8 +186,10 @@ int spx_host_free(spx_ctx* ctx, void* ptr);
  * GPU (own forward/backward and Adam kernels, FP32 master weights); the
  * deployed block holds FP16 weights for MFMA inference with FP32
  * accumulation.  With spx_set_refiner, spx_fused_peaks accepts
- * refine = SPX_REFINE_LEARNED (threshold search, nfft 1024..16384,
- * max_peaks <= 64); predictions outside [-0.5, 0.5] fall back to parabolic. */
+ * refine = SPX_REFINE_LEARNED (threshold search, nfft 1024..16384): the
+ * fused kernel stores the parabolic records and the features of each stored
+ * peak, and a second MFMA kernel replaces the offset wherever the network's
+ * prediction lies in [-0.5, 0.5]. */
 int spx_mlp_param_count(void);    /* 4929 FP32 master parameters */
 int spx_mlp_deployed_size(void);  /* bytes of the deployed block */
 int spx_mlp_init(spx_ctx* ctx, float* params, uint64_t seed);

@@ -227,6 +227,25 @@ int prepare(spx_ctx* c, FusedArgs& a, int* grid) {
   return SPX_OK;
 }
 
+// Learned refinement on the short paths: the fused kernel writes the
+// features of every stored peak (32 bytes each) into the context's scratch,
+// then k_mlp_refine updates those records.
+int learned_features(spx_ctx* c, FusedArgs& a) {
+  a.mlp_feat = nullptr;
+  if (a.refine != 2 || a.peak_mode != 1 || a.kcap == 0 || !a.peaks) return SPX_OK;
+  int rc = ensure_scratch(c, (uint64_t)a.batch * a.kcap * 32);
+  if (rc) return rc;
+  a.mlp_feat = c->scratch;
+  return SPX_OK;
+}
+
+int learned_refine(spx_ctx* c, const FusedArgs& a) {
+  if (!a.mlp_feat) return SPX_OK;
+  SPX_HIP(c, spx_launch_mlp_refine(a.mlp, a.mlp_feat, a.peaks, a.counts, a.batch, a.kcap, a.fs / (float)a.nfft,
+                                   c->stream));
+  return SPX_OK;
+}
+
 // Channels per launch group of the split long-channel peak search: bounds the
 // workspace (Z, candidate lists, edge rows) to about 2 GiB.
 uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch) {
@@ -275,9 +294,10 @@ int run_fused(spx_ctx* c, FusedArgs& a) {
     return SPX_OK;
   }
   if ((rc = prepare(c, a, &grid))) return rc;
+  if ((rc = learned_features(c, a))) return rc;
   SPX_HIP(c, hipSetDevice(c->device));
   SPX_HIP(c, spx_launch_fused(&a, grid, c->stream));
-  return SPX_OK;
+  return learned_refine(c, a);
 }
 
 int window_ptr(spx_ctx* c, uint32_t window, uint32_t n_point, const float** w) {
@@ -492,8 +512,8 @@ int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a) {
   if (d->refine > SPX_REFINE_LEARNED) return fail(c, SPX_EINVAL, "unknown refine mode");
   if (d->refine == SPX_REFINE_LEARNED) {
     if (!c->refiner) return fail(c, SPX_EINVAL, "learned refinement requested but no refiner is set (spx_set_refiner)");
-    if (d->peak_mode != SPX_PEAKS_THRESHOLD || d->nfft < 1024 || d->nfft > 16384 || d->max_peaks > 64)
-      return fail(c, SPX_ENOTSUP, "learned refinement: threshold search, nfft 1024..16384 and max_peaks <= 64 only");
+    if (d->peak_mode != SPX_PEAKS_THRESHOLD || d->nfft < 1024 || d->nfft > 16384)
+      return fail(c, SPX_ENOTSUP, "learned refinement: threshold search and nfft 1024..16384 only");
   }
   uint32_t s0 = d->search_start, s1 = d->search_end;
   if (d->peak_mode == SPX_PEAKS_ALL_MAXIMA) {
@@ -618,9 +638,10 @@ static int run_mixed(spx_ctx* c, FusedArgs& a, uint32_t window, uint32_t compute
   a.tw_a = (const float2*)t;
   int grid = c->cu_count > 0 ? c->cu_count : 256;
   if ((uint32_t)grid > a.batch) grid = (int)a.batch;
+  if ((rc = learned_features(c, a))) return rc;
   SPX_HIP(c, hipSetDevice(c->device));
   SPX_HIP(c, spx_launch_mixed(&a, (int)compute, grid, c->stream));
-  return SPX_OK;
+  return learned_refine(c, a);
 }
 
 static int mixed_check(spx_ctx* c, uint32_t nfft, uint32_t n_point, uint32_t dtype, uint32_t compute) {

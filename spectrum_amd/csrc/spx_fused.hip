@@ -51,10 +51,9 @@ struct SmallCfg {
   using LAY = LayContig<L, 1>;
 };
 
-// LEARN: refine the stored peaks with the learned refiner (a.mlp, deployed
-// block): features of every stored peak are staged in LDS, waves evaluate the
-// network 16 peaks at a time on MFMA, and a prediction outside [-0.5, 0.5]
-// (or not finite) falls back to the parabolic offset.  kcap <= 64.
+// LEARN: the records get the parabolic offset and the features of every
+// stored peak go to a.mlp_feat; the learned refinement (k_mlp_refine) runs
+// after the kernel and replaces the offset where its prediction is valid.
 template <int L, int MODE, bool LEARN = false>
 __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_per_eu(SmallCfg<L>::WPE))) k_fused_small(FusedArgs a) {
   using C = SmallCfg<L>;
@@ -154,33 +153,6 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
       }
       int total;
       int off = block_excl_scan<T>(c, total, red);
-      [[maybe_unused]] float* lres = nullptr;
-      if constexpr (LEARN) {
-        __shared__ __align__(16) _Float16 lfeat[64 * 16];
-        __shared__ __align__(16) _Float16 lh[16 * 64];
-        __shared__ float lout[64];
-        lres = lout;
-        if (owner && flags && off < (int)a.kcap) {
-          uint32_t fl = flags;
-          int pos = off;
-          while (fl && pos < (int)a.kcap) {
-            const int e = __builtin_ctz(fl);
-            fl &= fl - 1;
-            float f[mlp::IN];
-            mlp_features([&](uint32_t q) { return lds[LAY::idx(0, q)]; }, tid * E + e, L, f);
-#pragma unroll
-            for (int j = 0; j < 16; ++j) lfeat[pos * 16 + j] = (_Float16)(j < mlp::IN ? f[j] : 0.f);
-            ++pos;
-          }
-        }
-        __syncthreads();
-        const int nrb = (min(total, (int)a.kcap) + 15) / 16;
-        if (tid < 64) {  // wave 0, 16 peaks per pass
-          for (int rb = 0; rb < nrb; ++rb)
-            mlp_rows16(lfeat + rb * 16 * 16, lh, lout + rb * 16, reinterpret_cast<const unsigned char*>(a.mlp));
-        }
-        __syncthreads();
-      }
       if (owner && flags && off < (int)a.kcap) {
         MaxValue* out = a.peaks + (uint64_t)ch * a.kcap;
         int pos = off;
@@ -191,13 +163,9 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
           const float yc = mag_of(lds[LAY::idx(0, k)]);
           const float yl = mag_of(lds[LAY::idx(0, (k + L - 1) % L)]);
           const float yr = mag_of(lds[LAY::idx(0, (k + 1) % L)]);
-          if constexpr (LEARN) {
-            const float dl = lres[pos];
-            const float d = (fabsf(dl) <= 0.5f) ? dl : parabolic(yl, yc, yr);
-            write_record_d(out + pos, k, lds[LAY::idx(0, k)], yc, d, a.fs / (float)a.nfft);
-          } else {
-            write_record(out + pos, k, lds[LAY::idx(0, k)], yl, yc, yr, a);
-          }
+          write_record(out + pos, k, lds[LAY::idx(0, k)], yl, yc, yr, a);
+          if constexpr (LEARN)
+            mlp_store_features([&](uint32_t q) { return lds[LAY::idx(0, q)]; }, k, L, mlp_feat_row(a, ch, pos));
           ++pos;
         }
       }

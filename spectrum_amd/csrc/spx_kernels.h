@@ -38,6 +38,7 @@ struct FusedArgs {
   void* scratch;           // large: per-workgroup slots
   uint64_t slot_bytes;
   const void* mlp;         // learned refiner weights (refine == 2)
+  void* mlp_feat;          // refine == 2 on the short paths: [batch][kcap][16] fp16 features of the stored peaks
   // window as w[n] = c0 - c1 cos(2 pi n/(n_point-1)) + c2 cos(4 pi n/(n_point-1))
   // (kernels that evaluate it instead of reading `window`; wkind 0 = none)
   uint32_t wkind;
@@ -86,6 +87,8 @@ hipError_t spx_launch_mlp_adam(float* params, const float* grad, float* m, float
                                float eps, int step, hipStream_t st);
 hipError_t spx_launch_mlp_export(const float* params, void* dep, hipStream_t st);
 hipError_t spx_launch_mlp_predict(const void* dep, const float* X, uint32_t n, float* out, hipStream_t st);
+hipError_t spx_launch_mlp_refine(const void* dep, const void* feat, void* peaks, const uint32_t* counts,
+                                 uint32_t batch, uint32_t kcap, float bin_hz, hipStream_t st);
 hipError_t spx_launch_mixed(const FusedArgs* a, int compute, int grid, hipStream_t st);
 hipError_t spx_launch_batch(const BatchArgs* b, int grid, hipStream_t st);
 hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st);

@@ -49,7 +49,6 @@ constexpr int TAL_LDS = TB_LDS + 1024 * 8;        // W_16384^e, e < 128
 constexpr int TAH_LDS = TAL_LDS + 128 * 8;        // W_128^e, e < 128
 constexpr int LDS_BYTES = TAH_LDS + 128 * 8;
 static_assert(XPAD * 4 <= PING_BYTES && XPAD * 4 <= PONG_BYTES, "FP32 spectrum planes fit the operand buffers");
-static_assert(XPAD * 4 + 4352 <= PING_BYTES, "refiner staging fits behind the re plane");
 // global table block (bytes): TB [32][32] float2 | TAL [128] float2 | TAH [128] float2 | W16 frags | W32 frags
 constexpr int TB_OFF = 0;
 constexpr int TAL_OFF = TB_OFF + 1024 * 8;
@@ -132,12 +131,12 @@ __device__ __forceinline__ void load_raw(const FusedArgs& a, uint32_t ch, int w,
 }
 
 // Threshold peak search over the FP32 spectrum planes (bin k at k + k/32);
-// thread t owns natural bins 32t .. 32t+31.  LEARN: stored peaks are refined
-// by the learned refiner (a.mlp) evaluated on MFMA by wave 0 from features
-// staged in `stage` (>= 4352 bytes of LDS), parabolic as the fallback.
+// thread t owns natural bins 32t .. 32t+31.  LEARN: the records get the
+// parabolic offset and the features of every stored peak go to a.mlp_feat,
+// for the learned refinement that runs after the kernel.
 template <bool LEARN>
 __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, const float* xre, const float* xim,
-                                            BlockRed& red, unsigned char* stage) {
+                                            BlockRed& red) {
   constexpr int N = mx::N, T = mx::T, E = N / T;
   const int tid = threadIdx.x;
   auto adr = [](uint32_t k) -> int { return (int)(k + (k >> 5)); };
@@ -178,31 +177,6 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
   }
   int total;
   const int off = block_excl_scan<T>(cpk, total, red);
-  [[maybe_unused]] float* lout = reinterpret_cast<float*>(stage + 4096);
-  if constexpr (LEARN) {
-    _Float16* lfeat = reinterpret_cast<_Float16*>(stage);         // 64 x 16 halves
-    _Float16* lh = reinterpret_cast<_Float16*>(stage + 2048);     // 16 x 64 halves
-    if (flags && off < (int)a.kcap) {
-      uint32_t fl = flags;
-      int pos = off;
-      while (fl && pos < (int)a.kcap) {
-        const int e = __builtin_ctz(fl);
-        fl &= fl - 1;
-        float f[mlp::IN];
-        mlp_features(xat, k0 + e, (uint32_t)N, f);
-#pragma unroll
-        for (int j = 0; j < 16; ++j) lfeat[pos * 16 + j] = (_Float16)(j < mlp::IN ? f[j] : 0.f);
-        ++pos;
-      }
-    }
-    __syncthreads();
-    const int nrb = (min(total, (int)a.kcap) + 15) / 16;
-    if (tid < 64) {
-      for (int rb = 0; rb < nrb; ++rb)
-        mlp_rows16(lfeat + rb * 16 * 16, lh, lout + rb * 16, reinterpret_cast<const unsigned char*>(a.mlp));
-    }
-    __syncthreads();
-  }
   if (flags && off < (int)a.kcap) {
     MaxValue* out = a.peaks + (uint64_t)ch * a.kcap;
     int pos = off;
@@ -213,13 +187,8 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
       const float yc = mag_of(xat(k));
       const float yl = mag_of(xat((k + N - 1) % N));
       const float yr = mag_of(xat((k + 1) % N));
-      if constexpr (LEARN) {
-        const float dl = lout[pos];
-        const float d = (fabsf(dl) <= 0.5f) ? dl : parabolic(yl, yc, yr);
-        write_record_d(out + pos, k, xat(k), yc, d, a.fs / (float)a.nfft);
-      } else {
-        write_record(out + pos, k, xat(k), yl, yc, yr, a);
-      }
+      write_record(out + pos, k, xat(k), yl, yc, yr, a);
+      if constexpr (LEARN) mlp_store_features(xat, k, (uint32_t)N, mlp_feat_row(a, ch, pos));
       ++pos;
     }
   }
@@ -447,7 +416,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
       }
       __syncthreads();
       // the spectrum planes leave 6 KiB at the end of the stage-2 buffer for the refiner staging
-      mixed_peaks<LEARN>(a, ch, xre, xim, red, smem + XPAD * 4);
+      mixed_peaks<LEARN>(a, ch, xre, xim, red);
       __syncthreads();  // the planes are overwritten by the next channel
     }
   }

@@ -255,9 +255,62 @@ __global__ void __launch_bounds__(256) k_mlp_predict(const unsigned char* dep, c
   if (l < 16 && r0 + l < n) gp(out)[r0 + l] = res[wv][l];
 }
 
+// Learned refinement of the stored peak records of the short fused kernels:
+// one wave per channel, 16 peaks per MFMA pass, features from the rows the
+// fused kernel wrote (mlp_store_features).  A prediction inside [-0.5, 0.5]
+// replaces the record's parabolic offset and refined frequency; anything
+// else (outside, not finite) keeps them.
+__global__ void __launch_bounds__(256) k_mlp_refine(const unsigned char* dep, const _Float16* feat,
+                                                    MaxValue* peaks, const uint32_t* counts, uint32_t batch,
+                                                    uint32_t kcap, float bin_hz) {
+  __shared__ __align__(16) _Float16 fbuf[4][16 * 16];
+  __shared__ __align__(16) _Float16 hbuf[4][16 * 64];
+  __shared__ float res[4][16];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const uint32_t ch = blockIdx.x * 4 + wv;
+  if (ch >= batch) return;  // whole waves; the kernel has no workgroup barrier
+  const uint32_t n = min(gp(counts)[2 * ch], kcap);
+  if (n == 0) return;
+  const MlpRegs w = mlp_load(dep);
+  const mlp_h8* src = reinterpret_cast<const mlp_h8*>(feat + (uint64_t)ch * kcap * 16);
+  MaxValue* rec = peaks + (uint64_t)ch * kcap;
+  for (uint32_t r0 = 0; r0 < n; r0 += 16) {
+    // rows r0 .. r0+15 as 32 half-rows of 16 bytes, zeros past the last peak
+    if (l < 32) {
+      const uint32_t row = r0 + (uint32_t)(l >> 1);
+      mlp_h8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (row < n) v = gp(src)[2 * row + (l & 1)];
+      reinterpret_cast<mlp_h8*>(fbuf[wv])[l] = v;
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // LDS writes of this wave complete before mlp_rows16 reads them
+    __builtin_amdgcn_wave_barrier();
+    mlp_rows16(fbuf[wv], hbuf[wv], res[wv], w);
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (l < 16 && r0 + l < n) {
+      const float dl = res[wv][l];
+      if (fabsf(dl) <= 0.5f) {
+        MaxValue* r = rec + r0 + l;
+        const uint32_t k = gp(&r->index)[0];
+        gp(&r->freq_offset)[0] = dl;
+        gp(&r->refined_frequency)[0] = ((float)k + dl) * bin_hz;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // res and fbuf are rewritten by the next pass
+  }
+}
+
 }  // namespace spx
 
 extern "C" {
+
+hipError_t spx_launch_mlp_refine(const void* dep, const void* feat, void* peaks, const uint32_t* counts,
+                                 uint32_t batch, uint32_t kcap, float bin_hz, hipStream_t st) {
+  if (batch == 0 || kcap == 0) return hipSuccess;
+  hipLaunchKernelGGL(spx::k_mlp_refine, dim3((batch + 3) / 4), dim3(256), 0, st, (const unsigned char*)dep,
+                     (const _Float16*)feat, (spx::MaxValue*)peaks, counts, batch, kcap, bin_hz);
+  return hipGetLastError();
+}
 
 int spx_mlp_nparam(void) { return spx::mlp::NPARAM; }
 int spx_mlp_deployed_bytes(void) { return spx::mlp::D_BYTES; }

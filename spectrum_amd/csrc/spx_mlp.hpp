@@ -13,6 +13,7 @@
 // w3 held as float; FP32 accumulation (v_mfma_f32_16x16x32_f16).
 #pragma once
 #include "spx_device.hpp"
+#include "spx_kernels.h"
 
 namespace spx {
 namespace mlp {
@@ -47,21 +48,66 @@ __device__ __forceinline__ void mlp_features(XF X, uint32_t k, uint32_t n, float
   }
 }
 
+// The deployed weights one lane of the evaluating wave needs, in registers:
+// loaded by mlp_load well before mlp_rows16 so that their latency hides
+// behind other work (the global loads stay in flight across barriers).
+struct MlpRegs {
+  mlp_h8 w1[4];      // W1^T fragments, column blocks 0..3
+  mlp_h8 w2[2][4];   // W2^T fragments, K-steps 0..1, column blocks 0..3
+  float b1[4], b2[4], w3[4];
+  float b3;
+};
+
+__device__ __forceinline__ MlpRegs mlp_load(const unsigned char* wts) {
+  using namespace mlp;
+  const int l = threadIdx.x & 63, c = l & 15;
+  const mlp_h8* W1F = reinterpret_cast<const mlp_h8*>(wts + D_W1);
+  const mlp_h8* W2F = reinterpret_cast<const mlp_h8*>(wts + D_W2);
+  MlpRegs r;
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    r.w1[cb] = gp(W1F)[cb * 64 + l];
+    r.w2[0][cb] = gp(W2F)[(0 * 4 + cb) * 64 + l];
+    r.w2[1][cb] = gp(W2F)[(1 * 4 + cb) * 64 + l];
+    r.b1[cb] = gp(reinterpret_cast<const float*>(wts + D_B1))[16 * cb + c];
+    r.b2[cb] = gp(reinterpret_cast<const float*>(wts + D_B2))[16 * cb + c];
+    r.w3[cb] = gp(reinterpret_cast<const float*>(wts + D_W3))[16 * cb + c];
+  }
+  r.b3 = gp(reinterpret_cast<const float*>(wts + D_B3))[0];
+  return r;
+}
+
+// Feature rows of the fused kernels' learned refinement: stored peak `pos`
+// of channel ch at a.mlp_feat + (ch * kcap + pos) * 16 halves.
+__device__ __forceinline__ _Float16* mlp_feat_row(const FusedArgs& a, uint64_t ch, int pos) {
+  return reinterpret_cast<_Float16*>(a.mlp_feat) + (ch * a.kcap + (uint32_t)pos) * 16;
+}
+
+// Features of the peak at k as one 32-byte row of 16 halves (the 10
+// features, then zeros): the row layout mlp_rows16 reads.
+template <class XF>
+__device__ __forceinline__ void mlp_store_features(XF X, uint32_t k, uint32_t n, _Float16* row) {
+  float f[mlp::IN];
+  mlp_features(X, k, n, f);
+  mlp_h8 lo, hi;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    lo[j] = (_Float16)f[j];
+    hi[j] = (_Float16)(j + 8 < mlp::IN ? f[j + 8] : 0.f);
+  }
+  mlp_h8* r = reinterpret_cast<mlp_h8*>(row);
+  gp(r)[0] = lo;
+  gp(r)[1] = hi;
+}
+
 // One wave evaluates the network for 16 rows.  feat: 16 rows x 16 halves in
 // LDS (features in columns 0..9, zeros in 10..15; K columns 16..31 of the
 // MFMA operand are zero), hbuf: 16 x 64 halves of LDS scratch; out[row]
 // receives the offsets (rows >= the caller's count are computed but not
-// meaningful).  All 64 lanes of the wave must call it.
-__device__ __forceinline__ void mlp_rows16(const _Float16* feat, _Float16* hbuf, float* out,
-                                           const unsigned char* wts) {
-  using namespace mlp;
+// meaningful).  All 64 lanes of the wave must call it, with the registers
+// mlp_load gave each of them.
+__device__ __forceinline__ void mlp_rows16(const _Float16* feat, _Float16* hbuf, float* out, const MlpRegs& w) {
   const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
-  const mlp_h8* W1F = reinterpret_cast<const mlp_h8*>(wts + D_W1);
-  const mlp_h8* W2F = reinterpret_cast<const mlp_h8*>(wts + D_W2);
-  const float* B1 = reinterpret_cast<const float*>(wts + D_B1);
-  const float* B2 = reinterpret_cast<const float*>(wts + D_B2);
-  const float* W3 = reinterpret_cast<const float*>(wts + D_W3);
-  const float b3 = gp(reinterpret_cast<const float*>(wts + D_B3))[0];
   // layer 1: A = feat (row l%16, k = 8g + j), B = W1^T fragments
   mlp_h8 a1 = {0, 0, 0, 0, 0, 0, 0, 0};
   if (g < 2) a1 = *reinterpret_cast<const mlp_h8*>(feat + c * 16 + 8 * g);
@@ -69,14 +115,13 @@ __device__ __forceinline__ void mlp_rows16(const _Float16* feat, _Float16* hbuf,
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
     const mlp_f4 z = {0.f, 0.f, 0.f, 0.f};
-    h[cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, gp(W1F)[cb * 64 + l], z, 0, 0, 0);
+    h[cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, w.w1[cb], z, 0, 0, 0);
   }
   // lane holds h[cb][i] = H1[row 4g+i][unit 16cb+c]: bias, ReLU, stage as fp16 rows
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
-    const float bb = gp(B1)[16 * cb + c];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) hbuf[(4 * g + i) * 64 + 16 * cb + c] = (_Float16)fmaxf(h[cb][i] + bb, 0.f);
+    for (int i = 0; i < 4; ++i) hbuf[(4 * g + i) * 64 + 16 * cb + c] = (_Float16)fmaxf(h[cb][i] + w.b1[cb], 0.f);
   }
   __builtin_amdgcn_s_waitcnt(0);  // LDS writes of this wave complete before the reads below
   __builtin_amdgcn_wave_barrier();
@@ -87,11 +132,10 @@ __device__ __forceinline__ void mlp_rows16(const _Float16* feat, _Float16* hbuf,
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
     const mlp_f4 z = {0.f, 0.f, 0.f, 0.f};
-    mlp_f4 d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a20, gp(W2F)[(0 * 4 + cb) * 64 + l], z, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a21, gp(W2F)[(1 * 4 + cb) * 64 + l], d, 0, 0, 0);
-    const float bb = gp(B2)[16 * cb + c], ww = gp(W3)[16 * cb + c];
+    mlp_f4 d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a20, w.w2[0][cb], z, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a21, w.w2[1][cb], d, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) part[i] = fmaf(fmaxf(d[i] + bb, 0.f), ww, part[i]);
+    for (int i = 0; i < 4; ++i) part[i] = fmaf(fmaxf(d[i] + w.b2[cb], 0.f), w.w3[cb], part[i]);
   }
   // layer 3: sum the 16 unit-lanes of each row group
 #pragma unroll
@@ -101,8 +145,15 @@ __device__ __forceinline__ void mlp_rows16(const _Float16* feat, _Float16* hbuf,
   }
   if (c == 0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[4 * g + i] = part[i] + b3;
+    for (int i = 0; i < 4; ++i) out[4 * g + i] = part[i] + w.b3;
   }
+}
+
+// Same, loading the weights itself (callers with nothing to overlap).
+__device__ __forceinline__ void mlp_rows16(const _Float16* feat, _Float16* hbuf, float* out,
+                                           const unsigned char* wts) {
+  const MlpRegs w = mlp_load(wts);
+  mlp_rows16(feat, hbuf, out, w);
 }
 
 }  // namespace spx

@@ -116,3 +116,48 @@ def test_trained_refiner_beats_parabolic_and_fused_inference(ctx):
     assert np.max(np.abs(got[use] - pred[use])) < 2e-3
     assert np.allclose(recs["refined_frequency"][test.channel, test.slot],
                        (recs["index"][test.channel, test.slot] + got) / N, rtol=1e-5, atol=1e-7)
+
+
+def test_fused_learned_many_peaks_matches_predict(ctx):
+    """More than 64 stored peaks per channel (nsigma = 0 on tones in noise):
+    the records equal the parabolic ones except the offset, which is the
+    network's prediction on the peak's features wherever that lies in
+    [-0.5, 0.5] and the parabolic offset elsewhere."""
+    from spectrum_amd.refiner import DEFAULT_WEIGHTS, Refiner
+
+    N, K, B = 4096, 100, 4
+    print(f"[case] learned many peaks N={N} K={K}", flush=True)
+    rng = np.random.default_rng(106)
+    n = np.arange(N)
+    x = (rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))) * 0.3
+    for f in (0.1031, 0.2277, -0.3519):
+        x = x + 4.0 * np.exp(2j * np.pi * f * n)
+    x = x.astype(np.complex64)
+    r = Refiner(ctx).load(DEFAULT_WEIGHTS)
+    r.export()
+    d_in = ctx.to_device(x)
+    out = {}
+    for name, refine in (("learned", capi.REFINE_LEARNED), ("parabolic", capi.REFINE_PARABOLIC)):
+        pk, ct = ctx.alloc(B * K * 32), ctx.alloc(B * 8)
+        ctx.set_refiner(r.deployed if refine == capi.REFINE_LEARNED else None)
+        ctx.fused_peaks(d_in, B, N, N, pk, ct, window=1, nsigma=0.0, max_peaks=K, refine=refine)
+        ctx.sync()
+        out[name] = (pk.download(capi.MAXVALUE_DTYPE, (B, K)), ct.download(np.uint32, (B, 2)))
+    ctx.set_refiner(None)
+    (L, cl), (P, cp) = out["learned"], out["parabolic"]
+    assert np.array_equal(cl, cp) and np.all(cl[:, 0] == K)
+    for f in ("index", "real", "imag", "magnitude", "phase"):
+        assert L[f].tobytes() == P[f].tobytes(), f
+    spec = ctx.alloc(B * N * 8)
+    ctx.fft_c2c(d_in, spec, B, N, N, 1)
+    ctx.sync()
+    X = spec.download(np.complex64, (B, N)).astype(np.complex128)
+    feats = np.array([omlp.features(X[c], int(k)) for c in range(B) for k in L["index"][c]])
+    pred = r.predict(feats.astype(np.float32)).reshape(B, K)
+    got = L["freq_offset"]
+    inside = np.abs(pred) <= 0.5 - 2e-3
+    outside = np.abs(pred) > 0.5 + 2e-3
+    assert inside[:, 64:].any() and inside.sum() >= B * 16  # rows past 64 are covered
+    assert np.max(np.abs(got[inside] - pred[inside])) < 2e-3
+    assert got[outside].tobytes() == P["freq_offset"][outside].tobytes()
+    assert np.allclose(L["refined_frequency"], (L["index"] + got) / N, rtol=1e-5, atol=1e-7)
