@@ -541,8 +541,11 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
   }
 }
 
+// Column-kernel workgroup size per column length: 16 columns per workgroup
+// (128-byte row segments) up to N2 = 512; N2 = 1024 gets 8 (16 would need
+// 1024 threads, fewer registers than its transform needs).
 template <int N2>
-struct FsShape { static constexpr int TC = N2 >= 1024 ? 512 : 256; };
+struct FsShape { static constexpr int TC = N2 >= 512 ? 512 : 256; };
 
 // Dense outputs of long channels (spectrum, |X|, |X|^2 or {|X|, arg X}) after
 // k_fs_cols: the row transforms of k_fs_rows, then a transpose through the
