@@ -135,8 +135,8 @@ __device__ __forceinline__ void fft256_cols16_halfx(float* lf, float2 (&v)[16], 
 // every channel (they are the same for all channels, and re-reading them per
 // channel put as many bytes through the vector-memory pipeline as the input
 // itself).  The arithmetic of a channel does not depend on its neighbours.
-// N2 = 256 uses the half-width exchange fft256_cols16_halfx with its
-// pass-1 twiddles in LDS (twl); other lengths the generic column tile.
+// N2 = 256 uses the half-width exchange fft256_cols16_halfx, other lengths
+// the generic column tile; both read their pass twiddles from LDS (twl).
 constexpr int kFsColsChannels = 4;
 
 template <int N2, int DT, int TC>
@@ -194,7 +194,7 @@ __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& 
     // laundered thread id and table pointers: the transform's twiddle rows
     // are re-read per channel instead of being kept live across the loop
     if constexpr (HALFX) fft256_cols16_halfx<true>(reinterpret_cast<float*>(tile), v, opaque(twl), opaque_v(tid));
-    else fft_tile<LAY1, N2, CB, TC, PPT, 0, false>(tile, v, opaque(a.tw_b), opaque_v(tid));
+    else fft_tile<LAY1, N2, CB, TC, PPT, 0, true>(tile, v, opaque(twl), opaque_v(tid));
     float2* Z = w.Z + (uint64_t)ci * C::N;
 #pragma unroll
     for (int b = 0; b < PPT / RL1; ++b)
@@ -219,9 +219,12 @@ __global__ void __launch_bounds__(TC) k_fs_cols(FusedArgs a, FsWork w) {
   using LAY1 = typename C::LAY1;
   constexpr bool HALFX = (N2 == 256 && C::CB == 16);
   __shared__ __align__(16) float2 tile[HALFX ? 5104 / 2 : LAY1::SIZE];
-  __shared__ __align__(16) float2 twl[HALFX ? Plan<256, 0>::TWSIZE_LDS : 1];
+  // the column plan's twiddle rows (read only after the first exchange's
+  // barriers; from L2 per channel they would load the vector-memory
+  // address unit as much as the input itself)
+  __shared__ __align__(16) float2 twl[Plan<N2, 0>::TWSIZE_LDS];
   __shared__ double red[TC / 64];
-  if constexpr (HALFX) copy_tw_to_lds<256, 0, TC>(a.tw_b, twl, threadIdx.x);  // read after the exchange's barriers
+  copy_tw_to_lds<N2, 0, TC>(a.tw_b, twl, threadIdx.x);
   if (a.dtype == SPX_DTYPE_C64) fs_cols_multi<N2, SPX_DTYPE_C64, TC>(a, w, tile, twl, red);
   else if (a.dtype == SPX_DTYPE_C32F16) fs_cols_multi<N2, SPX_DTYPE_C32F16, TC>(a, w, tile, twl, red);
   else fs_cols_multi<N2, SPX_DTYPE_C32BF16, TC>(a, w, tile, twl, red);
