@@ -203,58 +203,63 @@ int stream_batch(spx_ctx* c, uint32_t batch, const std::vector<HostArray>& array
     return SPX_OK;
   };
 
-  for (uint32_t i = 0; i < nchunks; ++i) {
-    const int s = (int)(i & 1);
-    const uint32_t n = count(i);
-    // 1. input to pinned staging, once the upload of chunk i-2 has read it
-    if (i >= 2) SPX_HIP(c, hipEventSynchronize(h->ev_up[s]));
-    for (size_t j = 0; j < arrays.size(); ++j) {
-      if (!arrays[j].in || pinned[j]) continue;
-      par_copy((char*)h->pin_in[s] + off[j], (const char*)arrays[j].in + first(i) * arrays[j].per_chan,
-               n * arrays[j].per_chan);
-    }
-    // 2. upload, once the kernels of chunk i-2 are done with the slot
-    if (i >= 2) SPX_HIP(c, hipStreamWaitEvent(h->up, h->ev_comp[s], 0));
-    std::vector<const void*> din;
-    std::vector<void*> dout;
-    for (size_t j = 0; j < arrays.size(); ++j) {
-      if (arrays[j].in) {
-        const void* src = pinned[j] ? (const char*)arrays[j].in + first(i) * arrays[j].per_chan
-                                    : (const char*)h->pin_in[s] + off[j];
-        void* dst = (char*)h->dev_in[s] + off[j];
-        SPX_HIP(c, hipMemcpyAsync(dst, src, n * arrays[j].per_chan, hipMemcpyHostToDevice, h->up));
-        din.push_back(dst);
-      } else {
-        dout.push_back((char*)h->dev_out[s] + off[j]);
+  // every exit, failed or not, leaves no copy or kernel of this call in
+  // flight (the staging buffers and the caller's pinned arrays are free)
+  auto run = [&]() -> int {
+    for (uint32_t i = 0; i < nchunks; ++i) {
+      const int s = (int)(i & 1);
+      const uint32_t n = count(i);
+      // 1. input to pinned staging, once the upload of chunk i-2 has read it
+      if (i >= 2) SPX_HIP(c, hipEventSynchronize(h->ev_up[s]));
+      for (size_t j = 0; j < arrays.size(); ++j) {
+        if (!arrays[j].in || pinned[j]) continue;
+        par_copy((char*)h->pin_in[s] + off[j], (const char*)arrays[j].in + first(i) * arrays[j].per_chan,
+                 n * arrays[j].per_chan);
       }
+      // 2. upload, once the kernels of chunk i-2 are done with the slot
+      if (i >= 2) SPX_HIP(c, hipStreamWaitEvent(h->up, h->ev_comp[s], 0));
+      std::vector<const void*> din;
+      std::vector<void*> dout;
+      for (size_t j = 0; j < arrays.size(); ++j) {
+        if (arrays[j].in) {
+          const void* src = pinned[j] ? (const char*)arrays[j].in + first(i) * arrays[j].per_chan
+                                      : (const char*)h->pin_in[s] + off[j];
+          void* dst = (char*)h->dev_in[s] + off[j];
+          SPX_HIP(c, hipMemcpyAsync(dst, src, n * arrays[j].per_chan, hipMemcpyHostToDevice, h->up));
+          din.push_back(dst);
+        } else {
+          dout.push_back((char*)h->dev_out[s] + off[j]);
+        }
+      }
+      SPX_HIP(c, hipEventRecord(h->ev_up[s], h->up));
+      // 3. kernels, once the upload is in and the download of chunk i-2 has
+      //    read the output slot
+      SPX_HIP(c, hipStreamWaitEvent(c->stream, h->ev_up[s], 0));
+      if (i >= 2) SPX_HIP(c, hipStreamWaitEvent(c->stream, h->ev_down[s], 0));
+      if ((rc = compute(din, dout, n))) return rc;
+      SPX_HIP(c, hipEventRecord(h->ev_comp[s], c->stream));
+      // 4. download
+      SPX_HIP(c, hipStreamWaitEvent(h->down, h->ev_comp[s], 0));
+      for (size_t j = 0; j < arrays.size(); ++j) {
+        if (!arrays[j].out) continue;
+        void* dst = pinned[j] ? (char*)arrays[j].out + first(i) * arrays[j].per_chan : (char*)h->pin_out[s] + off[j];
+        SPX_HIP(c, hipMemcpyAsync(dst, (char*)h->dev_out[s] + off[j], n * arrays[j].per_chan, hipMemcpyDeviceToHost,
+                                  h->down));
+      }
+      SPX_HIP(c, hipEventRecord(h->ev_down[s], h->down));
+      // 5. the previous chunk's results to the caller
+      if (i >= 1 && (rc = drain(i - 1))) return rc;
     }
-    SPX_HIP(c, hipEventRecord(h->ev_up[s], h->up));
-    // 3. kernels, once the upload is in and the download of chunk i-2 has
-    //    read the output slot
-    SPX_HIP(c, hipStreamWaitEvent(c->stream, h->ev_up[s], 0));
-    if (i >= 2) SPX_HIP(c, hipStreamWaitEvent(c->stream, h->ev_down[s], 0));
-    if ((rc = compute(din, dout, n))) {
-      (void)hipStreamSynchronize(c->stream);
-      (void)hipStreamSynchronize(h->up);
-      (void)hipStreamSynchronize(h->down);
-      return rc;
-    }
-    SPX_HIP(c, hipEventRecord(h->ev_comp[s], c->stream));
-    // 4. download
-    SPX_HIP(c, hipStreamWaitEvent(h->down, h->ev_comp[s], 0));
-    for (size_t j = 0; j < arrays.size(); ++j) {
-      if (!arrays[j].out) continue;
-      void* dst = pinned[j] ? (char*)arrays[j].out + first(i) * arrays[j].per_chan : (char*)h->pin_out[s] + off[j];
-      SPX_HIP(c, hipMemcpyAsync(dst, (char*)h->dev_out[s] + off[j], n * arrays[j].per_chan, hipMemcpyDeviceToHost,
-                                h->down));
-    }
-    SPX_HIP(c, hipEventRecord(h->ev_down[s], h->down));
-    // 5. the previous chunk's results to the caller
-    if (i >= 1 && (rc = drain(i - 1))) return rc;
-  }
-  if ((rc = drain(nchunks - 1))) return rc;
-  SPX_HIP(c, hipStreamSynchronize(h->down));
-  SPX_HIP(c, hipStreamSynchronize(c->stream));
+    return drain(nchunks - 1);
+  };
+  rc = run();
+  const hipError_t e1 = hipStreamSynchronize(h->up);
+  const hipError_t e2 = hipStreamSynchronize(h->down);
+  const hipError_t e3 = hipStreamSynchronize(c->stream);
+  if (rc) return rc;
+  SPX_HIP(c, e1);
+  SPX_HIP(c, e2);
+  SPX_HIP(c, e3);
   return SPX_OK;
 }
 
