@@ -136,3 +136,30 @@ def test_host_errors_leave_context_usable(ctx):
         ctx.fft_c2c_host(x, 100)  # not a power of two
     got = ctx.fft_c2c_host(x, 128)
     assert got.shape == (2, 128) and abs(got[0, 0] - 100) < 1e-4
+
+
+@pytest.mark.parametrize("nfft,dtype", [(4096, capi.DTYPE_C32F16), (65536, capi.DTYPE_C32BF16)])
+def test_host_half_storage_with_scale(ctx, nfft, dtype):
+    """16-bit complex input (4 bytes per sample) and a per-channel load
+    scale, both sliced per chunk: same records as the device entry point."""
+    B, K = 9, 32
+    print(f"[case] host half storage nfft={nfft} dtype={dtype}", flush=True)
+    x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED00A0 + nfft, B, nfft)
+    u = capi.to_half_storage(x * 0.25, dtype)
+    scale = (1.0 + np.arange(B, dtype=np.float32)) * 0.5
+    ctx.set_host_chunk(2 * nfft * 8)
+    try:
+        got = ctx.fused_peaks_host(u, nfft, max_peaks=K, dtype=dtype, in_scale=scale, window=1, fs=1000.0)
+    finally:
+        ctx.set_host_chunk(32 << 20)
+    d_in, d_sc = ctx.to_device(u), ctx.to_device(scale)
+    pk, ct, g, st = ctx.alloc(B * K * 32), ctx.alloc(B * 8), ctx.alloc(B * 32), ctx.alloc(B * 16)
+    ctx.fused_peaks(d_in, B, nfft, nfft, pk, ct, max_peaks=K, dtype=dtype, in_scale=d_sc, gmax=g, stats=st,
+                    window=1, fs=1000.0)
+    ctx.sync()
+    ref = (pk.download(capi.MAXVALUE_DTYPE, (B, K)), ct.download(np.uint32, (B, 2)),
+           g.download(capi.MAXVALUE_DTYPE, (B,)), st.download(np.float32, (B, 4)))
+    assert_same_records(got, ref)
+    # the scale reached the kernels: the mean |X| of channel c scales with scale[c]
+    plain = ctx.fused_peaks_host(u, nfft, max_peaks=K, dtype=dtype, window=1, fs=1000.0)
+    assert np.allclose(got[3][:, 0], plain[3][:, 0] * scale, rtol=1e-5)
