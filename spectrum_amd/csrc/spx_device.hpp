@@ -343,34 +343,68 @@ __device__ __forceinline__ void fft_tile(float2* lds, float2 (&v)[PPT], const fl
 // ---------------------------------------------------------------------------
 // wave64 / block reductions and scans
 // ---------------------------------------------------------------------------
+// Cross-lane steps on DPP (data moved by the vector ALU, a few cycles per
+// step, instead of ds_bpermute through the LDS crossbar).  The scan is the
+// row-shift / row-broadcast sequence: within each row of 16 lanes, shifts by
+// 1, 2, 4, 8 give inclusive prefixes (a lane without a source lane keeps the
+// identity `id`); row_bcast15 then adds row 0's total to row 1 and row 2's
+// to row 3, and row_bcast31 adds rows 0-1's total to rows 2 and 3, so lane i
+// ends with the inclusive prefix over lanes 0..i.
+namespace dpp {
+constexpr int kShr1 = 0x111, kShr2 = 0x112, kShr4 = 0x114, kShr8 = 0x118, kBcast15 = 0x142, kBcast31 = 0x143;
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int mov(int id, int x) {
+  return __builtin_amdgcn_update_dpp(id, x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double mov(double id, double x) {
+  const int2 a = __builtin_bit_cast(int2, x), o = __builtin_bit_cast(int2, id);
+  return __builtin_bit_cast(double, make_int2(mov<CTRL, ROWS>(o.x, a.x), mov<CTRL, ROWS>(o.y, a.y)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float mov(float id, float x) {
+  return __builtin_bit_cast(float, mov<CTRL, ROWS>(__builtin_bit_cast(int, id), __builtin_bit_cast(int, x)));
+}
+// inclusive scan of x with the associative, commutative op f and identity id
+template <class V, class F>
+__device__ __forceinline__ V scan(V x, V id, F f) {
+  x = f(x, mov<kShr1, 0xf>(id, x));
+  x = f(x, mov<kShr2, 0xf>(id, x));
+  x = f(x, mov<kShr4, 0xf>(id, x));
+  x = f(x, mov<kShr8, 0xf>(id, x));
+  x = f(x, mov<kBcast15, 0xa>(id, x));
+  x = f(x, mov<kBcast31, 0xc>(id, x));
+  return x;
+}
+__device__ __forceinline__ double lane63(double x) {
+  const int2 a = __builtin_bit_cast(int2, x);
+  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(a.x, 63), __builtin_amdgcn_readlane(a.y, 63)));
+}
+}  // namespace dpp
+
+// sum over the wave, the same value in every lane
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+  return dpp::lane63(dpp::scan(x, 0.0, [](double a, double b) { return a + b; }));
 }
-__device__ __forceinline__ float wave_sumf(float x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
-// argmax with lowest index on ties; (m, k) pairs
+// argmax with lowest index on ties; (m, k) pairs, the result in every lane
 __device__ __forceinline__ void wave_argmax(float& m, uint32_t& k) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    float om = __shfl_xor(m, o, 64);
-    uint32_t ok = __shfl_xor(k, o, 64);
+  auto step = [&](float om, uint32_t ok) {
     if (om > m || (om == m && ok < k)) { m = om; k = ok; }
-  }
+  };
+  const float ni = -__builtin_inff();
+  const int nk = -1;  // 0xffffffff
+  step(dpp::mov<dpp::kShr1, 0xf>(ni, m), (uint32_t)dpp::mov<dpp::kShr1, 0xf>(nk, (int)k));
+  step(dpp::mov<dpp::kShr2, 0xf>(ni, m), (uint32_t)dpp::mov<dpp::kShr2, 0xf>(nk, (int)k));
+  step(dpp::mov<dpp::kShr4, 0xf>(ni, m), (uint32_t)dpp::mov<dpp::kShr4, 0xf>(nk, (int)k));
+  step(dpp::mov<dpp::kShr8, 0xf>(ni, m), (uint32_t)dpp::mov<dpp::kShr8, 0xf>(nk, (int)k));
+  step(dpp::mov<dpp::kBcast15, 0xa>(ni, m), (uint32_t)dpp::mov<dpp::kBcast15, 0xa>(nk, (int)k));
+  step(dpp::mov<dpp::kBcast31, 0xc>(ni, m), (uint32_t)dpp::mov<dpp::kBcast31, 0xc>(nk, (int)k));
+  m = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, m), 63));
+  k = (uint32_t)__builtin_amdgcn_readlane((int)k, 63);
 }
 // inclusive scan of ints across the wave
 __device__ __forceinline__ int wave_incl_scan(int x) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  return x;
+  return dpp::scan(x, 0, [](int a, int b) { return a + b; });
 }
 
 struct BlockRed {
