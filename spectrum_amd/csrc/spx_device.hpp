@@ -386,6 +386,11 @@ __device__ __forceinline__ double lane63(double x) {
 __device__ __forceinline__ double wave_sum(double x) {
   return dpp::lane63(dpp::scan(x, 0.0, [](double a, double b) { return a + b; }));
 }
+// maximum over the wave (fmaxf: NaN lanes ignored), the same value in every lane
+__device__ __forceinline__ float wave_maxf(float x) {
+  x = dpp::scan(x, -__builtin_inff(), [](float a, float b) { return fmaxf(a, b); });
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+}
 // argmax with lowest index on ties; (m, k) pairs, the result in every lane
 __device__ __forceinline__ void wave_argmax(float& m, uint32_t& k) {
   auto step = [&](float om, uint32_t ok) {

@@ -264,8 +264,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
         const float2 v = unpack_in<DT>(raw[t][j]);
         mx_ = fmaxf(mx_, fmaxf(fabsf(v.x), fabsf(v.y)));
       }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx_ = fmaxf(mx_, __shfl_xor(mx_, o, 64));
+    mx_ = wave_maxf(mx_);
     if (l == 0) red.m[w] = mx_;
     __syncthreads();
     float M = 0.f;
