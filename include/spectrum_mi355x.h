@@ -170,8 +170,8 @@ int spx_fft_magnitude_host(spx_ctx* ctx, const void* in, float* out, uint32_t ba
                            uint32_t nfft, uint32_t window, int squared, float scale);
 int spx_fused_peaks_host(spx_ctx* ctx, const spx_fused_desc* desc);
 /* Bytes per transfer chunk of the *_host calls (default 32 MiB; the larger of
- * a chunk's input and output).  The context keeps two chunks of pinned and
- * two of device memory per direction. */
+ * a chunk's input and output).  The context keeps three chunks of pinned and
+ * three of device memory per direction. */
 int spx_set_host_chunk(spx_ctx* ctx, uint64_t bytes);
 /* Pinned (page-locked) host memory for the *_host calls' buffers: transfers
  * from and to it need no staging copy. */

@@ -8,16 +8,19 @@
 // spectrum_processor_rocm.cpp:263-307, 444-543) upload the whole batch from
 // pageable memory, compute, then download the whole result, one step after
 // the other; its own profile of that path is ~96 % copies.  Here, per chunk
-// i (slot s = i % 2 of two):
+// i (slot s = i % 3 of three):
 //   host thread  copy chunk i of the caller's input into pinned staging
 //                (skipped when the caller's buffer is already pinned)
 //   stream up    pinned -> device input slot s
 //   ctx stream   the device entry point on slot s (spx_fft_c2c, ...,
 //                spx_fused_peaks), writing device output slot s
 //   stream down  device output slot s -> pinned (or the caller's pinned buffer)
-//   host thread  copy chunk i-1's results from pinned staging to the caller
+//   host thread  copy chunk i-2's results from pinned staging to the caller
 // Events order slot reuse: the upload into slot s waits for the kernels of
-// chunk i-2, the kernels wait for the download of chunk i-2.  Every call
+// chunk i-3, the kernels wait for the download of chunk i-3.  With two slots
+// the host had to wait for chunk i-1's download before it could stage chunk
+// i+1, so each chunk took upload + download in turn; the third slot lets the
+// host copy out a chunk whose download finished a chunk earlier.  Every call
 // returns with its results in host memory (the reference's synchronous
 // semantics).
 #include <algorithm>
@@ -29,13 +32,15 @@
 #include "spx_host.h"
 #include "spectrum_mi355x.h"
 
+constexpr int kSlots = 3;
+
 struct HostStage {
   hipStream_t up = nullptr, down = nullptr;
-  hipEvent_t ev_up[2] = {}, ev_comp[2] = {}, ev_down[2] = {};
-  void* pin_in[2] = {};
-  void* pin_out[2] = {};
-  void* dev_in[2] = {};
-  void* dev_out[2] = {};
+  hipEvent_t ev_up[kSlots] = {}, ev_comp[kSlots] = {}, ev_down[kSlots] = {};
+  void* pin_in[kSlots] = {};
+  void* pin_out[kSlots] = {};
+  void* dev_in[kSlots] = {};
+  void* dev_out[kSlots] = {};
   uint64_t in_cap = 0, out_cap = 0;  // bytes per slot
 };
 
@@ -47,7 +52,7 @@ void host_stage_free(spx_ctx* c) {
   (void)hipSetDevice(c->device);
   if (h->up) (void)hipStreamSynchronize(h->up);
   if (h->down) (void)hipStreamSynchronize(h->down);
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < kSlots; ++s) {
     if (h->pin_in[s]) (void)hipHostFree(h->pin_in[s]);
     if (h->pin_out[s]) (void)hipHostFree(h->pin_out[s]);
     if (h->dev_in[s]) (void)hipFree(h->dev_in[s]);
@@ -127,7 +132,7 @@ int stage_for(spx_ctx* c, uint64_t in_bytes, uint64_t out_bytes, HostStage** out
     c->hstage = h;
     SPX_HIP(c, hipStreamCreateWithFlags(&h->up, hipStreamNonBlocking));
     SPX_HIP(c, hipStreamCreateWithFlags(&h->down, hipStreamNonBlocking));
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kSlots; ++s) {
       SPX_HIP(c, hipEventCreateWithFlags(&h->ev_up[s], hipEventDisableTiming));
       SPX_HIP(c, hipEventCreateWithFlags(&h->ev_comp[s], hipEventDisableTiming));
       SPX_HIP(c, hipEventCreateWithFlags(&h->ev_down[s], hipEventDisableTiming));
@@ -136,14 +141,14 @@ int stage_for(spx_ctx* c, uint64_t in_bytes, uint64_t out_bytes, HostStage** out
   HostStage* h = c->hstage;
   int rc;
   if (in_bytes > h->in_cap) {
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kSlots; ++s) {
       if ((rc = grow(c, &h->pin_in[s], in_bytes, true))) return rc;
       if ((rc = grow(c, &h->dev_in[s], in_bytes, false))) return rc;
     }
     h->in_cap = in_bytes;
   }
   if (out_bytes > h->out_cap) {
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kSlots; ++s) {
       if ((rc = grow(c, &h->pin_out[s], out_bytes, true))) return rc;
       if ((rc = grow(c, &h->dev_out[s], out_bytes, false))) return rc;
     }
@@ -184,13 +189,24 @@ int stream_batch(spx_ctx* c, uint32_t batch, const std::vector<HostArray>& array
   if (rc) return rc;
   std::vector<bool> pinned(arrays.size());
   for (size_t i = 0; i < arrays.size(); ++i) pinned[i] = is_pinned(arrays[i].in ? arrays[i].in : arrays[i].out);
+  // Uploads straight from one large pinned caller buffer while downloads go
+  // straight into another ran one after the other on MI355X (1024 x 64 K
+  // complex spectrum: 20-22 ms); staging the input through the two small
+  // slots let the directions overlap (13-14 ms).  So with a dense output
+  // that is pinned, pinned inputs are staged as well.
+  bool dense_pinned_out = false;
+  for (size_t i = 0; i < arrays.size(); ++i)
+    dense_pinned_out |= arrays[i].out && pinned[i] && 2 * arrays[i].per_chan >= in_per;
+  if (dense_pinned_out)
+    for (size_t i = 0; i < arrays.size(); ++i)
+      if (arrays[i].in) pinned[i] = false;
 
   const uint32_t nchunks = (batch + chunk - 1) / chunk;
   auto first = [&](uint32_t i) { return (uint64_t)i * chunk; };
   auto count = [&](uint32_t i) { return (uint32_t)std::min<uint64_t>(chunk, batch - first(i)); };
   // results of chunk i from pinned staging to the caller's arrays
   auto drain = [&](uint32_t i) -> int {
-    const int s = (int)(i & 1);
+    const int s = (int)(i % kSlots);
     bool staged = false;
     for (size_t j = 0; j < arrays.size(); ++j) staged |= arrays[j].out && !pinned[j];
     if (!staged) return SPX_OK;
@@ -207,17 +223,17 @@ int stream_batch(spx_ctx* c, uint32_t batch, const std::vector<HostArray>& array
   // flight (the staging buffers and the caller's pinned arrays are free)
   auto run = [&]() -> int {
     for (uint32_t i = 0; i < nchunks; ++i) {
-      const int s = (int)(i & 1);
+      const int s = (int)(i % kSlots);
       const uint32_t n = count(i);
-      // 1. input to pinned staging, once the upload of chunk i-2 has read it
-      if (i >= 2) SPX_HIP(c, hipEventSynchronize(h->ev_up[s]));
+      // 1. input to pinned staging, once the upload of chunk i-3 has read it
+      if (i >= kSlots) SPX_HIP(c, hipEventSynchronize(h->ev_up[s]));
       for (size_t j = 0; j < arrays.size(); ++j) {
         if (!arrays[j].in || pinned[j]) continue;
         par_copy((char*)h->pin_in[s] + off[j], (const char*)arrays[j].in + first(i) * arrays[j].per_chan,
                  n * arrays[j].per_chan);
       }
-      // 2. upload, once the kernels of chunk i-2 are done with the slot
-      if (i >= 2) SPX_HIP(c, hipStreamWaitEvent(h->up, h->ev_comp[s], 0));
+      // 2. upload, once the kernels of chunk i-3 are done with the slot
+      if (i >= kSlots) SPX_HIP(c, hipStreamWaitEvent(h->up, h->ev_comp[s], 0));
       std::vector<const void*> din;
       std::vector<void*> dout;
       for (size_t j = 0; j < arrays.size(); ++j) {
@@ -232,10 +248,10 @@ int stream_batch(spx_ctx* c, uint32_t batch, const std::vector<HostArray>& array
         }
       }
       SPX_HIP(c, hipEventRecord(h->ev_up[s], h->up));
-      // 3. kernels, once the upload is in and the download of chunk i-2 has
+      // 3. kernels, once the upload is in and the download of chunk i-3 has
       //    read the output slot
       SPX_HIP(c, hipStreamWaitEvent(c->stream, h->ev_up[s], 0));
-      if (i >= 2) SPX_HIP(c, hipStreamWaitEvent(c->stream, h->ev_down[s], 0));
+      if (i >= kSlots) SPX_HIP(c, hipStreamWaitEvent(c->stream, h->ev_down[s], 0));
       if ((rc = compute(din, dout, n))) return rc;
       SPX_HIP(c, hipEventRecord(h->ev_comp[s], c->stream));
       // 4. download
@@ -247,10 +263,13 @@ int stream_batch(spx_ctx* c, uint32_t batch, const std::vector<HostArray>& array
                                   h->down));
       }
       SPX_HIP(c, hipEventRecord(h->ev_down[s], h->down));
-      // 5. the previous chunk's results to the caller
-      if (i >= 1 && (rc = drain(i - 1))) return rc;
+      // 5. the results of the chunk before the previous one to the caller
+      //    (its slot is reused by chunk i+1's download)
+      if (i >= 2 && (rc = drain(i - 2))) return rc;
     }
-    return drain(nchunks - 1);
+    for (uint32_t i = nchunks >= 2 ? nchunks - 2 : 0; i < nchunks; ++i)
+      if ((rc = drain(i))) return rc;
+    return SPX_OK;
   };
   rc = run();
   const hipError_t e1 = hipStreamSynchronize(h->up);
