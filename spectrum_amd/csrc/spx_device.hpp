@@ -40,6 +40,42 @@ __device__ __forceinline__ auto gp(T* p) {
 #endif
 }
 
+// Streaming (non-temporal) complex64 access for data a launch touches once:
+// the signal, the four-step intermediate Z.  On MI355X plain copies with the
+// split path's access patterns ran 7 % faster with non-temporal loads and
+// stores (16-column tiles) and plain reads of contiguous rows 14 % faster.
+typedef float stream_float2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const stream_float2 v =
+      __builtin_nontemporal_load((__attribute__((address_space(1))) const stream_float2*)p);
+  return make_float2(v.x, v.y);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ void st_stream(float2* p, float2 x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  stream_float2 v;
+  v.x = x.x;
+  v.y = x.y;
+  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) stream_float2*)p);
+#else
+  *p = x;
+#endif
+}
+
+template <bool STREAM>
+__device__ __forceinline__ float2 ld_f2(const float2* p) {
+  if constexpr (STREAM) return ld_stream(p);
+  else return *gp(p);
+}
+template <bool STREAM>
+__device__ __forceinline__ void st_f2(float2* p, float2 x) {
+  if constexpr (STREAM) st_stream(p, x);
+  else *gp(p) = x;
+}
+
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 constexpr int ipow16(int p) { return p <= 0 ? 1 : 16 * ipow16(p - 1); }
 
