@@ -76,6 +76,7 @@ __device__ __forceinline__ void st_f2(float2* p, float2 x) {
   else *gp(p) = x;
 }
 
+
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 constexpr int ipow16(int p) { return p <= 0 ? 1 : 16 * ipow16(p - 1); }
 

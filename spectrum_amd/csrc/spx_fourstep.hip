@@ -601,7 +601,7 @@ __global__ void __launch_bounds__(256) k_fs_rows_dense(FusedArgs a, FsWork w) {
   for (int q = 0; q < PPT; ++q) x[q].y = ldsf[17 * (k1_out + 16 * q) + s_out];
   const uint64_t cbase = och * (uint64_t)C::N + (uint64_t)(t * RB + s_out);
 #pragma unroll
-  for (int q = 0; q < PPT; ++q) store_dense(a, cbase + (uint64_t)N2 * (uint32_t)(k1_out + 16 * q), x[q]);
+  for (int q = 0; q < PPT; ++q) store_dense<C::STREAM_Z>(a, cbase + (uint64_t)N2 * (uint32_t)(k1_out + 16 * q), x[q]);
 }
 
 template <int N2, int TC>

@@ -136,8 +136,11 @@ __device__ __forceinline__ bool in_range(uint32_t k, const FusedArgs& a) { retur
 
 // dense spectrum / magnitude outputs (phase needs atan2f: kept out of line)
 __device__ __noinline__ void store_magphase(float2* dst, float2 x) { *gp(dst) = make_float2(mag_of(x), atan2f(x.y, x.x)); }
+// STREAM: non-temporal stores of the complex spectrum (measured faster on the
+// split path; for the 4-byte |X| outputs they measured slower and stay plain)
+template <bool STREAM = false>
 __device__ __forceinline__ void store_dense(const FusedArgs& a, uint64_t i, float2 x) {
-  if (a.spec_out) gp(a.spec_out)[i] = x;
+  if (a.spec_out) st_f2<STREAM>(a.spec_out + i, x);
   if (a.mag_out) {
     if (a.mag_kind == SPX_MAG_ABS) gp(a.mag_out)[i] = mag_of(x) * a.mag_scale;
     else if (a.mag_kind == SPX_MAG_SQUARED) gp(a.mag_out)[i] = fmaf(x.x, x.x, x.y * x.y) * a.mag_scale;
