@@ -40,10 +40,10 @@ __device__ __forceinline__ auto gp(T* p) {
 #endif
 }
 
-// Streaming (non-temporal) complex64 access for data a launch touches once:
-// the signal, the four-step intermediate Z.  On MI355X plain copies with the
-// split path's access patterns ran 7 % faster with non-temporal loads and
-// stores (16-column tiles) and plain reads of contiguous rows 14 % faster.
+// Streaming (non-temporal) complex64 access for data a launch touches once
+// (the four-step intermediate Z read by the row kernels, the dense spectrum
+// they write).  On MI355X plain reads of contiguous rows ran 14 % faster
+// with non-temporal loads.
 typedef float stream_float2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 ld_stream(const float2* p) {
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -51,8 +51,9 @@ struct FsCfg {
   static constexpr int TF = 64;                // k_fs_finish workgroup: one wave (its launch bound)
   static constexpr int BSH = ilog2(N) - ilog2(TF);
   static constexpr uint64_t CAND_CAP = N / 2 + 2;
-  // the row kernels read Z non-temporally when the column kernel wrote it so
-  // (16-column blocks)
+  // the row kernels read Z (and the dense path stores the spectrum) with
+  // non-temporal accesses where the column blocks are 16 wide (N2 = 256,
+  // 512): measured faster there, slower at N2 = 1024 and no faster at 128
   static constexpr bool STREAM_Z = (CB == 16);
   using LAY1 = LayInter<N2, CB>;
   using LAY2 = LayContig<N1, RB>;
@@ -149,10 +150,6 @@ __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& 
   constexpr int N1 = C::N1, CB = C::CB, NT1 = C::NT1, PPT = C::PPT, MC = kFsColsChannels;
   constexpr int RL1 = Plan<N2, 0>::RL;
   constexpr bool HALFX = (N2 == 256 && CB == 16);
-  // non-temporal input loads and Z stores where the block's row segments are
-  // 128 bytes (16 columns): 2^16 and 2^17 measured 3-5 % faster with them,
-  // 2^15 (32 columns) no faster and 2^18 (8 columns, 64-byte segments) slower
-  constexpr bool STREAM = (CB == 16);
   using LAY1 = typename C::LAY1;
   constexpr uint32_t ES = DT == SPX_DTYPE_C64 ? 8 : 4;
   const int tid = threadIdx.x;
@@ -191,7 +188,7 @@ __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& 
 #pragma unroll
     for (int r = 0; r < PPT; ++r) {
       const uint32_t n = n0 + r * stride;
-      const float2 raw = ld_raw<DT, STREAM>(cb, min(n, last));
+      const float2 raw = ld_raw<DT>(cb, min(n, last));
       v[r] = c_scale(raw, n < a.n_point ? wv[r] * sc : 0.0f);
     }
     float e = 0.f;
@@ -207,7 +204,7 @@ __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& 
     for (int b = 0; b < PPT / RL1; ++b)
 #pragma unroll
       for (int r = 0; r < RL1; ++r)
-        st_f2<STREAM>(Z + zoff[b] + (uint32_t)(r * (N2 / RL1)) * N1, c_mul(v[b * RL1 + r], t4[b * RL1 + r]));
+        gp(Z)[zoff[b] + (uint32_t)(r * (N2 / RL1)) * N1] = c_mul(v[b * RL1 + r], t4[b * RL1 + r]);
     const double E = block_sum_d<TC>((double)e, red);
     if (tid == 0) {
       gp(w.pe)[(uint64_t)ci * NT1 + t] = E;
