@@ -198,16 +198,21 @@ def extra_uneven(ctx, K=64):
     d = ctx.to_device(x)
     pk, ct = ctx.alloc(n * K * 32), ctx.alloc(n * 8)
     plan = ctx.batch_plan(ctx.batch_desc(d, off, L, pk, ct, max_peaks=K))
-    t_batch = _timer_ms(ctx, plan.execute)
-    info = plan.info()
-    plan.close()
     bufs = {Lx: (ctx.alloc(len(ids) * K * 32), ctx.alloc(len(ids) * 8)) for Lx, (o, ids) in groups.items()}
 
     def seq():
         for Lx, (o, ids) in sorted(groups.items()):
             ctx.fused_peaks(d.ptr + 8 * o, len(ids), Lx, Lx, bufs[Lx][0], bufs[Lx][1], max_peaks=K)
 
-    t_seq = _timer_ms(ctx, seq)
+    # the two forms alternate over three rounds (the first timed round of a
+    # run reads slow), medians reported
+    tb, ts = [], []
+    for _ in range(3):
+        tb.append(_timer_ms(ctx, plan.execute))
+        ts.append(_timer_ms(ctx, seq))
+    t_batch, t_seq = float(np.median(tb)), float(np.median(ts))
+    info = plan.info()
+    plan.close()
     alg = total * 8 + n * (K * 32 + 8)
     return {"workload": f"{n} channels, lengths 2^10..2^17 log-uniform (complex64, {total * 8 / 1e9:.2f} GB), "
                         f"Hann, mean+6std, K={K}", "batch_graph_ms": t_batch, "per_length_launches_ms": t_seq,
