@@ -41,9 +41,9 @@ __device__ __forceinline__ auto gp(T* p) {
 }
 
 // Streaming (non-temporal) complex64 access for data a launch touches once
-// (the four-step intermediate Z read by the row kernels, the dense spectrum
-// they write).  On MI355X plain reads of contiguous rows ran 14 % faster
-// with non-temporal loads.
+// and nothing reads again soon (the signal, the four-step intermediate Z as
+// the row kernels read it, the dense spectrum they write).  On MI355X plain
+// reads of contiguous rows ran 14 % faster with non-temporal loads.
 typedef float stream_float2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 ld_stream(const float2* p) {
 #if defined(__HIP_DEVICE_COMPILE__)

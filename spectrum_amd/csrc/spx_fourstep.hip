@@ -51,9 +51,11 @@ struct FsCfg {
   static constexpr int TF = 64;                // k_fs_finish workgroup: one wave (its launch bound)
   static constexpr int BSH = ilog2(N) - ilog2(TF);
   static constexpr uint64_t CAND_CAP = N / 2 + 2;
-  // the row kernels read Z (and the dense path stores the spectrum) with
-  // non-temporal accesses where the column blocks are 16 wide (N2 = 256,
-  // 512): measured faster there, slower at N2 = 1024 and no faster at 128
+  // where the column blocks are 16 wide (N2 = 256, 512) the column kernel
+  // reads the signal and the row kernels read Z with non-temporal loads, and
+  // the dense path stores the spectrum non-temporally: measured faster there,
+  // slower at N2 = 1024 and no faster at 128.  Z itself is stored plainly:
+  // the row kernels find part of it still in the Infinity Cache.
   static constexpr bool STREAM_Z = (CB == 16);
   using LAY1 = LayInter<N2, CB>;
   using LAY2 = LayContig<N1, RB>;
@@ -188,7 +190,7 @@ __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& 
 #pragma unroll
     for (int r = 0; r < PPT; ++r) {
       const uint32_t n = n0 + r * stride;
-      const float2 raw = ld_raw<DT>(cb, min(n, last));
+      const float2 raw = ld_raw<DT, C::STREAM_Z>(cb, min(n, last));
       v[r] = c_scale(raw, n < a.n_point ? wv[r] * sc : 0.0f);
     }
     float e = 0.f;

@@ -15,10 +15,11 @@ using MaxValue = spx_max_value;
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-template <int DT>
+// STREAM: non-temporal complex64 loads (the split path's column kernel)
+template <int DT, bool STREAM = false>
 __device__ __forceinline__ float2 ld_raw(const void* chan_base, uint32_t i) {
   if constexpr (DT == SPX_DTYPE_C64) {
-    return gp(reinterpret_cast<const float2*>(chan_base))[i];
+    return ld_f2<STREAM>(reinterpret_cast<const float2*>(chan_base) + i);
   } else if constexpr (DT == SPX_DTYPE_C32F16) {
     const __half2 h = gp(reinterpret_cast<const __half2*>(chan_base))[i];
     return __half22float2(h);
