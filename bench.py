@@ -286,8 +286,10 @@ def extra_refiner(ctx, K=64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    # defaults: the first few dozen calls after start-up run ~10 % slower
+    # (clocks ramping up); 50 warm-up calls reach the steady state
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--nfft", type=int, default=65536)
     ap.add_argument("--kpeaks", type=int, default=64)
