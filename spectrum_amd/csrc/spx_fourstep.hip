@@ -51,12 +51,13 @@ struct FsCfg {
   static constexpr int TF = 64;                // k_fs_finish workgroup: one wave (its launch bound)
   static constexpr int BSH = ilog2(N) - ilog2(TF);
   static constexpr uint64_t CAND_CAP = N / 2 + 2;
-  // where the column blocks are 16 wide (N2 = 256, 512) the column kernel
-  // reads the signal and the row kernels read Z with non-temporal loads, and
-  // the dense path stores the spectrum non-temporally: measured faster there,
-  // slower at N2 = 1024 and no faster at 128.  Z itself is stored plainly:
-  // the row kernels find part of it still in the Infinity Cache.
-  static constexpr bool STREAM_Z = (CB == 16);
+  // where the column blocks are at least 16 wide (N2 = 128 .. 512) the
+  // column kernel reads the signal and the row kernels read Z with
+  // non-temporal loads, and the dense path stores the spectrum
+  // non-temporally: measured faster there, slower at N2 = 1024 (8-column
+  // blocks).  Z itself is stored plainly: the row kernels find part of it
+  // still in the Infinity Cache.
+  static constexpr bool STREAM_Z = (CB >= 16);
   using LAY1 = LayInter<N2, CB>;
   using LAY2 = LayContig<N1, RB>;
   static_assert(CB >= 8 && NT1 >= 1 && RB >= 4 && NT2 >= 1, "tile shape");

@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
     for (int b = 0; b < PPT / R0; ++b) {
       const int j = tid + b * T;
       if (j < L / R0) {
-        load_run<R0>(a, ch, (uint32_t)j, (uint32_t)(L / R0), &v[b * R0]);
+        load_run<R0, MODE == 0>(a, ch, (uint32_t)j, (uint32_t)(L / R0), &v[b * R0]);
       }
     }
     fft_tile<LAY, L, 1, T, PPT>(lds, v, a.tw_a, tid);

@@ -15,7 +15,7 @@ using MaxValue = spx_max_value;
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-// STREAM: non-temporal complex64 loads (the split path's column kernel)
+// STREAM: non-temporal complex64 loads
 template <int DT, bool STREAM = false>
 __device__ __forceinline__ float2 ld_raw(const void* chan_base, uint32_t i) {
   if constexpr (DT == SPX_DTYPE_C64) {
@@ -33,14 +33,14 @@ __device__ __forceinline__ float2 ld_raw(const void* chan_base, uint32_t i) {
 // (PAD only).  Offsets are 32-bit from a per-channel scalar base, and indices
 // are clamped instead of branched on, so all R loads (and the R window loads)
 // are in flight together before the first use.
-template <int DT, int R, bool PAD>
+template <int DT, int R, bool PAD, bool STREAM = false>
 __device__ __forceinline__ void load_run_t(const FusedArgs& a, uint64_t ch, uint32_t n0, uint32_t stride, float2* v) {
   constexpr int ES = DT == SPX_DTYPE_C64 ? 8 : 4;
   const void* cb = reinterpret_cast<const char*>(a.in) + ch * (uint64_t)a.n_point * ES;
   const uint32_t last = a.n_point - 1;
   float2 raw[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) raw[r] = ld_raw<DT>(cb, PAD ? min(n0 + r * stride, last) : n0 + r * stride);
+  for (int r = 0; r < R; ++r) raw[r] = ld_raw<DT, STREAM>(cb, PAD ? min(n0 + r * stride, last) : n0 + r * stride);
   float wv[R];
   if (a.window != nullptr) {
 #pragma unroll
@@ -57,14 +57,16 @@ __device__ __forceinline__ void load_run_t(const FusedArgs& a, uint64_t ch, uint
   }
 }
 
-template <int R>
+// STREAM: non-temporal complex64 loads (the short kernels' dense outputs,
+// where they measured 4-5 % faster; the peak search measured no faster)
+template <int R, bool STREAM = false>
 __device__ __forceinline__ void load_run(const FusedArgs& a, uint64_t ch, uint32_t n0, uint32_t stride, float2* v) {
   if (a.n_point == a.nfft) {
-    if (a.dtype == SPX_DTYPE_C64) load_run_t<SPX_DTYPE_C64, R, false>(a, ch, n0, stride, v);
+    if (a.dtype == SPX_DTYPE_C64) load_run_t<SPX_DTYPE_C64, R, false, STREAM>(a, ch, n0, stride, v);
     else if (a.dtype == SPX_DTYPE_C32F16) load_run_t<SPX_DTYPE_C32F16, R, false>(a, ch, n0, stride, v);
     else load_run_t<SPX_DTYPE_C32BF16, R, false>(a, ch, n0, stride, v);
   } else {
-    if (a.dtype == SPX_DTYPE_C64) load_run_t<SPX_DTYPE_C64, R, true>(a, ch, n0, stride, v);
+    if (a.dtype == SPX_DTYPE_C64) load_run_t<SPX_DTYPE_C64, R, true, STREAM>(a, ch, n0, stride, v);
     else if (a.dtype == SPX_DTYPE_C32F16) load_run_t<SPX_DTYPE_C32F16, R, true>(a, ch, n0, stride, v);
     else load_run_t<SPX_DTYPE_C32BF16, R, true>(a, ch, n0, stride, v);
   }
