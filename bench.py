@@ -119,7 +119,7 @@ def cpu_baseline_mt(x, nfft, K, threads, budget_s=8.0):
                       f"then the peak rules on a {threads}-thread pool"}
 
 
-def _timer_ms(ctx, fn, reps=10, warm=3):
+def _timer_ms(ctx, fn, reps=30, warm=20):
     t = HipTimer(ctx.stream())
     for _ in range(warm):
         fn()
