@@ -355,14 +355,21 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
   if (bm >= lb) {  // some bin of this thread may clear the threshold
     Cand* cand = w.cand + (uint64_t)ci * C::CAND_CAP;
     uint32_t mhigh = 0, mlow = 0;
+    // only the bins at or above lb are tested (usually a few of the 16); a
+    // bin's own |X| and its neighbours come back from the tile
+    uint32_t pend = 0;
 #pragma unroll
-    for (int r = 0; r < PPT; ++r) {
+    for (int r = 0; r < PPT; ++r) pend |= (m[r] >= lb ? 1u : 0u) << r;
+    while (pend) {
+      const int r = __builtin_ctz(pend);
+      pend &= pend - 1;
       const int k1 = jj_row + 16 * r;
+      const float mc = ldsf[LAY2::idx(s_row, k1)];
       const float yl = ldsf[LAY2::idx(rl, k1)];
       const float yr = ldsf[LAY2::idx(rr, k1)];
       const bool ok = full_band || in_range(k2 + (uint32_t)N2 * (uint32_t)k1, a);
-      if (ok && (top || m[r] > yl) && (bot || m[r] > yr) && m[r] >= lb) {
-        if (m[r] >= bnd.hi) mhigh |= 1u << r;
+      if (ok && (top || mc > yl) && (bot || mc > yr)) {
+        if (mc >= bnd.hi) mhigh |= 1u << r;
         else mlow |= 1u << r;
       }
     }
