@@ -413,6 +413,35 @@ __device__ __forceinline__ V scan(V x, V id, F f) {
   x = f(x, mov<kBcast31, 0xc>(id, x));
   return x;
 }
+// The same six moves for a reduction read only in lane 63: the lanes a move
+// leaves unwritten (row_shr across a row start, rows outside the row mask)
+// keep whatever the destination held, so no identity has to be moved in
+// first.  Lane 63's chain reads only written lanes: position p of a row reads
+// p - k only for p >= k, row 3 takes lane 47 and lane 31, row 1 lane 15.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int movu(int x) {
+  return __builtin_amdgcn_mov_dpp(x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float movu(float x) {
+  return __builtin_bit_cast(float, movu<CTRL, ROWS>(__builtin_bit_cast(int, x)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double movu(double x) {
+  const int2 a = __builtin_bit_cast(int2, x);
+  return __builtin_bit_cast(double, make_int2(movu<CTRL, ROWS>(a.x), movu<CTRL, ROWS>(a.y)));
+}
+// reduction of x with f; the result is valid in lane 63 only
+template <class V, class F>
+__device__ __forceinline__ V reduce63(V x, F f) {
+  x = f(x, movu<kShr1, 0xf>(x));
+  x = f(x, movu<kShr2, 0xf>(x));
+  x = f(x, movu<kShr4, 0xf>(x));
+  x = f(x, movu<kShr8, 0xf>(x));
+  x = f(x, movu<kBcast15, 0xa>(x));
+  x = f(x, movu<kBcast31, 0xc>(x));
+  return x;
+}
 __device__ __forceinline__ double lane63(double x) {
   const int2 a = __builtin_bit_cast(int2, x);
   return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(a.x, 63), __builtin_amdgcn_readlane(a.y, 63)));
@@ -421,11 +450,11 @@ __device__ __forceinline__ double lane63(double x) {
 
 // sum over the wave, the same value in every lane
 __device__ __forceinline__ double wave_sum(double x) {
-  return dpp::lane63(dpp::scan(x, 0.0, [](double a, double b) { return a + b; }));
+  return dpp::lane63(dpp::reduce63(x, [](double a, double b) { return a + b; }));
 }
 // maximum over the wave (fmaxf: NaN lanes ignored), the same value in every lane
 __device__ __forceinline__ float wave_maxf(float x) {
-  x = dpp::scan(x, -__builtin_inff(), [](float a, float b) { return fmaxf(a, b); });
+  x = dpp::reduce63(x, [](float a, float b) { return fmaxf(a, b); });
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
 }
 // argmax with lowest index on ties; (m, k) pairs, the result in every lane
@@ -433,14 +462,13 @@ __device__ __forceinline__ void wave_argmax(float& m, uint32_t& k) {
   auto step = [&](float om, uint32_t ok) {
     if (om > m || (om == m && ok < k)) { m = om; k = ok; }
   };
-  const float ni = -__builtin_inff();
-  const int nk = -1;  // 0xffffffff
-  step(dpp::mov<dpp::kShr1, 0xf>(ni, m), (uint32_t)dpp::mov<dpp::kShr1, 0xf>(nk, (int)k));
-  step(dpp::mov<dpp::kShr2, 0xf>(ni, m), (uint32_t)dpp::mov<dpp::kShr2, 0xf>(nk, (int)k));
-  step(dpp::mov<dpp::kShr4, 0xf>(ni, m), (uint32_t)dpp::mov<dpp::kShr4, 0xf>(nk, (int)k));
-  step(dpp::mov<dpp::kShr8, 0xf>(ni, m), (uint32_t)dpp::mov<dpp::kShr8, 0xf>(nk, (int)k));
-  step(dpp::mov<dpp::kBcast15, 0xa>(ni, m), (uint32_t)dpp::mov<dpp::kBcast15, 0xa>(nk, (int)k));
-  step(dpp::mov<dpp::kBcast31, 0xc>(ni, m), (uint32_t)dpp::mov<dpp::kBcast31, 0xc>(nk, (int)k));
+  // only lane 63 is read back (see reduce63)
+  step(dpp::movu<dpp::kShr1, 0xf>(m), (uint32_t)dpp::movu<dpp::kShr1, 0xf>((int)k));
+  step(dpp::movu<dpp::kShr2, 0xf>(m), (uint32_t)dpp::movu<dpp::kShr2, 0xf>((int)k));
+  step(dpp::movu<dpp::kShr4, 0xf>(m), (uint32_t)dpp::movu<dpp::kShr4, 0xf>((int)k));
+  step(dpp::movu<dpp::kShr8, 0xf>(m), (uint32_t)dpp::movu<dpp::kShr8, 0xf>((int)k));
+  step(dpp::movu<dpp::kBcast15, 0xa>(m), (uint32_t)dpp::movu<dpp::kBcast15, 0xa>((int)k));
+  step(dpp::movu<dpp::kBcast31, 0xc>(m), (uint32_t)dpp::movu<dpp::kBcast31, 0xc>((int)k));
   m = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, m), 63));
   k = (uint32_t)__builtin_amdgcn_readlane((int)k, 63);
 }
