@@ -257,7 +257,7 @@ int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_varian
     p->scratch_bytes = ws;
     if ((e = hipMalloc(&p->d_scratch, p->scratch_bytes)) != hipSuccess) {
       release(p);
-      return fail(c, SPX_ENOMEM, "spx_batch: workspace allocation failed");
+      return oom(c, "spx_batch: workspace allocation failed");
     }
   }
   if (graph) {

@@ -25,8 +25,16 @@ int fail(spx_ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+// The runtime's last-error slot is cleared, so that the failure is reported
+// by this call only and not again by the next launch check.
 int hip_fail(spx_ctx* c, hipError_t e, const char* what) {
+  (void)hipGetLastError();
   return fail(c, SPX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int oom(spx_ctx* c, const char* what) {
+  (void)hipGetLastError();
+  return fail(c, SPX_ENOMEM, what);
 }
 
 bool is_pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
@@ -164,7 +172,7 @@ int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out) {
   void* d = nullptr;
   SPX_HIP(c, hipSetDevice(c->device));
   hipError_t e = hipMalloc(&d, host.size() * sizeof(float));
-  if (e != hipSuccess) return fail(c, SPX_ENOMEM, "table allocation failed");
+  if (e != hipSuccess) return oom(c, "table allocation failed");
   SPX_HIP(c, hipMemcpy(d, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice));
   c->tables[key] = d;
   *out = d;
@@ -183,7 +191,7 @@ int ensure_scratch(spx_ctx* c, uint64_t bytes) {
   hipError_t e = hipMalloc(&c->scratch, bytes);
   if (e != hipSuccess) {
     c->scratch = nullptr;
-    return fail(c, SPX_ENOMEM, "scratch allocation failed");
+    return oom(c, "scratch allocation failed");
   }
   c->scratch_bytes = bytes;
   return SPX_OK;
@@ -408,7 +416,10 @@ int spx_alloc(spx_ctx* c, uint64_t bytes, void** out) {
   if (!c || !out) return SPX_EINVAL;
   *out = nullptr;
   SPX_HIP(c, hipSetDevice(c->device));
-  if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return fail(c, SPX_ENOMEM, "hipMalloc failed");
+  if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+    *out = nullptr;
+    return oom(c, "hipMalloc failed");
+  }
   return SPX_OK;
 }
 
@@ -743,7 +754,10 @@ static int mlp_partial(spx_ctx* c, int groups) {
     c->mlp_partial = nullptr;
     c->mlp_partial_bytes = 0;
   }
-  if (hipMalloc(&c->mlp_partial, need) != hipSuccess) return fail(c, SPX_ENOMEM, "mlp partial buffer");
+  if (hipMalloc(&c->mlp_partial, need) != hipSuccess) {
+    c->mlp_partial = nullptr;
+    return oom(c, "mlp partial buffer");
+  }
   c->mlp_partial_bytes = need;
   return SPX_OK;
 }

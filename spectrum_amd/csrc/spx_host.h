@@ -36,6 +36,8 @@ enum TableKind { TW = 0, TW4 = 1, WIN = 2, MIX = 3 };
 
 int fail(spx_ctx* c, int code, const std::string& msg);
 int hip_fail(spx_ctx* c, hipError_t e, const char* what);
+// SPX_ENOMEM after a failed device or pinned allocation
+int oom(spx_ctx* c, const char* what);
 // Device table of kind TW (a = length, b = radix order), TW4 (a = length) or
 // WIN (a = window type, b = length) or MIX (a = compute type), generated in FP64
 // once per context.

@@ -140,7 +140,10 @@ int stage_for(spx_ctx* c, uint64_t in_bytes, uint64_t out_bytes, HostStage** out
   }
   HostStage* h = c->hstage;
   int rc;
+  // the capacity is zero while the slots are being replaced, so that a
+  // failed allocation leaves no slot that a later call would take as valid
   if (in_bytes > h->in_cap) {
+    h->in_cap = 0;
     for (int s = 0; s < kSlots; ++s) {
       if ((rc = grow(c, &h->pin_in[s], in_bytes, true))) return rc;
       if ((rc = grow(c, &h->dev_in[s], in_bytes, false))) return rc;
@@ -148,6 +151,7 @@ int stage_for(spx_ctx* c, uint64_t in_bytes, uint64_t out_bytes, HostStage** out
     h->in_cap = in_bytes;
   }
   if (out_bytes > h->out_cap) {
+    h->out_cap = 0;
     for (int s = 0; s < kSlots; ++s) {
       if ((rc = grow(c, &h->pin_out[s], out_bytes, true))) return rc;
       if ((rc = grow(c, &h->dev_out[s], out_bytes, false))) return rc;
@@ -174,8 +178,8 @@ int stream_batch(spx_ctx* c, uint32_t batch, const std::vector<HostArray>& array
     if (a.out) out_per += a.per_chan;
   }
   const uint64_t target = std::max<uint64_t>(c->host_chunk_bytes, 1);
-  uint32_t chunk = (uint32_t)std::max<uint64_t>(1, target / std::max<uint64_t>(std::max(in_per, out_per), 1));
-  chunk = std::min(chunk, batch);
+  const uint64_t fit = target / std::max<uint64_t>(std::max(in_per, out_per), 1);
+  const uint32_t chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(fit, 1), batch);
   // slot layout: each array's chunk at a 256-byte aligned offset
   std::vector<uint64_t> off(arrays.size());
   uint64_t in_bytes = 0, out_bytes = 0;
@@ -316,9 +320,8 @@ int spx_host_alloc(spx_ctx* c, uint64_t bytes, void** out) {
   SPX_HIP(c, hipSetDevice(c->device));
   const hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
-    (void)hipGetLastError();
     *out = nullptr;
-    return fail(c, SPX_ENOMEM, "spx_host_alloc: out of pinned host memory");
+    return oom(c, "spx_host_alloc: out of pinned host memory");
   }
   SPX_HIP(c, e);
   return SPX_OK;

@@ -30,7 +30,7 @@ def device_dense(ctx, x, nfft, window, kind):
 
 
 @pytest.mark.parametrize("B,n,nfft,chunk_bytes", [(37, 1000, 1024, 5 * 8192), (9, 65536, 65536, 2 * 524288),
-                                                 (5, 3000, 4096, 1 << 30)])
+                                                 (5, 3000, 4096, 1 << 30), (3, 3000, 4096, 1 << 62)])
 def test_host_dense_equals_device(ctx, B, n, nfft, chunk_bytes):
     print(f"[case] host dense B={B} n={n} nfft={nfft} chunk={chunk_bytes}", flush=True)
     rng = np.random.default_rng(B * 7 + n)
@@ -136,6 +136,19 @@ def test_host_errors_leave_context_usable(ctx):
         ctx.fft_c2c_host(x, 100)  # not a power of two
     got = ctx.fft_c2c_host(x, 128)
     assert got.shape == (2, 128) and abs(got[0, 0] - 100) < 1e-4
+
+
+def test_failed_allocation_leaves_context_usable(ctx):
+    """A device allocation larger than the GPU fails with SPX_ENOMEM (status
+    2); the runtime error it leaves is cleared, so the next launches succeed
+    with the same results as before."""
+    B, nfft, K = 4, 4096, 16
+    x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED00B0, B, nfft)
+    before = device_peaks(ctx, x, nfft, K, window=1, fs=1000.0)
+    with pytest.raises(capi.SpxError, match="status 2"):
+        ctx.alloc(1 << 52)
+    assert_same_records(device_peaks(ctx, x, nfft, K, window=1, fs=1000.0), before)
+    assert_same_records(ctx.fused_peaks_host(x, nfft, max_peaks=K, window=1, fs=1000.0), before)
 
 
 @pytest.mark.parametrize("nfft,dtype", [(4096, capi.DTYPE_C32F16), (65536, capi.DTYPE_C32BF16)])
