@@ -128,7 +128,9 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
       }
       s1 = fs1;
       s2 = fs2;
-      block_stats<T>(s1, s2, bm, bk, red);
+      // one channel per workgroup: `red` is written once per field, so
+      // neither reduction needs its closing barrier
+      block_stats<T, false>(s1, s2, bm, bk, red);
       const double cnt = (double)(a.s1 - a.s0);
       float mu, sd;
       const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
@@ -152,7 +154,7 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
         }
       }
       int total;
-      int off = block_excl_scan<T>(c, total, red);
+      int off = block_excl_scan<T, false>(c, total, red);
       if (owner && flags && off < (int)a.kcap) {
         MaxValue* out = a.peaks + (uint64_t)ch * a.kcap;
         int pos = off;
@@ -188,7 +190,6 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
         }
       }
     }
-    __syncthreads();
   }
 }
 

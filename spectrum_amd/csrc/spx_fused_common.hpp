@@ -196,7 +196,9 @@ __device__ __forceinline__ void block_stats(double& s1, double& s2, float& bm, u
 }
 
 // Block exclusive scan of per-thread counts; returns the offset, total in `total`.
-template <int T>
+// REUSE: end with a barrier so that `red.c` may be written again (not needed
+// when a later barrier comes first).
+template <int T, bool REUSE = true>
 __device__ __forceinline__ int block_excl_scan(int c, int& total, BlockRed& red) {
   constexpr int NW = (T + 63) / 64;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -209,7 +211,7 @@ __device__ __forceinline__ int block_excl_scan(int c, int& total, BlockRed& red)
     if (i < w) off += red.c[i];
     tot += red.c[i];
   }
-  __syncthreads();
+  if constexpr (REUSE) __syncthreads();
   total = tot;
   return off + inc - c;
 }

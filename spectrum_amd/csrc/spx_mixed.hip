@@ -155,7 +155,9 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
     }
   }
   double s1 = fs1, s2 = fs2;
-  block_stats<T>(s1, s2, bm, bk, red);
+  // the scan's first barrier follows every read of the sums, and the caller's
+  // barrier after the peaks every read of the scan totals
+  block_stats<T, false>(s1, s2, bm, bk, red);
   const double cnt = (double)(a.s1 - a.s0);
   float mu, sd;
   const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
@@ -176,7 +178,7 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
     yc = yr;
   }
   int total;
-  const int off = block_excl_scan<T>(cpk, total, red);
+  const int off = block_excl_scan<T, false>(cpk, total, red);
   if (flags && off < (int)a.kcap) {
     MaxValue* out = a.peaks + (uint64_t)ch * a.kcap;
     int pos = off;
