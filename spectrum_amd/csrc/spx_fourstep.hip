@@ -58,6 +58,15 @@ struct FsCfg {
   // blocks).  Z itself is stored plainly: the row kernels find part of it
   // still in the Infinity Cache.
   static constexpr bool STREAM_Z = (CB >= 16);
+  // channels per column workgroup, and the waves per SIMD the column kernel
+  // is compiled for.  N2 = 512: one channel, so that nothing stays live
+  // across channels and the kernel fits 128 registers; two of its 512-thread
+  // workgroups then share a CU (LDS 76 KiB each).  Measured at 512 x 131072:
+  // peak search 0.349 -> 0.334 ms, complex spectrum 0.424 -> 0.408 ms.  At
+  // N2 = 1024 the same form was slower (0.489 -> 0.510 ms), and capping the
+  // four-channel form at 128 registers spills (0.353 -> 0.396 ms at 2^17).
+  static constexpr int MC = (N2 == 512) ? 1 : 4;
+  static constexpr int COLS_WPE = (N2 == 512) ? 4 : 1;
   using LAY1 = LayInter<N2, CB>;
   using LAY2 = LayContig<N1, RB>;
   static_assert(CB >= 8 && NT1 >= 1 && RB >= 4 && NT2 >= 1, "tile shape");
@@ -136,7 +145,7 @@ __device__ __forceinline__ void fft256_cols16_halfx(float* lf, float2 (&v)[16], 
   Dft<16>::run(v);
 }
 
-// Column transforms of kFsColsChannels channels of the same column block
+// Column transforms of MC channels of the same column block
 // (CB columns of N2 rows) per workgroup: the window values and the four-step
 // twiddles W_N^(n1 k2) of the block are read once into registers and serve
 // every channel (they are the same for all channels, and re-reading them per
@@ -144,13 +153,11 @@ __device__ __forceinline__ void fft256_cols16_halfx(float* lf, float2 (&v)[16], 
 // itself).  The arithmetic of a channel does not depend on its neighbours.
 // N2 = 256 uses the half-width exchange fft256_cols16_halfx, other lengths
 // the generic column tile; both read their pass twiddles from LDS (twl).
-constexpr int kFsColsChannels = 4;
-
 template <int N2, int DT, int TC>
 __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& w, float2* tile, const float2* twl,
                                               double* red) {
   using C = FsCfg<N2, TC>;
-  constexpr int N1 = C::N1, CB = C::CB, NT1 = C::NT1, PPT = C::PPT, MC = kFsColsChannels;
+  constexpr int N1 = C::N1, CB = C::CB, NT1 = C::NT1, PPT = C::PPT, MC = C::MC;
   constexpr int RL1 = Plan<N2, 0>::RL;
   constexpr bool HALFX = (N2 == 256 && CB == 16);
   using LAY1 = typename C::LAY1;
@@ -221,7 +228,8 @@ __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& 
 
 // ---------------------------------------------------------------------------
 template <int N2, int TC>
-__global__ void __launch_bounds__(TC) k_fs_cols(FusedArgs a, FsWork w) {
+__global__ void __launch_bounds__(TC) __attribute__((amdgpu_waves_per_eu(FsCfg<N2, TC>::COLS_WPE)))
+k_fs_cols(FusedArgs a, FsWork w) {
   using C = FsCfg<N2, TC>;
   using LAY1 = typename C::LAY1;
   constexpr bool HALFX = (N2 == 256 && C::CB == 16);
@@ -618,7 +626,7 @@ static hipError_t launch_fs_dense(const FusedArgs& a, const FsWork& w0, uint32_t
     FsWork w = w0;
     w.ch0 = c0;
     w.nch = min(chunk, a.batch - c0);
-    const uint32_t groups = (w.nch + kFsColsChannels - 1) / kFsColsChannels;
+    const uint32_t groups = (w.nch + C::MC - 1) / C::MC;
     hipLaunchKernelGGL((k_fs_cols<N2, TC>), dim3(groups * C::NT1), dim3(TC), 0, st, a, w);
     hipLaunchKernelGGL((k_fs_rows_dense<N2, TC>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);
     hipError_t e = hipGetLastError();
@@ -654,7 +662,7 @@ static hipError_t launch_fs(const FusedArgs& a, const FsWork& w0, uint32_t chunk
     FsWork w = w0;
     w.ch0 = c0;
     w.nch = min(chunk, a.batch - c0);
-    const uint32_t groups = (w.nch + kFsColsChannels - 1) / kFsColsChannels;
+    const uint32_t groups = (w.nch + C::MC - 1) / C::MC;
     hipLaunchKernelGGL((k_fs_cols<N2, TC>), dim3(groups * C::NT1), dim3(TC), 0, st, a, w);
     hipLaunchKernelGGL((k_fs_rows<N2, TC>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);
     hipLaunchKernelGGL((k_fs_finish<N2, TC>), dim3(w.nch), dim3(C::TF), 0, st, a, w);
