@@ -62,7 +62,10 @@ typedef struct spx_ctx spx_ctx;
 
 /* ---- context ----------------------------------------------------------
  * Replaces the IBackend/GpuContext pair the reference facades are built on
- * (include/spectrum/fft_processor_rocm.hpp:59, spectrum_processor_rocm.hpp:98). */
+ * (include/spectrum/fft_processor_rocm.hpp:59, spectrum_processor_rocm.hpp:98).
+ * A context owns mutable state (workspace, tables, host staging, the last
+ * error text): one host thread at a time may call into it.  Threads that
+ * work at once take a context each; several contexts may share a device. */
 int spx_create(int device, spx_ctx** out);
 int spx_destroy(spx_ctx* ctx);
 /* borrow a caller stream (hipStream_t); NULL selects the context's own stream */
