@@ -73,6 +73,26 @@ def test_fused_threshold_sizes(ctx, nfft):
     check_channels(x, nfft, pk, ct, gm, st, K=K)
 
 
+@pytest.mark.parametrize("n_point,nfft", [(3000, 4096), (20000, 32768), (50000, 65536), (100000, 131072)])
+def test_fused_zero_padded(ctx, n_point, nfft):
+    """Ragged input (n_point < nfft): the Hann window spans the n_point
+    samples and the rest of the transform is zero padding, on the short path
+    and on the split path (2^17: one channel per column workgroup)."""
+    print(f"[case] fused zero padded n_point={n_point} nfft={nfft}", flush=True)
+    x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED00C0 + n_point, 4, n_point)
+    pk, ct, gm, st = run_fused(ctx, x, nfft, K=32)
+    check_channels(x, nfft, pk, ct, gm, st, K=32)
+
+
+def test_fused_empty_batch_is_a_no_op(ctx):
+    d, pk, ct = ctx.alloc(64), ctx.alloc(64), ctx.alloc(64)
+    ct.upload(np.full(16, 7, np.uint32))
+    for nfft in (4096, 65536):
+        ctx.fused_peaks(d, 0, nfft, nfft, pk, ct, max_peaks=2)
+    ctx.sync()
+    assert np.all(ct.download(np.uint32, (16,)) == 7)
+
+
 def test_config1_64x4096(ctx):
     """Plain FP32 path at the CPU-runnable shape: 64 x 4096, 3 tones, 10 dB, Hann, K=32."""
     print("[case] config1 64x4096", flush=True)
