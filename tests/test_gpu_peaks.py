@@ -84,6 +84,22 @@ def test_fused_zero_padded(ctx, n_point, nfft):
     check_channels(x, nfft, pk, ct, gm, st, K=32)
 
 
+@pytest.mark.parametrize("nfft", [4096, 65536, 131072])
+def test_fused_threshold_band(ctx, nfft):
+    """Threshold search over bins [s0, s1) only: statistics, local maxima and
+    the global maximum of the band (the split path's not-full-band branch)."""
+    s0, s1 = 100, nfft // 2 + 37
+    print(f"[case] fused threshold band nfft={nfft} s0={s0} s1={s1}", flush=True)
+    x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED00D0 + nfft, 6, nfft)
+    pk, ct, gm, st = run_fused(ctx, x, nfft, K=32, s0=s0, s1=s1)
+    check_channels(x, nfft, pk, ct, gm, st, K=32, s0=s0, s1=s1)
+    for b in range(x.shape[0]):
+        r = pk[b, : ct[b, 0]]
+        assert np.all((r["index"] >= s0) & (r["index"] < s1))
+        if ct[b, 1]:
+            assert s0 <= gm[b]["index"] < s1
+
+
 def test_fused_empty_batch_is_a_no_op(ctx):
     d, pk, ct = ctx.alloc(64), ctx.alloc(64), ctx.alloc(64)
     ct.upload(np.full(16, 7, np.uint32))
