@@ -63,7 +63,8 @@ struct FsCfg {
   // across channels and the kernel fits 128 registers; two of its 512-thread
   // workgroups then share a CU (LDS 76 KiB each).  Measured at 512 x 131072:
   // peak search 0.349 -> 0.334 ms, complex spectrum 0.424 -> 0.408 ms.  At
-  // N2 = 1024 the same form was slower (0.489 -> 0.510 ms), and capping the
+  // N2 = 1024 the same form was slower (0.489 -> 0.510 ms), at N2 = 128 too
+  // (2048 x 32768: 0.290 -> 0.305 ms), and capping the
   // four-channel form at 128 registers spills (0.353 -> 0.396 ms at 2^17).
   static constexpr int MC = (N2 == 512) ? 1 : 4;
   static constexpr int COLS_WPE = (N2 == 512) ? 4 : 1;
