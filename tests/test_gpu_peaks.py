@@ -100,6 +100,30 @@ def test_fused_threshold_band(ctx, nfft):
             assert s0 <= gm[b]["index"] < s1
 
 
+@pytest.mark.parametrize("nfft,dtype", [(4096, capi.DTYPE_C32F16), (65536, capi.DTYPE_C32BF16),
+                                        (131072, capi.DTYPE_C32F16)])
+def test_fused_half_storage_fp32_compute(ctx, nfft, dtype):
+    """16-bit complex storage with a per-channel load scale on the FP32 paths
+    (short kernel, split path): the oracle runs on exactly what the kernels
+    read, the decoded 16-bit values times the scale, so the same FP32 bar
+    applies as for complex64 input."""
+    B, K = 4, 32
+    print(f"[case] fused half storage nfft={nfft} dtype={dtype}", flush=True)
+    x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED00E0 + nfft, B, nfft)
+    u = capi.to_half_storage(x * 0.125, dtype)
+    scale = np.array([1.0, 0.5, 4.0, 2.0], np.float32)
+    d_in, d_sc = ctx.to_device(u), ctx.to_device(scale)
+    d_pk, d_ct, d_g, d_st = ctx.alloc(B * K * 32), ctx.alloc(B * 8), ctx.alloc(B * 32), ctx.alloc(B * 16)
+    ctx.fused_peaks(d_in, B, nfft, nfft, d_pk, d_ct, max_peaks=K, dtype=dtype, in_scale=d_sc, gmax=d_g, stats=d_st,
+                    window=1, fs=1.0)
+    ctx.sync()
+    pk, ct = d_pk.download(capi.MAXVALUE_DTYPE, (B, K)), d_ct.download(np.uint32, (B, 2))
+    gm, st = d_g.download(capi.MAXVALUE_DTYPE, (B,)), d_st.download(np.float32, (B, 4))
+    xr = (capi.from_half_storage(u, dtype) * scale[:, None]).astype(np.complex64)
+    check_channels(xr, nfft, pk, ct, gm, st, K=K)
+    assert np.all(ct[:, 1] >= 1)
+
+
 def test_fused_empty_batch_is_a_no_op(ctx):
     d, pk, ct = ctx.alloc(64), ctx.alloc(64), ctx.alloc(64)
     ct.upload(np.full(16, 7, np.uint32))
