@@ -160,7 +160,7 @@ def test_config2_full_size_subset(ctx):
         assert gm[b]["magnitude"] >= r["magnitude"].max() * (1 - 1e-6)
 
 
-@pytest.mark.parametrize("nfft", [1024, 65536])
+@pytest.mark.parametrize("nfft", [1024, 65536, 131072])
 def test_fused_cap_overflow_keeps_first_k(ctx, nfft):
     """nsigma=0 on noise gives far more peaks than K: the first K ascending are kept."""
     print(f"[case] cap overflow nfft={nfft}", flush=True)
@@ -186,7 +186,7 @@ def test_fused_degenerate_inputs(ctx, nfft):
     assert gm[1]["index"] == 0
 
 
-@pytest.mark.parametrize("nfft", [1024, 8192, 65536])
+@pytest.mark.parametrize("nfft", [1024, 8192, 65536, 131072])
 def test_all_maxima_from_signal(ctx, nfft):
     """AllMaxima rules on a raw signal (FindAllMaximaFromCPU): default range
     [1, N/2), cap min((s1-s0)/2, 1000), no refinement, uncapped count."""
