@@ -63,7 +63,7 @@ def check_channels(x, nfft, pk, ct, gm, st, *, K, nsigma=6.0, fs=1.0, window=1, 
     return n_exc
 
 
-@pytest.mark.parametrize("nfft", [256, 1024, 4096, 8192, 16384, 32768, 65536, 131072])
+@pytest.mark.parametrize("nfft", [256, 1024, 4096, 8192, 16384, 32768, 65536, 131072, 262144])
 def test_fused_threshold_sizes(ctx, nfft):
     B = max(4, min(64, (1 << 22) // nfft))
     K = 32
