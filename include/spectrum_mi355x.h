@@ -256,7 +256,7 @@ typedef struct spx_batch_desc {
   uint32_t dtype;             /* SPX_DTYPE_* */
   float sample_rate;
   float nsigma;
-  uint32_t max_peaks;         /* <= 2048 */
+  uint32_t max_peaks;         /* records kept per channel (>= 1) */
   uint32_t refine;            /* SPX_REFINE_NONE or SPX_REFINE_PARABOLIC */
   spx_max_value* peaks_out;   /* device [n_chan][max_peaks] */
   uint32_t* counts_out;       /* device [n_chan][2] = {stored, found} */

@@ -170,8 +170,10 @@ def all_maxima(X: np.ndarray, fs: float, s0: int, s1: int, cap: int):
 
 
 # ---------------------------------------------------------------- comparator
-def excused_bins(mags: np.ndarray, thr: float, tau: float, nfft: int, s0: int = 0, s1: int = 0, circular=True):
-    """Bins whose peak decision is within tau of a tie (local max, threshold or range)."""
+def excused_bins(mags: np.ndarray, thr: float, tau: float, nfft: int = 0, s0: int = 0, s1: int = 0, circular=True):
+    """Bins whose peak decision is within tau of a tie: a local-maximum test
+    (|m[k] - m[k-1]| or |m[k] - m[k+1]| <= tau, circular neighbours) or the
+    threshold test (|m[k] - thr| <= tau).  Range decisions are exact."""
     m = np.asarray(mags, dtype=np.float64)
     left = np.roll(m, 1)
     right = np.roll(m, -1)
@@ -181,29 +183,48 @@ def excused_bins(mags: np.ndarray, thr: float, tau: float, nfft: int, s0: int = 
     return close
 
 
-def compare_peak_lists(idx_gpu, idx_ref, mags, thr, tau, kcap, found_gpu=None, found_ref=None):
-    """Tie-tolerant comparison of ascending peak-index lists.
+def possible_survivors(mags: np.ndarray, thr: float, tau: float) -> np.ndarray:
+    """Bins that are a peak (strict local maximum above thr) under some
+    rounding of the spectrum within tau: m[k] + tau > m[k-1] - tau,
+    m[k] + tau > m[k+1] - tau and m[k] + tau > thr."""
+    m = np.asarray(mags, dtype=np.float64)
+    ok = (m + tau > np.roll(m, 1) - tau) & (m + tau > np.roll(m, -1) - tau)
+    if np.isfinite(thr):
+        ok &= m + tau > thr
+    return ok
 
-    Lists match when equal, or when every bin in the symmetric difference is
-    excused (its decisive comparison lies within tau of equality), or when the
-    difference is only past the cap and caused by excused bins earlier in the
-    list.  Returns (ok, detail)."""
+
+def compare_peak_lists(idx_gpu, idx_ref, mags, thr, tau, kcap, found_gpu=None, found_ref=None, s0=0, s1=0):
+    """Tie-tolerant comparison of two ascending peak-index lists, each the
+    first kcap peaks of its spectrum (a list shorter than kcap holds all).
+
+    A bin is excused when its decision is within tau of a tie (excused_bins).
+    Let L be the smaller last bin of the lists that are full (kcap entries);
+    up to L both lists are complete, so their non-excused bins must be equal
+    there.  Beyond L only the full list has entries and nothing is compared.
+    found_gpu / found_ref (uncapped counts, optional) may differ by at most
+    the number of excused bins in [s0, s1) (or the whole spectrum) that are
+    possible survivors.  Returns (ok, detail)."""
     a = [int(i) for i in idx_gpu]
     b = [int(i) for i in idx_ref]
+    m = np.asarray(mags, dtype=np.float64)
+    exc = excused_bins(m, thr, tau)
+    if found_gpu is not None and found_ref is not None and int(found_gpu) != int(found_ref):
+        lo, hi = (s0, s1) if s1 > s0 else (0, len(m))
+        slack = int(np.count_nonzero(exc[lo:hi] & possible_survivors(m, thr, tau)[lo:hi]))
+        if abs(int(found_gpu) - int(found_ref)) > slack:
+            return False, f"found {found_gpu} vs {found_ref}, more than the {slack} excused possible survivors"
     if a == b:
         return True, "equal"
-    exc = excused_bins(mags, thr, tau, len(mags))
-    sa, sb = set(a), set(b)
-    diff = sa.symmetric_difference(sb)
-    bad = [k for k in diff if not exc[k]]
-    if not bad:
-        return True, f"excused {sorted(diff)}"
-    # a decision flip before the cap shifts which bins make the first kcap
-    if len(a) == kcap and len(b) == kcap:
-        ua = [k for k in a if k not in sb]
-        ub = [k for k in b if k not in sa]
-        early = [k for k in (set(a) ^ set(b)) if exc[k]]
-        if early and all(k >= min(early) for k in bad):
-            return True, f"cap shift after excused {sorted(early)}"
-        del ua, ub
-    return False, f"unexcused {sorted(bad)[:10]} (gpu {len(a)}, ref {len(b)})"
+    if a != sorted(set(a)):
+        return False, "gpu list not strictly ascending"
+    la = a[-1] if len(a) >= kcap and kcap > 0 else np.inf
+    lb = b[-1] if len(b) >= kcap and kcap > 0 else np.inf
+    L = min(la, lb)
+    ca = [k for k in a if k <= L and not exc[k]]
+    cb = [k for k in b if k <= L and not exc[k]]
+    if ca != cb:
+        bad = sorted(set(ca) ^ set(cb))
+        return False, f"unexcused {bad[:10]} up to bin {L} (gpu {len(a)}, ref {len(b)}, kcap {kcap})"
+    diff = sorted(set(a) ^ set(b))
+    return True, f"excused {[k for k in diff if exc[k]][:10]}" + (f", tail past bin {L}" if np.isfinite(L) else "")

@@ -92,7 +92,7 @@ int check_desc(spx_ctx* c, const spx_batch_desc* d) {
   if (d->dtype > SPX_DTYPE_C32BF16) return fail(c, SPX_EINVAL, "spx_batch: bad dtype");
   if (d->window > SPX_WINDOW_BLACKMAN) return fail(c, SPX_EINVAL, "spx_batch: bad window");
   if (d->refine > SPX_REFINE_PARABOLIC) return fail(c, SPX_ENOTSUP, "spx_batch: refine must be NONE or PARABOLIC");
-  if (d->max_peaks == 0 || d->max_peaks > 2048) return fail(c, SPX_ECAP, "spx_batch: max_peaks must be 1..2048");
+  if (d->max_peaks == 0) return fail(c, SPX_EINVAL, "spx_batch: max_peaks must be >= 1");
   if (!(d->nsigma >= 0.0f)) return fail(c, SPX_EINVAL, "spx_batch: nsigma must be >= 0");
   for (uint32_t i = 0; i < d->n_chan; ++i) {
     const uint32_t L = d->lengths[i];

@@ -208,30 +208,14 @@ int check_fft_shape(spx_ctx* c, const void* in, const void* out, uint32_t batch,
   return SPX_OK;
 }
 
-// Fill table pointers and scratch for a fused launch; returns grid size in *grid.
+// Twiddle table of a short (one workgroup per channel) fused launch; one
+// workgroup per channel.
 int prepare(spx_ctx* c, FusedArgs& a, int* grid) {
+  const void* t;
   int rc;
-  if (spx_fused_is_large(a.nfft)) {
-    const uint32_t N2 = a.nfft / 256;
-    const void* t;
-    if ((rc = get_table(c, TW, 256, 0, &t))) return rc;
-    a.tw_a = (const float2*)t;
-    if ((rc = get_table(c, TW, N2, 0, &t))) return rc;
-    a.tw_b = (const float2*)t;
-    if ((rc = get_table(c, TW4, a.nfft, 0, &t))) return rc;
-    a.tw_4 = (const float2*)t;
-    int g = (c->cu_count > 0 ? c->cu_count : 256) * spx_large_groups_per_cu(a.nfft);
-    if ((uint32_t)g > a.batch) g = (int)a.batch;
-    a.slot_bytes = spx_large_slot_bytes(a.nfft);
-    if ((rc = ensure_scratch(c, a.slot_bytes * (uint64_t)g))) return rc;
-    a.scratch = c->scratch;
-    *grid = g;
-  } else {
-    const void* t;
-    if ((rc = get_table(c, TW, a.nfft, 0, &t))) return rc;
-    a.tw_a = (const float2*)t;
-    *grid = (int)a.batch;
-  }
+  if ((rc = get_table(c, TW, a.nfft, 0, &t))) return rc;
+  a.tw_a = (const float2*)t;
+  *grid = (int)a.batch;
   return SPX_OK;
 }
 
@@ -266,21 +250,12 @@ uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch) {
   return std::min(ch, batch);
 }
 
-// Long channels run on the split path (spx_fourstep.hip): three launches for
-// the threshold / local-maximum search, two for the dense outputs, faster
-// than the single persistent launch (k_fused_large) at every length 2^15 ..
-// 2^18 on MI355X.  SPX_LARGE_PATH=fused selects the single launch instead
-// (same bins and spectra to rounding; the tests cross-check them).
-bool use_fourstep(const FusedArgs& a) {
-  if (!spx_fused_is_large(a.nfft)) return false;
-  const char* env = std::getenv("SPX_LARGE_PATH");
-  return !(env && std::strcmp(env, "fused") == 0);
-}
-
 int run_fused(spx_ctx* c, FusedArgs& a) {
   if (a.batch == 0) return SPX_OK;
   int grid = 0, rc;
-  if (use_fourstep(a)) {
+  if (spx_fused_is_large(a.nfft)) {
+    // split four-step path (spx_fourstep.hip): three launches for the peak
+    // searches, two for the dense outputs
     const void* t;
     if ((rc = get_table(c, TW, 256, 0, &t))) return rc;
     a.tw_a = (const float2*)t;
@@ -346,6 +321,7 @@ int dense_fft(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_t n_
 }  // namespace spx_host
 
 using namespace spx_host;
+
 
 extern "C" {
 
@@ -536,8 +512,6 @@ int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a) {
     s1 = d->nfft;
   }
   if (s0 >= s1 || s1 > d->nfft) return fail(c, SPX_EINVAL, "empty or out-of-range search window");
-  if (spx_fused_is_large(d->nfft) && d->max_peaks > 2048)
-    return fail(c, SPX_ECAP, "max_peaks above 2048 is not supported for nfft >= 32768");
   a.in = d->input;
   a.in_scale = d->in_scale;
   a.batch = d->batch;

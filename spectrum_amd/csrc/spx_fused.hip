@@ -1,29 +1,16 @@
 // Fused window -> FFT -> |X| -> statistics -> local maxima -> threshold ->
 // ascending compaction -> sub-bin refinement, for gfx950.
 //
-// Two kernel families, both one workgroup per channel (no cross-workgroup
-// synchronisation, so no agent-scope L2 write-back/invalidate is ever needed):
+// One workgroup per channel (no cross-workgroup synchronisation):
 //
-//  * small  (N <= 16384): the whole channel lives in LDS.  The first Stockham
+//  * N <= 16384: the whole channel lives in LDS.  The first Stockham
 //    pass reads the input straight from HBM in butterfly order (window applied
 //    in registers), later passes go through LDS, the last pass stays in VGPRs.
 //    |X|, mean/std, local maxima, the block scan and the records are produced
 //    from the LDS copy of the spectrum; nothing but records leaves the chip.
 //
-//  * large  (N = 256 * N2, N >= 32768), the single-launch form: persistent
-//    grid, one workgroup per CU, channels strided over workgroups.  Phase 1
-//    runs N2-point column FFTs (16384/N2 columns per tile, interleaved LDS
-//    layout) and writes the twiddled intermediate Z[k2][n1] to a
-//    per-workgroup scratch slot; phase 2 runs 256-point row FFTs; local
-//    maxima of interior rows are decided from the LDS magnitude tile,
-//    tile-boundary rows from a small edge buffer.  Only local maxima that can
-//    clear the threshold are kept as candidates: by Parseval mean|X|^2 =
-//    sum|x_w|^2 =: P, and mean + k*std >= min(1,k)*sqrt(P), so candidates
-//    below that bound are dropped before the mean and std are known.  Phase 3
-//    filters candidates by the final threshold and selects the first K in
-//    ascending bin order (select_first_k).  It serves the dense outputs at
-//    these lengths; the peak search runs on the faster three-launch form in
-//    spx_fourstep.hip, which gives the same bins.
+// Longer channels (N = 256 * N2 >= 32768) run on the split four-step path
+// (spx_fourstep.hip).
 //
 // Peak rules (peak_mode 1, the fused north-star path; no reference kernel):
 //   m[k] = |X[k]| (FP32), range [s0,s1) (default the whole band, circular
@@ -193,27 +180,6 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
   }
 }
 
-// Launch shape of the fused large path per column length N2: workgroup size
-// (1024: one workgroup per CU; 512: two per CU, each with half the tile) and
-// whether the twiddle tables are copied to LDS.  Chosen from measurements on
-// MI355X (1024 x 65536, 2048 x 32768, 512 x 131072 channels).
-template <int N2> struct LargeShape { static constexpr int T = 1024; static constexpr bool TWL = true; };
-template <> struct LargeShape<128> { static constexpr int T = 512; static constexpr bool TWL = true; };
-template <> struct LargeShape<512> { static constexpr int T = 512; static constexpr bool TWL = false; };
-
-template <int N2, int MODE>
-__global__ void __launch_bounds__(LargeShape<N2>::T) __attribute__((amdgpu_waves_per_eu(4))) k_fused_large(FusedArgs a) {
-  constexpr int T = LargeShape<N2>::T;
-  constexpr bool TWL = LargeShape<N2>::TWL;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  LargeShared<N2, T, TWL>& sh = *reinterpret_cast<LargeShared<N2, T, TWL>*>(smem_raw);
-  unsigned char* slot = reinterpret_cast<unsigned char*>(a.scratch) + (uint64_t)blockIdx.x * a.slot_bytes;
-  for (uint32_t ch = blockIdx.x; ch < a.batch; ch += gridDim.x)
-    large_channel<N2, MODE, 0, T, TWL>(a, ch, ch, slot, sh);
-}
-
-constexpr uint64_t cmax(uint64_t x, uint64_t y) { return x > y ? x : y; }
-
 // ===========================================================================
 // dispatch
 // ===========================================================================
@@ -232,54 +198,11 @@ static hipError_t launch_small(const FusedArgs& a, int grid, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int N2, int MODE>
-static hipError_t launch_large_mode(const FusedArgs& a, int grid, hipStream_t st) {
-  constexpr int T = LargeShape<N2>::T;
-  const size_t smem = sizeof(LargeShared<N2, T, LargeShape<N2>::TWL>);
-  static bool attr_done = false;  // the attribute is per kernel function
-  if (!attr_done) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_fused_large<N2, MODE>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e != hipSuccess) return e;
-    attr_done = true;
-  }
-  hipLaunchKernelGGL((k_fused_large<N2, MODE>), dim3(grid), dim3(T), smem, st, a);
-  return hipGetLastError();
-}
-
-template <int N2>
-static hipError_t launch_large(const FusedArgs& a, int grid, hipStream_t st) {
-  return a.peak_mode != 0 ? launch_large_mode<N2, 1>(a, grid, st) : launch_large_mode<N2, 0>(a, grid, st);
-}
-
 }  // namespace spx
 
 extern "C" {
 
-uint64_t spx_large_slot_bytes(uint32_t nfft) {
-  using namespace spx;
-  switch (nfft) {
-    case 1u << 15: return cmax(LargeCfg<128, 1024>::SLOT_BYTES, LargeCfg<128, LargeShape<128>::T>::SLOT_BYTES);
-    case 1u << 16: return cmax(LargeCfg<256, 1024>::SLOT_BYTES, LargeCfg<256, LargeShape<256>::T>::SLOT_BYTES);
-    case 1u << 17: return cmax(LargeCfg<512, 1024>::SLOT_BYTES, LargeCfg<512, LargeShape<512>::T>::SLOT_BYTES);
-    case 1u << 18: return cmax(LargeCfg<1024, 1024>::SLOT_BYTES, LargeCfg<1024, LargeShape<1024>::T>::SLOT_BYTES);
-    default: return 0;
-  }
-}
-
 int spx_fused_is_large(uint32_t nfft) { return nfft >= (1u << 15) ? 1 : 0; }
-
-// persistent workgroups per CU of the fused large path for length nfft
-int spx_large_groups_per_cu(uint32_t nfft) {
-  using namespace spx;
-  switch (nfft) {
-    case 1u << 15: return 1024 / LargeShape<128>::T;
-    case 1u << 16: return 1024 / LargeShape<256>::T;
-    case 1u << 17: return 1024 / LargeShape<512>::T;
-    case 1u << 18: return 1024 / LargeShape<1024>::T;
-    default: return 1;
-  }
-}
 
 // concatenated per-pass twiddle table length (float2 entries) of plan (L, V)
 int spx_twiddle_table_len(uint32_t L, int V) {
@@ -310,10 +233,6 @@ hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st) {
     case 1u << 12: return launch_small<4096>(*a, grid, st);
     case 1u << 13: return launch_small<8192>(*a, grid, st);
     case 1u << 14: return launch_small<16384>(*a, grid, st);
-    case 1u << 15: return launch_large<128>(*a, grid, st);
-    case 1u << 16: return launch_large<256>(*a, grid, st);
-    case 1u << 17: return launch_large<512>(*a, grid, st);
-    case 1u << 18: return launch_large<1024>(*a, grid, st);
     default: return hipErrorInvalidValue;
   }
 }

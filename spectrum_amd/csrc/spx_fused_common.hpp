@@ -226,98 +226,53 @@ struct Cand {
   float re, im, ml, mc, mr;
 };
 
-__device__ __forceinline__ void push_cand(Cand* cand, uint32_t* ncand, uint32_t cap, uint32_t k, float2 x, float yl,
-                                          float yc, float yr) {
-  const uint32_t p = atomicAdd(ncand, 1u);
-  if (p < cap) {
-    Cand cd;
-    cd.k = k; cd.re = x.x; cd.im = x.y; cd.ml = yl; cd.mc = yc; cd.mr = yr;
-    gp(cand)[p] = cd;
-  }
-}
-
-// Selection of the first K threshold survivors in ascending bin order from a
-// candidate list (local maxima that may clear the threshold), plus the counts
-// and stats outputs of channel och.  One bucket per thread (T buckets of
-// 2^BSH bins) is histogrammed and scanned to find the bucket holding the K-th
-// survivor; survivors up to that bucket are gathered into gat (GMAX entries of
-// LDS) and bitonic-sorted by bin.  hist: T ints of LDS.  cand(i) returns
-// candidate i (a candidate with mc = NaN is no survivor).  Every thread of the
-// T-thread workgroup calls it.
-// candidate i of a list in global memory, as stored
-struct CandGlobal {
-  const Cand* c;
-  __device__ __forceinline__ Cand operator()(uint32_t i) const { return gp(c)[i]; }
-};
-
-template <int T, int BSH, int GMAX, class SRC>
+// Selection of the first K threshold survivors in ascending bin order from an
+// unsorted candidate list (local maxima that may clear the threshold), plus the
+// counts and stats outputs of channel och.  Survivor bins are marked in a
+// bitmap of NW = N/32 words in LDS; a survivor's rank is the number of marked
+// bins below it, from the popcount prefix of the bitmap words, so the records
+// land in order with no sort and for any K.  Thread t owns words
+// [t*WPT, (t+1)*WPT): base[t] holds the survivors below its first word and
+// pre[w] the survivors in its earlier words (at most 32*WPT, 16 bits).
+// cand(i) returns candidate i (mc = NaN: no survivor).  Every thread of the
+// T-thread workgroup calls it; bits, pre, base are LDS.
+template <int T, int NW, class SRC>
 __device__ __forceinline__ void select_first_k(const FusedArgs& a, SRC cand, uint32_t ncand, float thr, float mu,
-                                               float sd, double P, uint64_t och, int* hist, uint2* gat, int& s_bstar,
-                                               uint32_t& s_ngather, BlockRed& red) {
-  const int tid0 = threadIdx.x;
-  for (int i = opaque_v(tid0); i < T; i += T) hist[i] = 0;
+                                               float sd, double P, uint64_t och, uint32_t* bits, uint16_t* pre,
+                                               int* base, BlockRed& red) {
+  constexpr int WPT = NW / T;
+  static_assert(NW % T == 0 && 32 * WPT < 65536, "bitmap words per thread");
+  const int tid = threadIdx.x;
+  for (int i = opaque_v(tid); i < NW; i += T) bits[i] = 0u;
   __syncthreads();
-  for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
-    Cand cd[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) cd[q] = cand(min(i0 + q * T, ncand - 1));
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (i0 + q * T < ncand && cd[q].mc > thr) atomicAdd(&hist[cd[q].k >> BSH], 1);
+  for (uint32_t i = opaque_v(tid); i < ncand; i += T) {
+    const Cand q = cand(i);
+    if (q.mc > thr) atomicOr(&bits[q.k >> 5], 1u << (q.k & 31));
   }
   __syncthreads();
-  // exclusive scan over buckets (one bucket per thread)
-  int hv = hist[tid0];
+  int run = 0;
+  for (int w = 0; w < WPT; ++w) {
+    pre[tid * WPT + w] = (uint16_t)run;
+    run += __popc(bits[tid * WPT + w]);
+  }
   int total;
-  int excl = block_excl_scan<T>(hv, total, red);
-  const int kcap = (int)a.kcap;
-  if (tid0 == 0) { s_bstar = T - 1; s_ngather = 0; }
+  base[tid] = block_excl_scan<T>(run, total, red);
   __syncthreads();
-  if (total > kcap && excl < kcap && excl + hv >= kcap) s_bstar = tid0;
-  __syncthreads();
-  const int bstar = s_bstar;
-  // gather (key = bin, value = candidate index) of survivors in buckets <= bstar
-  for (uint32_t i0 = opaque_v(tid0); i0 < ncand; i0 += 4 * T) {
-    Cand cd[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) cd[q] = cand(min(i0 + q * T, ncand - 1));
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t i = i0 + q * T;
-      if (i < ncand && cd[q].mc > thr && (int)(cd[q].k >> BSH) <= bstar) {
-        const uint32_t p = atomicAdd(&s_ngather, 1u);
-        if (p < (uint32_t)GMAX) gat[p] = make_uint2(cd[q].k, i);
+  const uint32_t kcap = a.kcap;
+  MaxValue* out = a.peaks + och * kcap;
+  if (kcap > 0 && total > 0) {
+    for (uint32_t i = opaque_v(tid); i < ncand; i += T) {
+      const Cand q = cand(i);
+      if (q.mc > thr) {
+        const uint32_t w = q.k >> 5;
+        const uint32_t rank = (uint32_t)base[w / WPT] + pre[w] + __popc(bits[w] & ((1u << (q.k & 31)) - 1u));
+        if (rank < kcap) write_record(out + rank, q.k, make_float2(q.re, q.im), q.ml, q.mc, q.mr, a);
       }
     }
   }
-  __syncthreads();
-  const int ng = min((int)s_ngather, GMAX);
-  int np2 = 1;
-  while (np2 < ng) np2 <<= 1;
-  for (int i = ng + opaque_v(tid0); i < np2; i += T) gat[i] = make_uint2(0xffffffffu, 0u);
-  __syncthreads();
-  // bitonic sort by key
-  for (int size = 2; size <= np2; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = opaque_v(tid0); i < np2 / 2; i += T) {
-        const int lo = 2 * i - (i & (stride - 1));
-        const int hi = lo + stride;
-        const bool up = ((lo & size) == 0);
-        uint2 x = gat[lo], y = gat[hi];
-        if ((x.x > y.x) == up) { gat[lo] = y; gat[hi] = x; }
-      }
-      __syncthreads();
-    }
-  }
-  const int nout = min(total, kcap);
-  MaxValue* out = a.peaks + (uint64_t)och * a.kcap;
-  for (int i = opaque_v(tid0); i < nout && i < ng; i += T) {
-    const Cand cd = cand(gat[i].y);
-    write_record(out + i, cd.k, make_float2(cd.re, cd.im), cd.ml, cd.mc, cd.mr, a);
-  }
-  if (tid0 == 0) {
+  if (tid == 0) {
     if (a.counts) {
-      gp(a.counts)[2 * och + 0] = (uint32_t)nout;
+      gp(a.counts)[2 * och + 0] = min((uint32_t)total, kcap);
       gp(a.counts)[2 * och + 1] = (uint32_t)total;
     }
     if (a.stats) {
@@ -327,318 +282,6 @@ __device__ __forceinline__ void select_first_k(const FusedArgs& a, SRC cand, uin
       gp(a.stats)[4 * och + 3] = (float)P;
     }
   }
-}
-
-template <int N2_, int T_ = 1024>
-struct LargeCfg {
-  static constexpr int N1 = 256;                // phase-2 row length (radix 16 x 16)
-  static constexpr int N2 = N2_;                // phase-1 column length
-  static constexpr int N = N1 * N2;
-  static constexpr int T = T_;                  // workgroup size (1024, or 512 for two workgroups per CU)
-  static constexpr int PPT = 16;
-  static constexpr int TILE = T * PPT;          // points per tile
-  static constexpr int CB = TILE / N2;          // phase-1 columns per tile
-  static constexpr int RB = TILE / N1;          // phase-2 rows per tile
-  static constexpr int NT1 = N1 / CB;           // phase-1 tiles
-  static constexpr int NT2 = N2 / RB;           // phase-2 tiles
-  using LAY1 = LayInter<N2, CB>;
-  using LAY2 = LayContig<N1, RB>;
-  static constexpr int LDS = LAY2::SIZE > LAY1::SIZE ? LAY2::SIZE : LAY1::SIZE;
-  // scratch slot layout (bytes)
-  static constexpr uint64_t Z_BYTES = (uint64_t)N * 8;
-  static constexpr uint64_t CAND_CAP = N / 2 + 2;
-  static constexpr uint64_t CAND_BYTES = (CAND_CAP * sizeof(Cand) + 255) / 256 * 256;
-  static constexpr uint64_t EDGE_M_BYTES = (uint64_t)NT2 * 4 * N1 * 4;
-  static constexpr uint64_t EDGE_X_BYTES = (uint64_t)NT2 * 2 * N1 * 8;
-  static constexpr uint64_t SLOT_BYTES = Z_BYTES + CAND_BYTES + EDGE_M_BYTES + EDGE_X_BYTES;
-  static constexpr int NBK = T;                 // selection buckets, one per thread
-  static constexpr int BSH = ilog2(N) - ilog2(T);  // bucket = k >> BSH
-  static_assert(NT1 >= 1 && NT2 >= 1 && CB >= 8 && RB >= 4, "tile shape");
-  static_assert(Plan<N1>::RL == 16, "rows end on a radix-16 pass");
-};
-
-// TWL: the per-pass twiddle tables of the 256-point rows and of the column
-// length are copied per channel into LDS (padded rows) instead of being read
-// from global memory.
-constexpr int kTwRows = Plan<256>::TWSIZE_LDS;
-template <int N2>
-constexpr int tw_cols_lds() {
-  return Plan<N2, 0>::TWSIZE_LDS > Plan<N2, 1>::TWSIZE_LDS ? Plan<N2, 0>::TWSIZE_LDS : Plan<N2, 1>::TWSIZE_LDS;
-}
-
-template <int N2, int T = 1024, bool TWL = true>
-struct LargeShared {
-  using C = LargeCfg<N2, T>;
-  float2 tile[C::LDS];
-  float2 twa[TWL ? kTwRows : 1];
-  float2 twb[TWL ? tw_cols_lds<N2>() : 1];
-  BlockRed red;
-  int hist[C::NBK];
-  uint32_t ncand;
-  uint32_t ngather;
-  int bstar;
-};
-
-// One channel of length N = 256*N2 through the three phases, run by a whole
-// workgroup of TW threads.  ich indexes the input (a.in + ich*n_point), och the
-// outputs; slot is this workgroup's scratch slot.  V picks the radix order of
-// the N2-point column transform (Plan<N2, V>; tw_b must hold its tables).
-// MODE 0: dense outputs only (spectrum / |X| / |X|^2 / mag+phase); MODE 1: peaks only.
-template <int N2, int MODE, int V, int TW = 1024, bool TWL = true>
-__device__ __forceinline__ void large_channel(const FusedArgs& a, uint64_t ich, uint64_t och, unsigned char* slot,
-                                              LargeShared<N2, TW, TWL>& sh) {
-  constexpr bool kLargeTwLds = TWL;
-  using C = LargeCfg<N2, TW>;
-  constexpr int N = C::N, N1 = C::N1, T = C::T, PPT = C::PPT, CB = C::CB, RB = C::RB;
-  constexpr int NT1 = C::NT1, NT2 = C::NT2;
-  constexpr int R01 = Plan<N2, V>::R0, RL1 = Plan<N2, V>::RL;
-  using LAY1 = typename C::LAY1;
-  using LAY2 = typename C::LAY2;
-
-  float2* lds = sh.tile;
-  float* ldsf = reinterpret_cast<float*>(sh.tile);
-  const int tid0 = threadIdx.x;
-
-  float2* Z = reinterpret_cast<float2*>(slot);
-  Cand* cand = reinterpret_cast<Cand*>(slot + C::Z_BYTES);
-  float* edgeM = reinterpret_cast<float*>(slot + C::Z_BYTES + C::CAND_BYTES);
-  float2* edgeX = reinterpret_cast<float2*>(slot + C::Z_BYTES + C::CAND_BYTES + C::EDGE_M_BYTES);
-
-  constexpr bool peaks = (MODE == 1);
-  const bool full_band = (a.s0 == 0 && a.s1 == (uint32_t)N);
-
-  const float2* twb_src = a.tw_b;  // column tables: LDS copy or global
-  const float2* twa_src = a.tw_a;  // row tables
-  if constexpr (kLargeTwLds) {
-    static_assert(Plan<N2, V>::TWSIZE_LDS <= tw_cols_lds<N2>() && Plan<256>::TWSIZE_LDS <= kTwRows, "LDS tables");
-    __syncthreads();  // the previous channel may still read them
-    copy_tw_to_lds<N2, V, T>(a.tw_b, sh.twb, opaque_v(tid0));
-    copy_tw_to_lds<256, 0, T>(a.tw_a, sh.twa, opaque_v(tid0));
-    __syncthreads();
-    twb_src = sh.twb;
-    twa_src = sh.twa;
-  }
-  {
-    int tid = opaque_v(tid0);
-    FusedArgs ai = a;
-    // ------------------------------------------------------------------
-    // phase 1
-    // ------------------------------------------------------------------
-    float energy = 0.f;
-#pragma unroll 1
-    for (int t = 0; t < NT1; ++t) {
-      tid = opaque_v(tid0);
-      ai.tw_b = opaque(a.tw_b);
-      ai.tw_4 = opaque(a.tw_4);
-      ai.window = opaque(a.window);
-      float2 v[PPT];
-      if constexpr (R01 == 16) {
-        const int c = tid % CB, jj = tid / CB;  // jj in [0, N2/16)
-        const int n1 = t * CB + c;
-        load_run<16>(ai, ich, (uint32_t)(n1 + N1 * jj), (uint32_t)(N1 * (N2 / 16)), v);
-      } else {
-#pragma unroll
-        for (int b = 0; b < PPT / R01; ++b) {
-          const int j = tid + b * T;
-          const int c = j % CB, jj = j / CB;
-          const int n1 = t * CB + c;
-          load_run<R01>(ai, ich, (uint32_t)(n1 + N1 * jj), (uint32_t)(N1 * (N2 / R01)), &v[b * R01]);
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < 16; ++r) energy = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, energy));
-      if constexpr (kLargeTwLds) fft_tile<LAY1, N2, CB, T, PPT, V, true>(lds, v, twb_src, tid);
-      else fft_tile<LAY1, N2, CB, T, PPT, V>(lds, v, ai.tw_b, tid);
-      // v[b*RL1 + r] = Y[column s][k2 = jj' + r*N2/RL1] for j = tid + b*T
-#pragma unroll
-      for (int b = 0; b < PPT / RL1; ++b) {
-        int s, jb;
-        LAY1::template map<RL1>(tid + b * T, s, jb);
-        const uint32_t col = (uint32_t)(t * CB + s);
-#pragma unroll
-        for (int r = 0; r < RL1; ++r) {
-          const uint32_t e = (uint32_t)(jb + r * (N2 / RL1)) * N1 + col;
-          gp(Z)[e] = c_mul(v[b * RL1 + r], gp(ai.tw_4)[e]);
-        }
-      }
-    }
-    __syncthreads();
-
-    // block reduce the input energy -> P = mean |X|^2 (Parseval)
-    double P;
-    {
-      double e = energy, dummy = 0.0;
-      float bmx = -1.f;
-      uint32_t bkx = 0;
-      block_stats<T>(e, dummy, bmx, bkx, sh.red);
-      P = e;  // sum_n |x_w[n]|^2 = (1/N) sum_k |X[k]|^2
-    }
-    float lb = 0.f;
-    if (a.peak_mode == 1 && full_band && a.nsigma >= 0.f) lb = (float)(fmin(1.0, (double)a.nsigma) * sqrt(P) * 0.999);
-    if (tid == 0) sh.ncand = 0;
-
-    // ------------------------------------------------------------------
-    // phase 2
-    // ------------------------------------------------------------------
-    float fs1 = 0.f, fs2 = 0.f;
-    float bm = -1.f;
-    uint32_t bk = 0xffffffffu;
-    float2 bx = make_float2(0.f, 0.f);
-    float bl = 0.f, br = 0.f;
-
-#pragma unroll 1
-    for (int t = 0; t < NT2; ++t) {
-      tid = opaque_v(tid0);
-      ai.tw_a = opaque(a.tw_a);
-      const int s_row = tid / (N1 / 16), jj_row = tid % (N1 / 16);  // radix-16 mapping of both passes
-      float2 v[PPT];
-      {
-        const float2* zr = Z + (uint32_t)((t * RB + s_row) * N1 + jj_row);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) v[r] = gp(zr)[r * (N1 / 16)];
-      }
-      if constexpr (kLargeTwLds) fft_tile<LAY2, N1, RB, T, PPT, 0, true>(lds, v, twa_src, tid);
-      else fft_tile<LAY2, N1, RB, T, PPT>(lds, v, ai.tw_a, tid);
-      // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]  ->  bin k2 + N2*k1
-      const int k2 = t * RB + s_row;
-      float m[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        m[r] = mag_of(v[r]);
-        if (full_band || in_range((uint32_t)k2 + (uint32_t)N2 * (uint32_t)(jj_row + 16 * r), a)) {
-          fs1 += m[r];
-          fs2 = fmaf(m[r], m[r], fs2);
-        }
-      }
-      __syncthreads();  // LDS free again
-      if constexpr (!peaks) {
-        // transpose through LDS so that natural-order runs of RB bins are written
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int e = (jj_row + 16 * r) * RB + s_row;
-          lds[e + (e >> 4)] = v[r];
-        }
-        __syncthreads();
-        const uint64_t cbase = (uint64_t)och * N;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int e = tid + i * T;
-          const int k1 = e / RB, s = e % RB;
-          store_dense(a, cbase + (uint64_t)(t * RB + s) + (uint64_t)N2 * k1, lds[e + (e >> 4)]);
-        }
-      } else {
-        // magnitude tile [s][k1] and the edge rows 0, 1, RB-2, RB-1
-#pragma unroll
-        for (int r = 0; r < 16; ++r) ldsf[s_row * N1 + jj_row + 16 * r] = m[r];
-        {
-          int slotm = -1, slotx = -1;
-          if (s_row == 0) { slotm = 0; slotx = 0; }
-          else if (s_row == 1) slotm = 1;
-          else if (s_row == RB - 2) slotm = 2;
-          else if (s_row == RB - 1) { slotm = 3; slotx = 1; }
-          if (slotm >= 0) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-              const int k1 = jj_row + 16 * r;
-              gp(edgeM)[(t * 4 + slotm) * N1 + k1] = m[r];
-              if (slotx >= 0) gp(edgeX)[(t * 2 + slotx) * N1 + k1] = v[r];
-            }
-          }
-        }
-        __syncthreads();
-        if (s_row >= 1 && s_row <= RB - 2) {
-          // best-bin tracking with selects; local maxima that can clear the
-          // threshold are rare and handled one by one afterwards
-          uint32_t mask = 0;
-#pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const int k1 = jj_row + 16 * r;
-            const uint32_t k = (uint32_t)k2 + (uint32_t)N2 * (uint32_t)k1;
-            const float yl = ldsf[(s_row - 1) * N1 + k1];
-            const float yr = ldsf[(s_row + 1) * N1 + k1];
-            const bool ok = full_band || in_range(k, a);
-            const bool upd = ok && (m[r] > bm || (m[r] == bm && k < bk));
-            bm = upd ? m[r] : bm;
-            bk = upd ? k : bk;
-            bx.x = upd ? v[r].x : bx.x;
-            bx.y = upd ? v[r].y : bx.y;
-            bl = upd ? yl : bl;
-            br = upd ? yr : br;
-            if (ok && m[r] > yl && m[r] > yr && m[r] >= lb) mask |= 1u << r;
-          }
-          while (mask) {
-            const int r = __builtin_ctz(mask);
-            mask &= mask - 1;
-            const int k1 = jj_row + 16 * r;
-            float2 x = v[0];
-            float mc = m[0];
-#pragma unroll
-            for (int q = 1; q < 16; ++q) {
-              x = (q == r) ? v[q] : x;
-              mc = (q == r) ? m[q] : mc;
-            }
-            push_cand(cand, &sh.ncand, (uint32_t)C::CAND_CAP, (uint32_t)k2 + (uint32_t)N2 * (uint32_t)k1, x,
-                      ldsf[(s_row - 1) * N1 + k1], mc, ldsf[(s_row + 1) * N1 + k1]);
-          }
-        }
-        __syncthreads();
-      }
-    }
-
-    if constexpr (peaks) {
-      // phase 2b: tile-boundary rows from the edge buffers
-      constexpr int NBND = 2 * NT2 * N1;
-      for (int i = opaque_v(tid0); i < NBND; i += T) {
-        const int k1 = i % N1;
-        const int q = i / N1;            // 0 .. 2*NT2-1
-        const int t = q >> 1;
-        const bool last = (q & 1) != 0;  // row RB-1 of tile t, else row 0
-        float yl, yc, yr;
-        float2 x;
-        uint32_t k2;
-        if (!last) {
-          k2 = (uint32_t)(t * RB);
-          yc = gp(edgeM)[(t * 4 + 0) * N1 + k1];
-          x = gp(edgeX)[(t * 2 + 0) * N1 + k1];
-          yr = gp(edgeM)[(t * 4 + 1) * N1 + k1];
-          if (t > 0) yl = gp(edgeM)[((t - 1) * 4 + 3) * N1 + k1];
-          else yl = gp(edgeM)[((NT2 - 1) * 4 + 3) * N1 + (k1 + N1 - 1) % N1];  // k-1 wraps to row N2-1, column k1-1
-        } else {
-          k2 = (uint32_t)(t * RB + RB - 1);
-          yc = gp(edgeM)[(t * 4 + 3) * N1 + k1];
-          x = gp(edgeX)[(t * 2 + 1) * N1 + k1];
-          yl = gp(edgeM)[(t * 4 + 2) * N1 + k1];
-          if (t < NT2 - 1) yr = gp(edgeM)[((t + 1) * 4 + 0) * N1 + k1];
-          else yr = gp(edgeM)[(0 * 4 + 0) * N1 + (k1 + 1) % N1];  // k+1 wraps to row 0, column k1+1
-        }
-        const uint32_t k = k2 + (uint32_t)N2 * (uint32_t)k1;
-        if (full_band || in_range(k, a)) {
-          if (yc > bm || (yc == bm && k < bk)) { bm = yc; bk = k; bx = x; bl = yl; br = yr; }
-          if (yc > yl && yc > yr && yc >= lb) push_cand(cand, &sh.ncand, (uint32_t)C::CAND_CAP, k, x, yl, yc, yr);
-        }
-      }
-      __syncthreads();
-
-      // ------------------------------------------------------------------
-      // phase 3: statistics, threshold, selection of the first K survivors
-      // ------------------------------------------------------------------
-      double s1 = fs1, s2 = fs2;
-      float gm = bm;
-      uint32_t gk = bk;
-      block_stats<T>(s1, s2, gm, gk, sh.red);
-      const double cnt = (double)(a.s1 - a.s0);
-      float mu, sd;
-      const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
-      if (a.gmax && gk != 0xffffffffu && bk == gk && bm == gm) write_record(a.gmax + och, bk, bx, bl, bm, br, a);
-
-      const uint32_t ncand = min(sh.ncand, (uint32_t)C::CAND_CAP);
-      select_first_k<T, C::BSH, (C::LDS > 4096 ? 4096 : C::LDS)>(a, CandGlobal{cand}, ncand, thr, mu, sd, P, och,
-                                                                sh.hist,
-                                                                reinterpret_cast<uint2*>(lds), sh.bstar,
-                                                                sh.ngather, sh.red);
-    }
-    __syncthreads();
-    }
 }
 
 }  // namespace spx

@@ -35,8 +35,6 @@ struct FusedArgs {
   const float2* tw_a;      // per-pass twiddle tables: small path length N, large path length 256
   const float2* tw_b;      // large path: per-pass tables of the column length N2 = N/256
   const float2* tw_4;      // large path: W_N^(n1*k2) at [k2*256 + n1]
-  void* scratch;           // large: per-workgroup slots
-  uint64_t slot_bytes;
   const void* mlp;         // learned refiner weights (refine == 2)
   void* mlp_feat;          // refine == 2 on the short paths: [batch][kcap][16] fp16 features of the stored peaks
   // window as w[n] = c0 - c1 cos(2 pi n/(n_point-1)) + c2 cos(4 pi n/(n_point-1))
@@ -92,9 +90,7 @@ hipError_t spx_launch_mlp_refine(const void* dep, const void* feat, void* peaks,
 hipError_t spx_launch_mixed(const FusedArgs* a, int compute, int grid, hipStream_t st);
 hipError_t spx_launch_batch(const BatchArgs* b, int grid, hipStream_t st);
 hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st);
-uint64_t spx_large_slot_bytes(uint32_t nfft);
 int spx_fused_is_large(uint32_t nfft);
-int spx_large_groups_per_cu(uint32_t nfft);
 uint64_t spx_fs_workspace_bytes(uint32_t nfft, uint32_t chunk);
 hipError_t spx_launch_fourstep(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st);
 hipError_t spx_launch_fourstep_ex(const FusedArgs* a, void* ws, uint32_t chunk, const uint32_t* chan,

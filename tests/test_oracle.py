@@ -269,3 +269,42 @@ def test_tie_tolerant_comparator():
     assert ok, why  # bin 4 vs 3 is a near tie
     ok, _ = orc.compare_peak_lists([6], [1, 6], m, 1.0, 1e-3, 10)
     assert not ok  # bin 1 is a clear peak
+
+
+def _capped_spectrum():
+    # maxima at 10, 20, 30, 33, 40, 43, 50; bin 20 is a near tie with bin 21
+    m = np.zeros(64)
+    for k, v in ((10, 5.0), (20, 4.0), (30, 6.0), (33, 6.0), (40, 7.0), (43, 7.0), (50, 8.0)):
+        m[k] = v
+    m[21] = 4.0 - 1e-6
+    return m
+
+
+def test_comparator_cap_shift_needs_matching_prefix():
+    """At the cap an excused flip may shift the tail, but every non-excused
+    bin up to the smaller last bin must agree: the lists below differ at 30,
+    33, 40 and 43, which are clear decisions, and must fail."""
+    m = _capped_spectrum()
+    ok, why = orc.compare_peak_lists([10, 20, 33, 43], [10, 30, 40, 50], m, 1.0, 1e-3, 4)
+    assert not ok, why
+    # the legitimate form of the same flip: gpu keeps 20 and loses 50 past the cap
+    ok, why = orc.compare_peak_lists([10, 20, 30, 33], [10, 30, 33, 40], m, 1.0, 1e-3, 4)
+    assert ok, why
+    # a full list against a complete shorter one: the shorter one holds every
+    # peak, so a clear bin below the full list's last bin must be in both
+    ok, _ = orc.compare_peak_lists([10, 30, 33, 40], [10, 30, 40], m, 1.0, 1e-3, 4)
+    assert not ok  # 33 is missing from the complete reference list
+    ok, why = orc.compare_peak_lists([10, 20, 30, 33], [10, 30, 33], m, 1.0, 1e-3, 4)
+    assert ok, why  # the only difference is the excused 20
+
+
+def test_comparator_found_counts_under_tie_rule():
+    m = _capped_spectrum()
+    # bins 20 and 21 are within tau of each other, so either may be the
+    # maximum there: the counts may differ by up to two, not by three
+    ok, why = orc.compare_peak_lists([10, 30], [10, 30], m, 1.0, 1e-3, 2, found_gpu=8, found_ref=6)
+    assert ok, why
+    ok, _ = orc.compare_peak_lists([10, 30], [10, 30], m, 1.0, 1e-3, 2, found_gpu=9, found_ref=6)
+    assert not ok
+    ok, _ = orc.compare_peak_lists([10, 30], [10, 30], m, 1.0, 1e-3, 2, found_gpu=6, found_ref=6)
+    assert ok
