@@ -153,6 +153,19 @@ typedef struct spx_fused_desc {
 
 int spx_fused_peaks(spx_ctx* ctx, const spx_fused_desc* desc);
 
+/* spx_fused_peaks without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * workspace is a fixed 64 MiB device array per GPU, held until the process
+ * ends; long channels (nfft >= 32768) run in launch groups that fit it, and a
+ * learned-refinement call needs batch * max_peaks * 32 bytes of it (more:
+ * SPX_ECAP).  The context's tables for the call's nfft, n_point and window
+ * must exist already (any earlier spx_fused_peaks call with them creates
+ * them; otherwise SPX_EINVAL, nothing allocated).  Results equal
+ * spx_fused_peaks' bit for bit.  All calls on one GPU share the array:
+ * calls that overlap in time on one GPU race. */
+int spx_fused_peaks_noalloc(spx_ctx* ctx, const spx_fused_desc* desc);
+
 /* ---- host-resident data ------------------------------------------------------
  * The same operations on HOST buffers (every pointer, including all of the
  * descriptor's, is a host address; pageable or pinned).  The batch streams

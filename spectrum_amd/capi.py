@@ -81,6 +81,7 @@ SIGNATURES = [
     ("spx_peaks_top", _I, [_P, _P, _P, _U32, _U32, _U32, _F, _I]),
     ("spx_all_maxima", _I, [_P, _P, _P, _P, _U32, _U32, _F, _U32, _U32, _U32]),
     ("spx_fused_peaks", _I, [_P, C.POINTER(SpxFusedDesc)]),
+    ("spx_fused_peaks_noalloc", _I, [_P, C.POINTER(SpxFusedDesc)]),
     ("spx_fft_c2c_host", _I, [_P, _P, _P, _U32, _U32, _U32, _U32]),
     ("spx_fft_mag_phase_host", _I, [_P, _P, _P, _U32, _U32, _U32, _U32]),
     ("spx_fft_magnitude_host", _I, [_P, _P, _P, _U32, _U32, _U32, _U32, _I, _F]),
@@ -240,7 +241,10 @@ class Context:
 
     def fused_peaks(self, inp, batch, n_point, nfft, peaks, counts, *, window=1, dtype=DTYPE_C64, fs=1.0,
                     peak_mode=PEAKS_THRESHOLD, nsigma=6.0, max_peaks=32, s0=0, s1=0, refine=REFINE_PARABOLIC,
-                    gmax=None, stats=None, in_scale=None):
+                    gmax=None, stats=None, in_scale=None, noalloc=False):
+        """spx_fused_peaks, or with noalloc=True spx_fused_peaks_noalloc (fixed
+        workspace, no device allocation; the context must hold this length's
+        tables already)."""
         d = SpxFusedDesc()
         d.input = _ptr(inp)
         d.in_scale = _ptr(in_scale)
@@ -250,7 +254,10 @@ class Context:
         d.search_start, d.search_end, d.refine = s0, s1, refine
         d.peaks_out, d.counts_out = _ptr(peaks), _ptr(counts)
         d.gmax_out, d.stats_out = _ptr(gmax), _ptr(stats)
-        self.check(self.lib.spx_fused_peaks(self.h, C.byref(d)), "spx_fused_peaks")
+        if noalloc:
+            self.check(self.lib.spx_fused_peaks_noalloc(self.h, C.byref(d)), "spx_fused_peaks_noalloc")
+        else:
+            self.check(self.lib.spx_fused_peaks(self.h, C.byref(d)), "spx_fused_peaks")
 
     # -- host-resident data (numpy in, numpy out; synchronous, streamed in chunks)
     def set_host_chunk(self, nbytes: int):

@@ -159,6 +159,10 @@ int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out) {
     *out = it->second;
     return SPX_OK;
   }
+  if (c->ws_fixed)
+    return fail(c, SPX_EINVAL,
+                "spx_fused_peaks_noalloc: the tables of this length and window are not in the context yet; make one "
+                "spx_fused_peaks call with the same nfft, n_point and window first (it creates them)");
   std::vector<float> host;
   if (kind == TW) {
     host = make_twiddles(a, (int)b);
@@ -180,6 +184,13 @@ int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out) {
 }
 
 int ensure_scratch(spx_ctx* c, uint64_t bytes) {
+  if (c->ws_fixed) {
+    if (bytes > c->ws_fixed_bytes)
+      return fail(c, SPX_ECAP, "spx_fused_peaks_noalloc: needs " + std::to_string(bytes) +
+                                   " bytes of workspace, the fixed workspace holds " +
+                                   std::to_string(c->ws_fixed_bytes));
+    return SPX_OK;
+  }
   if (c->scratch_bytes >= bytes) return SPX_OK;
   SPX_HIP(c, hipSetDevice(c->device));
   if (c->scratch) {
@@ -227,7 +238,7 @@ int learned_features(spx_ctx* c, FusedArgs& a) {
   if (a.refine != 2 || a.peak_mode != 1 || a.kcap == 0 || !a.peaks) return SPX_OK;
   int rc = ensure_scratch(c, (uint64_t)a.batch * a.kcap * 32);
   if (rc) return rc;
-  a.mlp_feat = c->scratch;
+  a.mlp_feat = c->ws_fixed ? c->ws_fixed : c->scratch;
   return SPX_OK;
 }
 
@@ -271,9 +282,16 @@ int run_fused(spx_ctx* c, FusedArgs& a) {
       SPX_HIP(c, spx_launch_fourstep_dense(&a, c->scratch, chunk, c->stream));
       return SPX_OK;
     }
-    const uint32_t chunk = fourstep_chunk(a.nfft, a.batch);
+    uint32_t chunk = fourstep_chunk(a.nfft, a.batch);
+    if (c->ws_fixed) {
+      // as many channels per launch group as the fixed workspace holds
+      const uint64_t per = spx_fs_workspace_bytes(a.nfft, 1) - spx_fs_workspace_bytes(a.nfft, 0);
+      const uint64_t fit = (c->ws_fixed_bytes - spx_fs_workspace_bytes(a.nfft, 0)) / per;
+      if (fit == 0) return ensure_scratch(c, spx_fs_workspace_bytes(a.nfft, 1));  // reports the shortfall
+      chunk = (uint32_t)std::min<uint64_t>(fit, a.batch);
+    }
     if ((rc = ensure_scratch(c, spx_fs_workspace_bytes(a.nfft, chunk)))) return rc;
-    SPX_HIP(c, spx_launch_fourstep(&a, c->scratch, chunk, c->stream));
+    SPX_HIP(c, spx_launch_fourstep(&a, c->ws_fixed ? c->ws_fixed : c->scratch, chunk, c->stream));
     return SPX_OK;
   }
   if ((rc = prepare(c, a, &grid))) return rc;
@@ -343,6 +361,7 @@ int spx_create(int device, spx_ctx** out) {
   c->hbm = p.totalGlobalMem;
   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SPX_EHIP; }
   c->stream = c->own_stream;
+  noalloc_init(device);
   *out = c;
   return SPX_OK;
 }

@@ -28,6 +28,10 @@ struct spx_ctx {
   uint64_t mlp_partial_bytes = 0;
   HostStage* hstage = nullptr;             // created by the first *_host call
   uint64_t host_chunk_bytes = 32ull << 20;  // per-chunk transfer size of the *_host calls
+  // set only during spx_fused_peaks_noalloc: the fixed workspace that call
+  // uses instead of `scratch`; tables are then looked up, never created
+  void* ws_fixed = nullptr;
+  uint64_t ws_fixed_bytes = 0;
 };
 
 namespace spx_host {
@@ -47,6 +51,12 @@ int window_ptr(spx_ctx* c, uint32_t window, uint32_t n_point, const float** w);
 void host_stage_free(spx_ctx* c);
 // Channels per launch group of the split long-channel path (bounds its workspace).
 uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch);
+// Validates a fused descriptor into kernel arguments; runs them.
+int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a);
+int run_fused(spx_ctx* c, FusedArgs& a);
+// Device address of the fixed workspace array on `device` (spx_noalloc.hip);
+// taken once per device at spx_create, with that device current.
+void noalloc_init(int device);
 
 }  // namespace spx_host
 
