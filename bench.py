@@ -65,36 +65,65 @@ class HipTimer:
         return float(ms.value)
 
 
-def cpu_baseline(x, nfft, K, budget_s=10.0):
-    """Reference CPU FFT (pocketfft float, one thread, plan per call) + oracle peak rules."""
+def cpu_model() -> str:
+    """Host CPU model name (from /proc/cpuinfo) and logical CPU count."""
+    name = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{name}, {os.cpu_count()} logical cpus"
+
+
+def _median_runs(fn, runs=5):
+    """One warm-up run, then the median wall time of `runs` runs (seconds)."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(x, nfft, K, target_s=2.0, runs=5):
+    """Reference CPU FFT (pocketfft float, one thread, plan per call) + oracle
+    peak rules on a bounded sample of channels: 1 warm-up run, median of
+    `runs` runs, the sample sized so that one run takes about target_s."""
     from oracle import oracle as orc
 
     w = orc.window(1, nfft).astype(np.float32)
     kind = "reference" if orc.have_reference() else "port"
-    done = 0
-    t0 = time.perf_counter()
-    for b in range(x.shape[0]):
+
+    def one(b):
         xw = (x[b] * w).astype(np.complex64)[None]
         X = orc.ref_cpufft(xw)[0] if kind == "reference" else orc.spectrum64(xw, nfft, 0)[0].astype(np.complex64)
         orc.fused_peaks(X, 1.0, nsigma=6.0, kcap=K)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    one(0)
+    per = max(time.perf_counter() - t0, 1e-6)
+    S = int(min(x.shape[0], max(8, target_s / per)))
+    med, ts = _median_runs(lambda: [one(b) for b in range(S)], runs)
     return {
-        "value": done / dt,
+        "value": S / med,
         "unit": "channels/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"{done} of {x.shape[0]} channels x {nfft} c64 (Hann, pocketfft c2c<float> via the reference "
-                  f"CpuFFT::ForwardC2C, then |X|/mean+6std/local-max/K={K}/parabolic), single thread, "
-                  f"host {os.cpu_count()} logical cpus",
+        "sample": f"{S} of {x.shape[0]} channels x {nfft} c64 (Hann, pocketfft c2c<float> via the reference "
+                  f"CpuFFT::ForwardC2C, then |X|/mean+6std/local-max/K={K}/parabolic), single thread; "
+                  f"1 warm-up run, median of {runs} runs ({', '.join(f'{t:.2f}' for t in ts)} s)",
+        "host": cpu_model(),
     }
 
 
-def cpu_baseline_mt(x, nfft, K, threads, budget_s=8.0):
+def cpu_baseline_mt(x, nfft, K, threads, runs=5):
     """Best-effort multi-core CPU path: pocketfft c2c over [B, N] with `threads` threads
-    (reference sources, oracle/_ref) + the oracle's peak rules over channels on a thread pool."""
+    (reference sources, oracle/_ref) + the oracle's peak rules over channels on a thread pool;
+    1 warm-up run, median of `runs` runs over a sample of up to 512 channels."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import oracle as orc
@@ -102,21 +131,18 @@ def cpu_baseline_mt(x, nfft, K, threads, budget_s=8.0):
     if not orc.have_reference():
         return None
     w = orc.window(1, nfft).astype(np.float32)
-    chunk = 64
-    done = 0
-    t0 = time.perf_counter()
+    S = min(x.shape[0], 512)
+    xw = (x[:S] * w).astype(np.complex64)
     with ThreadPoolExecutor(threads) as pool:
-        for c0 in range(0, x.shape[0], chunk):
-            xw = (x[c0:c0 + chunk] * w).astype(np.complex64)
+        def one():
             X = orc.ref_pocketfft_mt(xw, threads)
             list(pool.map(lambda r: orc.fused_peaks(r, 1.0, nsigma=6.0, kcap=K), list(X)))
-            done += len(xw)
-            if time.perf_counter() - t0 > budget_s:
-                break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "channels/s", "cores": threads, "kind": "reference",
-            "sample": f"{done} channels x {nfft}: pocketfft c2c<float> with {threads} threads (reference sources), "
-                      f"then the peak rules on a {threads}-thread pool"}
+
+        med, ts = _median_runs(one, runs)
+    return {"value": S / med, "unit": "channels/s", "cores": threads, "kind": "reference",
+            "sample": f"{S} channels x {nfft}: pocketfft c2c<float> with {threads} threads (reference sources), "
+                      f"then the peak rules on a {threads}-thread pool; 1 warm-up run, median of {runs} runs "
+                      f"({', '.join(f'{t:.2f}' for t in ts)} s)"}
 
 
 def _timer_ms(ctx, fn, reps=30, warm=20):

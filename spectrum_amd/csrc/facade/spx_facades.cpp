@@ -25,6 +25,19 @@ double ms_since(Clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
 }
 
+// A stage time as the reference's ROCmProfilingData (MakeROCmDataFromEvents,
+// include/spectrum/utils/rocm_profiling_helpers.hpp:60-63): start 0, end and
+// complete = the duration; kind 0 upload, 1 compute, 2 download.
+drv_gpu_lib::ROCmProfilingData prof_data(double ms, uint32_t kind, const char* op) {
+  drv_gpu_lib::ROCmProfilingData d{};
+  const uint64_t ns = (uint64_t)(ms * 1e6);
+  d.end_ns = ns;
+  d.complete_ns = ns;
+  d.kind = kind;
+  d.op_string = op;
+  return d;
+}
+
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -88,11 +101,56 @@ void ROCmBackend::Initialize(int device_index) {
   hip_check(hipSetDevice(device_), "hipSetDevice");
   hipDeviceProp_t p;
   hip_check(hipGetDeviceProperties(&p, device_), "hipGetDeviceProperties");
-  name_ = p.name;
-  mem_ = p.totalGlobalMem;
+  info_.name = p.name;
+  info_.global_mem_bytes = p.totalGlobalMem;
+  info_.compute_units = p.multiProcessorCount;
+  info_.warp_size = p.warpSize;
+  info_.device_index = device_;
+  core_.warp_size_ = p.warpSize;
+  core_.compute_units_ = p.multiProcessorCount;
+  if (stream_) {
+    (void)hipStreamDestroy((hipStream_t)stream_);
+    stream_ = nullptr;
+  }
   hipStream_t s;
   hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
   stream_ = s;
+}
+
+void* ROCmBackend::Allocate(size_t bytes) {
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  void* p = nullptr;
+  hip_check(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc");
+  return p;
+}
+
+void ROCmBackend::Free(void* ptr) {
+  if (!ptr) return;
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  hip_check(hipFree(ptr), "hipFree");
+}
+
+void ROCmBackend::MemcpyHostToDevice(void* dst, const void* src, size_t bytes) {
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream_), "hipMemcpyAsync H2D");
+  hip_check(hipStreamSynchronize((hipStream_t)stream_), "hipStreamSynchronize");
+}
+
+void ROCmBackend::MemcpyDeviceToHost(void* dst, const void* src, size_t bytes) {
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream_), "hipMemcpyAsync D2H");
+  hip_check(hipStreamSynchronize((hipStream_t)stream_), "hipStreamSynchronize");
+}
+
+void ROCmBackend::MemcpyDeviceToDevice(void* dst, const void* src, size_t bytes) {
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream_), "hipMemcpyAsync D2D");
+  hip_check(hipStreamSynchronize((hipStream_t)stream_), "hipStreamSynchronize");
+}
+
+void ROCmBackend::Synchronize() {
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  hip_check(hipStreamSynchronize((hipStream_t)stream_), "hipStreamSynchronize");
 }
 
 int ROCmCore::GetAvailableDeviceCount() {
@@ -217,9 +275,9 @@ std::vector<FFTComplexResult> FFTProcessorROCm::ProcessComplex(const std::vector
   prof_.post_processing_time_ms = 0.0;
   prof_.total_time_ms = ms_since(t0);
   if (prof_events) {
-    prof_events->push_back({"Upload", {prof_.upload_time_ms}});
-    prof_events->push_back({"FFT", {prof_.fft_time_ms}});
-    prof_events->push_back({"Download", {prof_.download_time_ms}});
+    prof_events->push_back({"Upload", prof_data(prof_.upload_time_ms, 0, "Upload")});
+    prof_events->push_back({"FFT", prof_data(prof_.fft_time_ms, 1, "FFT")});
+    prof_events->push_back({"Download", prof_data(prof_.download_time_ms, 2, "Download")});
   }
   return r;
 }
@@ -267,9 +325,9 @@ std::vector<FFTMagPhaseResult> FFTProcessorROCm::ProcessMagPhase(const std::vect
   auto r = pack_magphase(inter, B, nFFT_, params);
   prof_.total_time_ms = ms_since(t0);
   if (prof_events) {
-    prof_events->push_back({"Upload", {prof_.upload_time_ms}});
-    prof_events->push_back({"FFT+MagPhase", {prof_.fft_time_ms}});
-    prof_events->push_back({"Download", {prof_.download_time_ms}});
+    prof_events->push_back({"Upload", prof_data(prof_.upload_time_ms, 0, "Upload")});
+    prof_events->push_back({"FFT+MagPhase", prof_data(prof_.fft_time_ms, 1, "FFT+MagPhase")});
+    prof_events->push_back({"Download", prof_data(prof_.download_time_ms, 2, "Download")});
   }
   return r;
 }
@@ -309,7 +367,7 @@ void FFTProcessorROCm::ProcessMagnitudesToGPU(void* gpu_data, void* gpu_out, con
                                  (uint32_t)window, squared ? 1 : 0, 1.0f), "spx_fft_magnitude");
   spx_check(c, spx_synchronize(c), "spx_fft_magnitude");
   prof_.fft_time_ms = ms_since(t0);
-  if (prof_events) prof_events->push_back({"FFT+Magnitude", {prof_.fft_time_ms}});
+  if (prof_events) prof_events->push_back({"FFT+Magnitude", prof_data(prof_.fft_time_ms, 1, "FFT+Magnitude")});
 }
 
 // ===========================================================================
@@ -641,9 +699,9 @@ std::vector<SpectrumResult> SpectrumProcessorROCm::ProcessBatch(const std::vecto
   auto r = PeaksFromDevice(in_, count, start_antenna);
   prof_.total_time_ms = ms_since(t0);
   if (prof_events) {
-    prof_events->push_back({"Upload", {prof_.upload_time_ms}});
-    prof_events->push_back({"FFT+Post", {prof_.fft_time_ms}});
-    prof_events->push_back({"Download", {prof_.download_time_ms}});
+    prof_events->push_back({"Upload", prof_data(prof_.upload_time_ms, 0, "Upload")});
+    prof_events->push_back({"FFT+Post", prof_data(prof_.fft_time_ms, 1, "FFT+Post")});
+    prof_events->push_back({"Download", prof_data(prof_.download_time_ms, 2, "Download")});
   }
   return r;
 }
@@ -760,8 +818,8 @@ AllMaximaResult SpectrumProcessorROCm::FindAllMaximaFromCPU(const std::vector<st
     r.destination = dest;
     fill_cpu_beams(r, all, found, cap);
     if (prof_events) {
-      prof_events->push_back({"Upload", {prof_.upload_time_ms}});
-      prof_events->push_back({"FusedAllMaxima", {prof_.fft_time_ms}});
+      prof_events->push_back({"Upload", prof_data(prof_.upload_time_ms, 0, "Upload")});
+      prof_events->push_back({"FusedAllMaxima", prof_data(prof_.fft_time_ms, 1, "FusedAllMaxima")});
     }
     return r;
   }
@@ -771,8 +829,8 @@ AllMaximaResult SpectrumProcessorROCm::FindAllMaximaFromCPU(const std::vector<st
   prof_.upload_time_ms = ms_since(t0);
   auto r = AllMaximaFromSignalDevice(in_, params_.antenna_count, dest, s0, s1);
   if (prof_events) {
-    prof_events->push_back({"Upload", {prof_.upload_time_ms}});
-    prof_events->push_back({"FusedAllMaxima", {prof_.fft_time_ms}});
+    prof_events->push_back({"Upload", prof_data(prof_.upload_time_ms, 0, "Upload")});
+    prof_events->push_back({"FusedAllMaxima", prof_data(prof_.fft_time_ms, 1, "FusedAllMaxima")});
   }
   return r;
 }

@@ -6,6 +6,7 @@
 // check prints one line.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -227,6 +228,86 @@ int main() {
     }
     if (d) (void)hipFree(d);
     check(ok2, "ComplexToMagPhase: norm_coeff -1 -> 1/n, ToBuffer");
+  }
+  // ---- the backend as reference callers use it: device memory through the
+  // backend (tests/test_fft_processor_rocm.hpp:211-238,
+  // tests/test_complex_to_mag_phase_rocm.hpp:133-223), GetDeviceInfo / GetCore,
+  // and the ROCmProfilingData of a profiled call
+  {
+    const drv_gpu_lib::DeviceInfo info = backend.GetDeviceInfo();
+    check(info.name == backend.GetDeviceName() && info.warp_size == 64 && backend.GetCore().GetWarpSize() == 64 &&
+              backend.GetGlobalMemorySize() > 0 && info.global_mem_bytes == backend.GetGlobalMemorySize(),
+          "backend: GetDeviceInfo().name, GetCore().GetWarpSize() == 64, GetGlobalMemorySize");
+
+    // FFT from a device buffer the backend allocated and filled
+    fft_processor::FFTProcessorROCm fft(&backend);
+    cvec tone(1024);
+    for (int t = 0; t < 1024; ++t) tone[t] = std::polar(1.0f, 2.0f * (float)M_PI * 100.0f * t / 1000.0f);
+    const size_t bytes = tone.size() * sizeof(std::complex<float>);
+    void* gpu_data = backend.Allocate(bytes);
+    backend.MemcpyHostToDevice(gpu_data, tone.data(), bytes);
+    fft_processor::FFTProcessorParams p;
+    p.beam_count = 1; p.n_point = 1024; p.sample_rate = 1000.0f;
+    p.output_mode = fft_processor::FFTOutputMode::COMPLEX;
+    auto r = fft.ProcessComplex(gpu_data, p, bytes);
+    backend.Free(gpu_data);
+    int best = 0;
+    for (int k = 1; k < 512; ++k)
+      if (std::abs(r[0].spectrum[k]) > std::abs(r[0].spectrum[best])) best = k;
+    check(r.size() == 1 && best == (int)(100.0f * r[0].nFFT / 1000.0f),
+          "backend.Allocate + MemcpyHostToDevice -> ProcessComplex(void*) -> backend.Free: bin 102");
+
+    // profiling events carry the reference's timestamps
+    fft_processor::ROCmProfEvents ev;
+    (void)fft.ProcessComplex(tone, p, &ev);
+    bool okp = ev.size() == 3;
+    for (auto& e : ev)
+      okp = okp && e.second.start_ns == 0 && e.second.complete_ns == e.second.end_ns && !e.second.op_string.empty() &&
+            e.second.op_string == e.first;
+    okp = okp && ev[1].second.kind == 1 && ev[1].second.end_ns > 0;
+    check(okp, "ProcessComplex(prof_events): Upload/FFT/Download with end_ns, op_string, kind");
+
+    // ComplexToMagPhaseROCm: GPU input -> CPU output, CPU -> GPU, GPU -> GPU
+    fft_processor::ComplexToMagPhaseROCm conv(&backend);
+    const uint32_t beams = 4, n = 2048;
+    cvec data((size_t)beams * n);
+    for (uint32_t b = 0; b < beams; ++b)
+      for (uint32_t t = 0; t < n; ++t)
+        data[(size_t)b * n + t] = std::polar(1.0f + b, 2.0f * (float)M_PI * 150.0f * t / 1000.0f + 0.3f * b);
+    const size_t sz = data.size() * sizeof(std::complex<float>);
+    void* gpu_in = backend.Allocate(sz);
+    backend.MemcpyHostToDevice(gpu_in, data.data(), sz);
+    fft_processor::MagPhaseParams mpp;
+    mpp.beam_count = beams; mpp.n_point = n;
+    auto res = conv.Process(gpu_in, mpp, sz);
+    float worst = 0.f;
+    for (uint32_t b = 0; b < beams && res.size() == beams; ++b)
+      for (uint32_t t = 0; t < n; ++t) {
+        const auto z = data[(size_t)b * n + t];
+        worst = std::max(worst, std::fabs(res[b].magnitude[t] - std::abs(z)));
+        float pe = std::fabs(res[b].phase[t] - std::arg(z));
+        if (pe > (float)M_PI) pe = 2.0f * (float)M_PI - pe;
+        worst = std::max(worst, pe);
+      }
+    check(res.size() == beams && worst < 1e-3f, "ComplexToMagPhase: backend buffer in -> host results (|z|, arg z)");
+    void* gpu_out = conv.ProcessToGPU(gpu_in, mpp, sz);
+    backend.Free(gpu_in);
+    std::vector<float> raw((size_t)beams * n * 2);
+    backend.MemcpyDeviceToHost(raw.data(), gpu_out, raw.size() * sizeof(float));
+    backend.Free(gpu_out);
+    worst = 0.f;
+    for (size_t i = 0; i < data.size(); ++i) worst = std::max(worst, std::fabs(std::abs(data[i]) - raw[2 * i]));
+    check(worst < 1e-3f, "ComplexToMagPhase: ProcessToGPU(void*) -> backend.MemcpyDeviceToHost -> backend.Free");
+    void* gpu_out2 = conv.ProcessToGPU(data, mpp);
+    backend.MemcpyDeviceToHost(raw.data(), gpu_out2, raw.size() * sizeof(float));
+    backend.Free(gpu_out2);
+    worst = 0.f;
+    for (size_t i = 0; i < data.size(); ++i) {
+      float pe = std::fabs(std::arg(data[i]) - raw[2 * i + 1]);
+      if (pe > (float)M_PI) pe = 2.0f * (float)M_PI - pe;
+      worst = std::max(worst, pe);
+    }
+    check(worst < 1e-3f, "ComplexToMagPhase: ProcessToGPU(host vector) phases");
   }
   std::printf("%s (%d failed)\n", g_fail ? "FAILED" : "ALL PASSED", g_fail);
   return g_fail ? 1 : 0;
