@@ -197,13 +197,13 @@ FFTProcessorROCm& FFTProcessorROCm::operator=(FFTProcessorROCm&& o) noexcept {
 uint32_t FFTProcessorROCm::NextPowerOf2(uint32_t n) { return next_pow2(n); }
 
 // nFFT = nextPow2(n_point) * repeat_count (fft_processor_rocm.cpp:665-677); the
-// gfx950 transforms cover power-of-two lengths 2 .. 262144.
+// gfx950 transforms cover power-of-two lengths 2 .. 2^24.
 void FFTProcessorROCm::CalculateNFFT(const FFTProcessorParams& p) {
   if (p.n_point == 0) throw std::invalid_argument("FFTProcessorROCm: n_point must be > 0");
   if (p.repeat_count == 0) throw std::invalid_argument("FFTProcessorROCm: repeat_count must be >= 1");
   const uint64_t n = (uint64_t)next_pow2(p.n_point) * p.repeat_count;
-  if (n > (1u << 18) || !is_pow2((uint32_t)n))
-    throw std::invalid_argument("FFTProcessorROCm: nFFT must be a power of two <= 262144 (repeat_count a power of two)");
+  if (n > (1u << 24) || !is_pow2((uint32_t)n))
+    throw std::invalid_argument("FFTProcessorROCm: nFFT must be a power of two <= 2^24 = 16777216 (repeat_count a power of two)");
   n_point_ = p.n_point;
   nFFT_ = (uint32_t)std::max<uint64_t>(n, 2);
 }
@@ -616,8 +616,8 @@ void SpectrumProcessorROCm::Initialize(const SpectrumParams& params) {
   if (p.repeat_count == 0) throw std::invalid_argument("Initialize: repeat_count must be >= 1");
   p.base_fft = next_pow2(p.n_point);
   const uint64_t n = (uint64_t)p.base_fft * p.repeat_count;
-  if (n > (1u << 18) || !is_pow2((uint32_t)n) || n < 4)
-    throw std::invalid_argument("Initialize: nFFT must be a power of two in [4, 262144]");
+  if (n > (1u << 24) || !is_pow2((uint32_t)n) || n < 4)
+    throw std::invalid_argument("Initialize: nFFT must be a power of two in [4, 2^24 = 16777216]");
   p.nFFT = (uint32_t)n;
   if (p.search_range == 0) p.search_range = p.nFFT / 4;
   params_ = p;

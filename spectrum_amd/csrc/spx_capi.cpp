@@ -90,7 +90,7 @@ float bf16_bits_to_f32(uint16_t h) {
   return v;
 }
 
-constexpr uint32_t kMaxN = 1u << 18;
+constexpr uint32_t kMaxN = 1u << 24;  // 2^19 .. 2^24: spx_large.hip
 
 // Per-pass twiddle tables of a length-L transform under radix order V (V = 0:
 // radix-16 passes, the leftover radix last; V = 1: the leftover first).  Pass
@@ -166,9 +166,11 @@ int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out) {
   std::vector<float> host;
   if (kind == TW) {
     host = make_twiddles(a, (int)b);
-    const int want = spx_twiddle_table_len(a, (int)b);
-    if (want < 0 || (size_t)std::max(want, 1) * 2 != host.size())
-      return fail(c, SPX_EINVAL, "twiddle table size mismatch");
+    if (a <= 16384) {  // lengths with a compile-time plan: the table must match it
+      const int want = spx_twiddle_table_len(a, (int)b);
+      if (want < 0 || (size_t)std::max(want, 1) * 2 != host.size())
+        return fail(c, SPX_EINVAL, "twiddle table size mismatch");
+    }
   }
   else if (kind == TW4) host = make_fourstep(a);
   else if (kind == MIX) host = make_mixed_tables(a);
@@ -212,7 +214,7 @@ int check_fft_shape(spx_ctx* c, const void* in, const void* out, uint32_t batch,
                     uint32_t window) {
   if (!c) return SPX_EINVAL;
   if (!in || !out) return fail(c, SPX_EINVAL, "null device pointer");
-  if (!is_pow2(nfft) || nfft > kMaxN) return fail(c, SPX_EINVAL, "nfft must be a power of two in [2, 262144]");
+  if (!is_pow2(nfft) || nfft > kMaxN) return fail(c, SPX_EINVAL, "nfft must be a power of two in [2, 2^24]");
   if (n_point == 0 || n_point > nfft) return fail(c, SPX_EINVAL, "n_point must be in [1, nfft]");
   if (window > SPX_WINDOW_BLACKMAN) return fail(c, SPX_EINVAL, "unknown window type");
   (void)batch;
@@ -264,6 +266,19 @@ uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch) {
 int run_fused(spx_ctx* c, FusedArgs& a) {
   if (a.batch == 0) return SPX_OK;
   int grid = 0, rc;
+  if (spx_large_is(a.nfft)) {
+    // 2^19 .. 2^24: global-memory Stockham passes, peak search on the stored spectrum
+    const void* t;
+    if ((rc = get_table(c, TW, a.nfft, 0, &t))) return rc;
+    const uint64_t per = spx_large_workspace_bytes(a.nfft, 1);
+    const uint64_t budget = c->ws_fixed ? c->ws_fixed_bytes : (2ull << 30);
+    uint32_t chunk = (uint32_t)std::min<uint64_t>(a.batch, std::max<uint64_t>(c->ws_fixed ? 0 : 1, budget / per));
+    if (chunk == 0) return ensure_scratch(c, per);  // the fixed workspace is too small: reports the shortfall
+    if ((rc = ensure_scratch(c, spx_large_workspace_bytes(a.nfft, chunk)))) return rc;
+    SPX_HIP(c, hipSetDevice(c->device));
+    SPX_HIP(c, spx_launch_large(&a, (const float2*)t, c->ws_fixed ? c->ws_fixed : c->scratch, chunk, c->stream));
+    return SPX_OK;
+  }
   if (spx_fused_is_large(a.nfft)) {
     // split four-step path (spx_fourstep.hip): three launches for the peak
     // searches, two for the dense outputs

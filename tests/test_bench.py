@@ -25,7 +25,7 @@ def test_cpu_baseline_fields():
     from spectrum_amd import capi
 
     x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED0002, 3, 4096)
-    out = bench.cpu_baseline(x, 4096, 64, budget_s=5.0)
+    out = bench.cpu_baseline(x, 4096, 64, target_s=5.0)
     assert set(out) >= {"value", "unit", "cores", "kind", "sample"}
     assert out["unit"] == "channels/s" and out["cores"] == 1 and out["kind"] in ("reference", "port")
     assert np.isfinite(out["value"]) and out["value"] > 0
