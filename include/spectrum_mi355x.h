@@ -113,6 +113,17 @@ int spx_magnitude(spx_ctx* ctx, const void* in, float* out, uint64_t total, floa
  * SpectrumPostOp::Execute, spectrum_post_op.hpp:77-106). */
 int spx_peaks_top(spx_ctx* ctx, const void* spectrum, void* maxval_out, uint32_t batch, uint32_t nfft,
                   uint32_t search_range, float sample_rate, int two_peaks);
+/* ... and the same search fused into the transform: zero-pad n_point ->
+ * nfft, optional window, FFT, both argmaxes and the records, with no spectrum
+ * stored for nfft <= 2^18 (one launch per batch up to 2^14, three per chunk
+ * of channels from 2^15 to 2^18; from 2^19 the spectrum of a chunk goes
+ * through a context buffer).  Same records as spx_fft_c2c + spx_peaks_top.
+ * Replaces SpectrumProcessorROCm::ProcessFromCPU / ProcessFromGPU's
+ * pad -> hipfftExecC2C -> SpectrumPostOp chain (spectrum_processor_rocm.cpp:
+ * 179-420: ExecutePadKernel, ExecuteFFT, ExecutePostKernel; post_kernel_one_peak /
+ * _two_peaks, fft_kernel_sources_rocm.hpp:122-519). */
+int spx_fused_peaks_top(spx_ctx* ctx, const void* signal, void* maxval_out, uint32_t batch, uint32_t n_point,
+                        uint32_t nfft, uint32_t window, uint32_t search_range, float sample_rate, int two_peaks);
 /* AllMaxima: strict local maxima of |X| in [s0, s1) (s0 >= 1, s1 <= nfft - 1),
  * ascending, at most cap records per beam, counts[b] = number found (uncapped)
  * (AllMaximaPipelineROCm::Execute, all_maxima_pipeline_rocm.cpp:258-472;

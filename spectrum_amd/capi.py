@@ -79,6 +79,7 @@ SIGNATURES = [
     ("spx_mag_phase", _I, [_P, _P, _P, _U64]),
     ("spx_magnitude", _I, [_P, _P, _P, _U64, _F, _I]),
     ("spx_peaks_top", _I, [_P, _P, _P, _U32, _U32, _U32, _F, _I]),
+    ("spx_fused_peaks_top", _I, [_P, _P, _P, _U32, _U32, _U32, _U32, _U32, _F, _I]),
     ("spx_all_maxima", _I, [_P, _P, _P, _P, _U32, _U32, _F, _U32, _U32, _U32]),
     ("spx_fused_peaks", _I, [_P, C.POINTER(SpxFusedDesc)]),
     ("spx_fused_peaks_noalloc", _I, [_P, C.POINTER(SpxFusedDesc)]),
@@ -234,6 +235,14 @@ class Context:
     def peaks_top(self, spec, out, batch, nfft, search_range, fs, two_peaks=False):
         self.check(self.lib.spx_peaks_top(self.h, _ptr(spec), _ptr(out), batch, nfft, search_range, float(fs),
                                           int(bool(two_peaks))), "spx_peaks_top")
+
+    def fused_peaks_top(self, inp, out, batch, n_point, nfft, search_range, fs, two_peaks=False, window=0):
+        """Pad + window + FFT + reference ONE_PEAK (4 records per channel) or
+        TWO_PEAKS (8) in the transform's launch; `out` holds batch * 4 (8)
+        MAXVALUE_DTYPE records."""
+        self.check(self.lib.spx_fused_peaks_top(self.h, _ptr(inp), _ptr(out), batch, n_point, nfft, window,
+                                                search_range, float(fs), int(bool(two_peaks))),
+                   "spx_fused_peaks_top")
 
     def all_maxima(self, spec, out, counts, batch, nfft, fs, s0, s1, cap):
         self.check(self.lib.spx_all_maxima(self.h, _ptr(spec), _ptr(out), _ptr(counts), batch, nfft, float(fs),

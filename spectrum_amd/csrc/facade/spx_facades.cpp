@@ -644,12 +644,12 @@ std::vector<SpectrumResult> SpectrumProcessorROCm::PeaksFromDevice(const void* s
   const uint32_t N = params_.nFFT;
   const bool two = params_.peak_mode == PeakSearchMode::TWO_PEAKS;
   const size_t R = two ? 8 : 4;
-  Reserve(&spec_, &spec_cap_, beams * N * 8);
   Reserve(&rec_, &rec_cap_, beams * R * sizeof(MaxValue));
   auto t0 = Clock::now();
-  spx_check(c, spx_fft_c2c(c, signal, spec_, (uint32_t)beams, params_.n_point, N, SPX_WINDOW_NONE), "spx_fft_c2c");
-  spx_check(c, spx_peaks_top(c, spec_, rec_, (uint32_t)beams, N, params_.search_range, params_.sample_rate,
-                             two ? 1 : 0), "spx_peaks_top");
+  // pad + FFT + both argmaxes + records in the transform's own launch (three
+  // launches on the split path, 2^15 .. 2^18), no spectrum in HBM
+  spx_check(c, spx_fused_peaks_top(c, signal, rec_, (uint32_t)beams, params_.n_point, N, SPX_WINDOW_NONE,
+                                   params_.search_range, params_.sample_rate, two ? 1 : 0), "spx_fused_peaks_top");
   spx_check(c, spx_synchronize(c), "peaks");
   prof_.fft_time_ms = ms_since(t0);
   t0 = Clock::now();

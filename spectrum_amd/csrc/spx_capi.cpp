@@ -387,6 +387,7 @@ int spx_destroy(spx_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->tables) (void)hipFree(kv.second);
   if (c->scratch) (void)hipFree(c->scratch);
+  if (c->spec_buf) (void)hipFree(c->spec_buf);
   if (c->mlp_partial) (void)hipFree(c->mlp_partial);
   host_stage_free(c);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -498,6 +499,78 @@ int spx_peaks_top(spx_ctx* c, const void* spec, void* out, uint32_t batch, uint3
   if (search_range > nfft) search_range = nfft;
   SPX_HIP(c, hipSetDevice(c->device));
   SPX_HIP(c, spx_launch_peaks_top(spec, out, batch, nfft, search_range, fs, two_peaks, c->stream));
+  return SPX_OK;
+}
+
+int spx_fused_peaks_top(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_t n_point, uint32_t nfft,
+                        uint32_t window, uint32_t search_range, float fs, int two_peaks) {
+  int rc = check_fft_shape(c, in, out, batch, n_point, nfft, window);
+  if (rc || batch == 0) return rc;
+  if (search_range > nfft) search_range = nfft;
+  const uint32_t half = search_range / 2;
+  FusedArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in;
+  a.batch = batch;
+  a.n_point = n_point;
+  a.nfft = nfft;
+  a.dtype = SPX_DTYPE_C64;
+  a.fs = fs;
+  if ((rc = window_ptr(c, window, n_point, &a.window))) return rc;
+  a.peak_mode = two_peaks ? 4u : 3u;
+  a.s0 = half;
+  a.s1 = nfft - half;
+  a.peaks = (spx_max_value*)out;
+  if (!spx_fused_is_large(nfft)) {
+    // one launch: transform, both argmaxes and the records per workgroup
+    int grid = 0;
+    if ((rc = prepare(c, a, &grid))) return rc;
+    SPX_HIP(c, hipSetDevice(c->device));
+    SPX_HIP(c, spx_launch_fused(&a, grid, c->stream));
+    return SPX_OK;
+  }
+  if (!spx_large_is(nfft) && half >= 1) {
+    // three launches per chunk of channels, no spectrum stored
+    const void* t;
+    if ((rc = get_table(c, TW, 256, 0, &t))) return rc;
+    a.tw_a = (const float2*)t;
+    if ((rc = get_table(c, TW, nfft / 256, 0, &t))) return rc;
+    a.tw_b = (const float2*)t;
+    if ((rc = get_table(c, TW4, nfft, 0, &t))) return rc;
+    a.tw_4 = (const float2*)t;
+    const uint64_t per = spx_fs_top_workspace_bytes(nfft, 1);
+    const uint32_t chunk = (uint32_t)std::min<uint64_t>(batch, std::max<uint64_t>(64, (2ull << 30) / per));
+    if ((rc = ensure_scratch(c, spx_fs_top_workspace_bytes(nfft, chunk)))) return rc;
+    SPX_HIP(c, hipSetDevice(c->device));
+    SPX_HIP(c, spx_launch_fourstep_top(&a, c->scratch, chunk, c->stream));
+    return SPX_OK;
+  }
+  // 2^19 .. 2^24 (and an empty search range): the spectrum of a chunk of
+  // channels into spec_buf, then the search on it
+  const uint32_t recs = two_peaks ? 8u : 4u;
+  const uint64_t per = (uint64_t)nfft * 8;
+  const uint32_t chunk = (uint32_t)std::min<uint64_t>(batch, std::max<uint64_t>(1, (1ull << 30) / per));
+  if (c->spec_bytes < per * chunk) {
+    SPX_HIP(c, hipSetDevice(c->device));
+    if (c->spec_buf) {
+      SPX_HIP(c, hipStreamSynchronize(c->stream));
+      SPX_HIP(c, hipFree(c->spec_buf));
+      c->spec_buf = nullptr;
+      c->spec_bytes = 0;
+    }
+    if (hipMalloc(&c->spec_buf, per * chunk) != hipSuccess) {
+      c->spec_buf = nullptr;
+      return oom(c, "spectrum buffer allocation failed");
+    }
+    c->spec_bytes = per * chunk;
+  }
+  for (uint32_t c0 = 0; c0 < batch; c0 += chunk) {
+    const uint32_t nb = std::min(chunk, batch - c0);
+    const float2* src = (const float2*)in + (uint64_t)c0 * n_point;
+    if ((rc = dense_fft(c, src, c->spec_buf, nb, n_point, nfft, window, 0, 1.0f))) return rc;
+    SPX_HIP(c, spx_launch_peaks_top(c->spec_buf, (spx_max_value*)out + (uint64_t)c0 * recs, nb, nfft, search_range, fs,
+                                    two_peaks, c->stream));
+  }
   return SPX_OK;
 }
 

@@ -443,6 +443,222 @@ static hipError_t launch_fs_dense(const FusedArgs& a, const FsWork& w0, uint32_t
   return hipSuccess;
 }
 
+// ---------------------------------------------------------------------------
+// Reference ONE_PEAK / TWO_PEAKS (peak_mode 3 / 4) on long channels, taken on
+// the row transforms without storing the spectrum: k_fs_cols, then
+// k_fs_rows_top (per row block and side the largest |X|, lowest bin on ties,
+// with X there and at both neighbours; the block's first and last rows as
+// complex edge rows), then k_fs_top_finish (one wave per channel: the
+// winners over the blocks, neighbours outside their block looked up in the
+// edge rows, the records of write_top_records).
+// ---------------------------------------------------------------------------
+struct FsTopSide {
+  float m;          // largest |X| of the side in the block, -1 if none
+  uint32_t k;       // its bin, 0xffffffff if none
+  float2 x, xl, xr; // X at k, k-1, k+1 (xl.x / xr.x NaN: the neighbour is in another block)
+};
+struct FsTopPart {
+  FsTopSide s[2];   // left range [0, s0), right range [s1, nfft)
+};
+
+template <int N2, int TC>
+__global__ void __launch_bounds__(256) k_fs_rows_top(FusedArgs a, FsWork w) {
+  using C = FsCfg<N2, TC>;
+  constexpr int N1 = C::N1, RB = C::RB, NT2 = C::NT2, PPT = C::PPT, T = C::TR;
+  using LAY2 = typename C::LAY2;
+  // the transform's float tile first, then the block's complex spectrum
+  __shared__ __align__(16) float2 cx[LAY2::SIZE];
+  __shared__ __align__(16) float2 twl[Plan<256, 0>::TWSIZE_LDS];
+  __shared__ BlockRed red;
+  const int tid = threadIdx.x;
+  const uint32_t ci = blockIdx.x / NT2, t = blockIdx.x % NT2;
+  const int s_row = tid / (N1 / 16), jj_row = tid % (N1 / 16);
+  float2 v[PPT];
+  {
+    const float2* zr = w.Z + (uint64_t)ci * C::N + (uint32_t)((t * RB + s_row) * N1 + jj_row);
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) v[r] = ld_f2<C::STREAM_Z>(zr + r * (N1 / 16));
+  }
+  copy_tw_to_lds<256, 0, T>(a.tw_a, twl, tid);
+  __syncthreads();
+  fft256_rows_halfx<RB, T, true>(reinterpret_cast<float*>(cx), v, twl, tid);
+  __syncthreads();  // every wave is done with the float tile
+  // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]  ->  bin k2 + N2*k1
+  const uint32_t k2 = t * RB + s_row;
+  float2* edge = reinterpret_cast<float2*>(w.edge) + ((uint64_t)ci * NT2 + t) * 2 * N1;
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) cx[LAY2::idx(s_row, jj_row + 16 * r)] = v[r];
+  if (s_row == 0 || s_row == RB - 1) {
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) gp(edge)[(s_row == 0 ? 0 : N1) + jj_row + 16 * r] = v[r];
+  }
+  float lm = -1.f, rm = -1.f;
+  uint32_t li = 0xffffffffu, ri = 0xffffffffu;
+  int lr = 0, rr = 0;
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const uint32_t k = k2 + (uint32_t)N2 * (uint32_t)(jj_row + 16 * r);
+    const float m = cabs_rn(v[r]);
+    if (k < a.s0 && (m > lm || (m == lm && k < li))) { lm = m; li = k; lr = r; }
+    if (k >= a.s1 && (m > rm || (m == rm && k < ri))) { rm = m; ri = k; rr = r; }
+  }
+  float glm = lm, grm = rm;
+  uint32_t gli = li, gri = ri;
+  double d0 = 0.0, d1 = 0.0;
+  block_stats<T, true>(d0, d1, glm, gli, red);  // its closing barrier also orders the cx writes
+  block_stats<T, false>(d0, d1, grm, gri, red);
+  FsTopPart* pp = reinterpret_cast<FsTopPart*>(w.part) + (uint64_t)ci * NT2 + t;
+  const float qnan = __builtin_nanf("");
+  auto put = [&](int side, float m, uint32_t k, int r) {
+    FsTopSide sd;
+    sd.m = m;
+    sd.k = k;
+    float2 x = v[0];
+#pragma unroll
+    for (int q = 1; q < PPT; ++q) x = (q == r) ? v[q] : x;
+    sd.x = x;
+    const int k1 = jj_row + 16 * r;
+    sd.xl = s_row > 0 ? cx[LAY2::idx(s_row - 1, k1)] : make_float2(qnan, qnan);
+    sd.xr = s_row < RB - 1 ? cx[LAY2::idx(s_row + 1, k1)] : make_float2(qnan, qnan);
+    gp(&pp->s[side])[0] = sd;
+  };
+  auto none = [&](int side) {
+    FsTopSide sd;
+    sd.m = -1.f;
+    sd.k = 0xffffffffu;
+    sd.x = sd.xl = sd.xr = make_float2(0.f, 0.f);
+    gp(&pp->s[side])[0] = sd;
+  };
+  if (gli == 0xffffffffu) {
+    if (tid == 0) none(0);
+  } else if (li == gli) {
+    put(0, lm, li, lr);
+  }
+  if (gri == 0xffffffffu) {
+    if (tid == 0) none(1);
+  } else if (ri == gri) {
+    put(1, rm, ri, rr);
+  }
+}
+
+// X at a neighbour of bin k that lies in another row block (side -1: k - 1,
+// the previous block's last row; side +1: k + 1, the next block's first row;
+// a channel's bins 0 and nfft-1 have no outer neighbour, callers never ask)
+template <int N2, int RB>
+__device__ __forceinline__ float2 fs_outer_cx(const float2* edge, uint32_t k, int side) {
+  constexpr int N1 = 256, NT2 = N2 / RB;
+  const uint32_t k2 = k & (N2 - 1), k1 = k / N2;
+  if (side < 0) {
+    if (k2 > 0) return gp(edge)[((k2 / RB - 1) * 2 + 1) * N1 + k1];
+    return gp(edge)[((NT2 - 1) * 2 + 1) * N1 + (k1 + N1 - 1) % N1];  // row N2-1, column k1-1
+  }
+  if (k2 < (uint32_t)N2 - 1) return gp(edge)[((k2 / RB + 1) * 2 + 0) * N1 + k1];
+  return gp(edge)[0 * N1 + (k1 + 1) % N1];  // row 0, column k1+1
+}
+
+// The three bins around a peak, for write_peak4.
+struct TripletBins {
+  uint32_t c;
+  float2 xl, xc, xr;
+  __device__ __forceinline__ float2 operator()(uint32_t i) const { return i == c ? xc : (i < c ? xl : xr); }
+};
+
+template <int N2, int TC>
+__global__ void __launch_bounds__(64) k_fs_top_finish(FusedArgs a, FsWork w) {
+  using C = FsCfg<N2, TC>;
+  constexpr int NT2 = C::NT2, RB = C::RB;
+  static_assert(NT2 <= 64, "one row block per lane");
+  const int lane = threadIdx.x;
+  const uint32_t ci = blockIdx.x;
+  const uint64_t och = (uint64_t)w.ch0 + ci;
+  const FsTopPart* parts = reinterpret_cast<const FsTopPart*>(w.part) + (uint64_t)ci * NT2;
+  const float2* edge = reinterpret_cast<const float2*>(w.edge) + (uint64_t)ci * NT2 * 2 * 256;
+  const uint32_t N = a.nfft;
+  float m[2] = {-1.f, -1.f};
+  uint32_t k[2] = {0xffffffffu, 0xffffffffu};
+  FsTopSide sd[2];
+  if (lane < NT2) {
+    sd[0] = gp(&parts[lane].s[0])[0];
+    sd[1] = gp(&parts[lane].s[1])[0];
+    m[0] = sd[0].m;
+    k[0] = sd[0].k;
+    m[1] = sd[1].m;
+    k[1] = sd[1].k;
+  }
+  TripletBins tb[2];
+  float gm[2];
+  uint32_t gk[2];
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    gm[side] = m[side];
+    gk[side] = k[side];
+    wave_argmax(gm[side], gk[side]);
+    const uint64_t own = __ballot(lane < NT2 && k[side] == gk[side] && gk[side] != 0xffffffffu);
+    const int src = own ? __builtin_ctzll(own) : 0;
+    auto rd = [&](float f) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, f), src)); };
+    TripletBins t3;
+    t3.c = gk[side];
+    t3.xc = make_float2(rd(sd[side].x.x), rd(sd[side].x.y));
+    t3.xl = make_float2(rd(sd[side].xl.x), rd(sd[side].xl.y));
+    t3.xr = make_float2(rd(sd[side].xr.x), rd(sd[side].xr.y));
+    if (gk[side] != 0xffffffffu) {
+      if (__builtin_isnan(t3.xl.x) && t3.c > 0) t3.xl = fs_outer_cx<N2, RB>(edge, t3.c, -1);
+      if (__builtin_isnan(t3.xr.x) && t3.c < N - 1) t3.xr = fs_outer_cx<N2, RB>(edge, t3.c, 1);
+    } else {
+      // no bin compared greater than -1 (|X| NaN throughout the side): the
+      // side's first bin, as the sequential rule leaves it, with NaN values
+      const float qnan = __builtin_nanf("");
+      t3.c = side == 0 ? 0u : min(a.s1, N - 1);
+      t3.xc = t3.xl = t3.xr = make_float2(qnan, qnan);
+    }
+    tb[side] = t3;
+  }
+  if (lane == 0) {
+    MaxValue* out = a.peaks + och * (a.peak_mode == 3 ? 4 : 8);
+    if (a.peak_mode == 3) {
+      const int right = gm[1] > gm[0] ? 1 : 0;
+      write_peak4(out, tb[right], N, tb[right].c, right, a.fs);
+    } else {
+      write_peak4(out, tb[0], N, tb[0].c, 0, a.fs);
+      write_peak4(out + 4, tb[1], N, tb[1].c, 1, a.fs);
+    }
+  }
+}
+
+template <int N2>
+static uint64_t fs_top_channel_bytes() {
+  using C = FsCfg<N2, FsShape<N2>::TC>;
+  auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
+  return r((uint64_t)C::N * 8) + 256 + r((uint64_t)C::NT1 * 8) + r((uint64_t)C::NT2 * sizeof(FsTopPart)) +
+         r((uint64_t)C::NT2 * 2 * 256 * 8);
+}
+
+template <int N2>
+static hipError_t launch_fs_top_n(const FusedArgs& a, void* ws, uint32_t chunk, hipStream_t st) {
+  using C = FsCfg<N2, FsShape<N2>::TC>;
+  constexpr int TC = FsShape<N2>::TC;
+  auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
+  unsigned char* p = reinterpret_cast<unsigned char*>(ws);
+  FsWork w;
+  std::memset(&w, 0, sizeof(w));
+  w.Z = reinterpret_cast<float2*>(p);            p += r((uint64_t)chunk * C::N * 8);
+  w.ncand = reinterpret_cast<uint32_t*>(p);      p += r((uint64_t)chunk * 8);
+  w.pe = reinterpret_cast<double*>(p);           p += r((uint64_t)chunk * C::NT1 * 8);
+  w.part = reinterpret_cast<FsRowPart*>(p);      p += r((uint64_t)chunk * C::NT2 * sizeof(FsTopPart));
+  w.edge = reinterpret_cast<float*>(p);
+  for (uint32_t c0 = 0; c0 < a.batch; c0 += chunk) {
+    w.ch0 = c0;
+    w.nch = min(chunk, a.batch - c0);
+    const uint32_t groups = (w.nch + C::MC - 1) / C::MC;
+    hipLaunchKernelGGL((k_fs_cols<N2, TC>), dim3(groups * C::NT1), dim3(TC), 0, st, a, w);
+    hipLaunchKernelGGL((k_fs_rows_top<N2, TC>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);
+    hipLaunchKernelGGL((k_fs_top_finish<N2, TC>), dim3(w.nch), dim3(64), 0, st, a, w);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 template <int N2>
 static uint64_t fs_dense_channel_bytes() {
   using C = FsCfg<N2, FsShape<N2>::TC>;
@@ -562,6 +778,32 @@ uint64_t spx_fs_dense_workspace_bytes(uint32_t nfft, uint32_t chunk) {
     default: return 0;
   }
   return per * (uint64_t)chunk + 3 * 256;
+}
+
+// ONE_PEAK / TWO_PEAKS (a->peak_mode 3 / 4) of long channels on the split path.
+uint64_t spx_fs_top_workspace_bytes(uint32_t nfft, uint32_t chunk) {
+  using namespace spx;
+  uint64_t per = 0;
+  switch (nfft) {
+    case 1u << 15: per = fs_top_channel_bytes<128>(); break;
+    case 1u << 16: per = fs_top_channel_bytes<256>(); break;
+    case 1u << 17: per = fs_top_channel_bytes<512>(); break;
+    case 1u << 18: per = fs_top_channel_bytes<1024>(); break;
+    default: return 0;
+  }
+  return per * (uint64_t)chunk + 5 * 256;
+}
+
+hipError_t spx_launch_fourstep_top(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st) {
+  using namespace spx;
+  if (chunk == 0 || (a->peak_mode != 3 && a->peak_mode != 4)) return hipErrorInvalidValue;
+  switch (a->nfft) {
+    case 1u << 15: return launch_fs_top_n<128>(*a, ws, chunk, st);
+    case 1u << 16: return launch_fs_top_n<256>(*a, ws, chunk, st);
+    case 1u << 17: return launch_fs_top_n<512>(*a, ws, chunk, st);
+    case 1u << 18: return launch_fs_top_n<1024>(*a, ws, chunk, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t spx_launch_fourstep_dense(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st) {

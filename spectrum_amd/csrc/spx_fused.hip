@@ -20,6 +20,9 @@
 //   as in include/spectrum/kernels/fft_kernel_sources_rocm.hpp:228-239.
 // peak_mode 2 is the AllMaxima rule set (all_maxima_kernel_sources_rocm.hpp:57-87,
 // 205-264): no threshold, record freq_offset 0 and f = k*fs/N, count uncapped.
+// peak_modes 3 / 4 are the reference ONE_PEAK / TWO_PEAKS searches
+// (fft_kernel_sources_rocm.hpp:122-519) taken on the spectrum the kernel has
+// just computed, without storing it.
 #include "spx_fused_common.hpp"
 #include "spx_mlp.hpp"
 
@@ -177,6 +180,39 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
         }
       }
     }
+
+    if constexpr (MODE == 3) {
+      // reference ONE_PEAK / TWO_PEAKS on the spectrum in LDS: argmax of |X|
+      // over [0, half) and [nfft - half, nfft) (a.s0 = half, a.s1 = nfft -
+      // half), lowest bin on ties, then the records of write_top_records
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < PPT / RL; ++b) {
+        const int j = tid + b * T;
+        if (j < L / RL) {
+#pragma unroll
+          for (int r = 0; r < RL; ++r) lds[LAY::idx(0, j + r * (L / RL))] = v[b * RL + r];
+        }
+      }
+      __syncthreads();
+      constexpr int E = C::E;
+      float lm = -1.f, rm = -1.f;
+      uint32_t li = 0xffffffffu, ri = 0xffffffffu;
+      if (tid < C::OWN) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const uint32_t k = tid * E + e;
+          const float m = cabs_rn(lds[LAY::idx(0, k)]);
+          if (k < a.s0 && m > lm) { lm = m; li = k; }
+          if (k >= a.s1 && m > rm) { rm = m; ri = k; }
+        }
+      }
+      double d0 = 0.0, d1 = 0.0;
+      block_stats<T, true>(d0, d1, lm, li, red);
+      block_stats<T, false>(d0, d1, rm, ri, red);
+      if (tid == 0)
+        write_top_records(a.peaks + (uint64_t)ch * (a.peak_mode == 3 ? 4 : 8), LdsBins<LAY>{lds}, a, lm, li, rm, ri);
+    }
   }
 }
 
@@ -191,7 +227,9 @@ static hipError_t launch_small(const FusedArgs& a, int grid, hipStream_t st) {
       return hipGetLastError();
     }
   }
-  if (a.peak_mode != 0)
+  if (a.peak_mode >= 3)
+    hipLaunchKernelGGL((k_fused_small<L, 3>), dim3(grid), dim3(SmallCfg<L>::T), 0, st, a);
+  else if (a.peak_mode != 0)
     hipLaunchKernelGGL((k_fused_small<L, 1>), dim3(grid), dim3(SmallCfg<L>::T), 0, st, a);
   else
     hipLaunchKernelGGL((k_fused_small<L, 0>), dim3(grid), dim3(SmallCfg<L>::T), 0, st, a);

@@ -1,6 +1,7 @@
 // Device code shared by the fused kernels (one channel per workgroup) and the
 // persistent uneven-length batch kernel: input loaders, magnitude and record
-// helpers, block reductions and the whole large-N (four-step) channel body.
+// helpers, block reductions, the first-K survivor selection and the
+// ONE_PEAK / TWO_PEAKS record writers.
 #pragma once
 #include "spx_device.hpp"
 #include "spx_types.h"
@@ -146,6 +147,93 @@ __device__ __forceinline__ void store_dense(const FusedArgs& a, uint64_t i, floa
     if (a.mag_kind == SPX_MAG_ABS) gp(a.mag_out)[i] = mag_of(x) * a.mag_scale;
     else if (a.mag_kind == SPX_MAG_SQUARED) gp(a.mag_out)[i] = fmaf(x.x, x.x, x.y * x.y) * a.mag_scale;
     else store_magphase(reinterpret_cast<float2*>(a.mag_out) + i, x);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ONE_PEAK / TWO_PEAKS records (include/spectrum/kernels/fft_kernel_sources_rocm.hpp:122-519)
+// ---------------------------------------------------------------------------
+// |X| with the correctly rounded square root, as the reference's __fsqrt_rn
+__device__ __forceinline__ float cabs_rn(float2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
+
+// Bin accessors for write_peak4: a stored spectrum in global memory, or a
+// channel's spectrum in LDS (padded layout LAY).
+struct GlobalBins {
+  const float2* X;
+  __device__ __forceinline__ float2 operator()(uint32_t i) const { return X[i]; }
+};
+template <class LAY>
+struct LdsBins {
+  const float2* lds;
+  __device__ __forceinline__ float2 operator()(uint32_t i) const { return lds[LAY::idx(0, (int)i)]; }
+};
+
+// The four records of one peak at bin c: the parabolic result, then the left
+// neighbour, the centre and the right neighbour (non-circular: bins 0 and
+// nfft-1 have one neighbour, and the offset stays 0 without both).  mirror:
+// right-side peak, frequencies reported as fs - f.
+template <class BINS>
+__device__ void write_peak4(MaxValue* out, BINS X, uint32_t nfft, uint32_t c, int mirror, float fs) {
+  const float bw = fs / (float)nfft;
+  const float2 cv = X(c);
+  const float yc = cabs_rn(cv);
+  float2 lv = make_float2(0.f, 0.f), rv = make_float2(0.f, 0.f);
+  float yl = 0.f, yr = 0.f;
+  const int hl = c > 0 ? 1 : 0;
+  const int hr = c < nfft - 1 ? 1 : 0;
+  if (hl) { lv = X(c - 1); yl = cabs_rn(lv); }
+  if (hr) { rv = X(c + 1); yr = cabs_rn(rv); }
+  float fo = 0.f;
+  float rf = (float)c * bw;
+  if (hl && hr) {
+    const float den = yl - 2.0f * yc + yr;
+    if (fabsf(den) > 1e-10f) {
+      fo = 0.5f * (yl - yr) / den;
+      if (fo < -0.5f) fo = -0.5f;
+      if (fo > 0.5f) fo = 0.5f;
+      rf = ((float)c + fo) * bw;
+    }
+  }
+  if (mirror) rf = fs - rf;
+  MaxValue m;
+  m.pad = 0;
+  m.index = c; m.real = cv.x; m.imag = cv.y; m.magnitude = yc;
+  m.phase = atan2f(cv.y, cv.x) * 57.29577951f; m.freq_offset = fo; m.refined_frequency = rf;
+  gp(out)[0] = m;
+  float rfl = hl ? (float)(c - 1) * bw : 0.f;
+  if (mirror && hl) rfl = fs - rfl;
+  m.index = hl ? c - 1 : 0; m.real = lv.x; m.imag = lv.y; m.magnitude = yl;
+  m.phase = hl ? atan2f(lv.y, lv.x) * 57.29577951f : 0.f; m.freq_offset = 0.f; m.refined_frequency = rfl;
+  gp(out)[1] = m;
+  float rfc = (float)c * bw;
+  if (mirror) rfc = fs - rfc;
+  m.index = c; m.real = cv.x; m.imag = cv.y; m.magnitude = yc;
+  m.phase = atan2f(cv.y, cv.x) * 57.29577951f; m.freq_offset = 0.f; m.refined_frequency = rfc;
+  gp(out)[2] = m;
+  float rfr = hr ? (float)(c + 1) * bw : 0.f;
+  if (mirror && hr) rfr = fs - rfr;
+  m.index = hr ? c + 1 : 0; m.real = rv.x; m.imag = rv.y; m.magnitude = yr;
+  m.phase = hr ? atan2f(rv.y, rv.x) * 57.29577951f : 0.f; m.freq_offset = 0.f; m.refined_frequency = rfr;
+  gp(out)[3] = m;
+}
+
+// Winner of one side, and which record set to write: ONE_PEAK (peak_mode 3)
+// writes the larger of the two sides (the left one on ties) at out[0..3];
+// TWO_PEAKS (peak_mode 4) writes the left side at out[0..3] and the right
+// side at out[4..7].  Empty ranges keep the reference's initial bins (0 on
+// the left, nfft - half on the right, clamped into the spectrum).
+template <class BINS>
+__device__ __forceinline__ void write_top_records(MaxValue* out, BINS X, const FusedArgs& a, float lm, uint32_t li,
+                                                  float rm, uint32_t ri) {
+  const uint32_t N = a.nfft;
+  if (li == 0xffffffffu) li = 0;
+  if (ri == 0xffffffffu) ri = a.s1 < N ? a.s1 : N - 1;
+  if (a.peak_mode == 3) {
+    const int right = rm > lm ? 1 : 0;
+    write_peak4(out, X, N, right ? ri : li, right, a.fs);
+  } else {
+    write_peak4(out, X, N, li, 0, a.fs);
+    write_peak4(out + 4, X, N, ri, 1, a.fs);
   }
 }
 

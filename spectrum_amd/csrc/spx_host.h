@@ -23,6 +23,10 @@ struct spx_ctx {
   std::map<std::tuple<int, uint32_t, uint32_t>, void*> tables;  // (kind, a, b) -> device table
   void* scratch = nullptr;
   uint64_t scratch_bytes = 0;
+  // stored spectrum of spx_fused_peaks_top where its search needs one (the
+  // transform of those lengths uses `scratch` itself)
+  void* spec_buf = nullptr;
+  uint64_t spec_bytes = 0;
   const void* refiner = nullptr;  // deployed learned-refiner block (caller-owned device memory)
   void* mlp_partial = nullptr;    // per-workgroup gradient partials
   uint64_t mlp_partial_bytes = 0;

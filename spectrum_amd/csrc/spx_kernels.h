@@ -21,7 +21,8 @@ struct FusedArgs {
   float* mag_out;          // [batch][nfft] float (ABS/SQUARED) or float2 (PHASE: {|X|, arg X})
   uint32_t mag_kind;
   float mag_scale;
-  // peak search: 0 none, 1 threshold (mean + nsigma*std), 2 local maxima only
+  // peak search: 0 none, 1 threshold (mean + nsigma*std), 2 local maxima only,
+  // 3 / 4 the reference ONE_PEAK / TWO_PEAKS (s0 = search_range/2, s1 = nfft - s0)
   uint32_t peak_mode;
   float nsigma;
   uint32_t kcap;
@@ -104,6 +105,10 @@ int spx_twiddle_table_len(uint32_t L, int V);
 hipError_t spx_launch_mag_phase(const void* in, void* out, uint64_t total, hipStream_t st);
 hipError_t spx_launch_magnitude(const void* in, float* out, uint64_t total, float inv_n, int squared,
                                 hipStream_t st);
+// ONE_PEAK / TWO_PEAKS (peak_mode 3 / 4, s0 >= 1) on the split path, nfft 2^15 .. 2^18:
+// k_fs_cols, k_fs_rows_top, k_fs_top_finish per chunk of channels, no spectrum stored
+uint64_t spx_fs_top_workspace_bytes(uint32_t nfft, uint32_t chunk);
+hipError_t spx_launch_fourstep_top(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st);
 hipError_t spx_launch_peaks_top(const void* spec, void* out, uint32_t batch, uint32_t nfft, uint32_t search_range,
                                 float fs, int two_peaks, hipStream_t st);
 hipError_t spx_launch_all_maxima(const void* spec, void* out, uint32_t* counts, uint32_t batch, uint32_t nfft,

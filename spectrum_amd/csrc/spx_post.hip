@@ -3,12 +3,9 @@
 //  * ONE_PEAK / TWO_PEAKS argmax with parabolic refinement (SpectrumPostOp)
 //  * AllMaxima: strict local maxima in [s0,s1), ascending compaction with a cap
 //    (detect + Blelloch scan + compact of the reference, done in one pass per beam)
-#include "spx_device.hpp"
-#include "spx_kernels.h"
+#include "spx_fused_common.hpp"
 
 namespace spx {
-
-using MaxValue = spx_max_value;
 
 // ---------------------------------------------------------------------------
 // elementwise: reference kernels complex_to_mag_phase / complex_to_magnitude(_squared)
@@ -38,53 +35,6 @@ __global__ void __launch_bounds__(256) k_magnitude(const float2* __restrict__ in
 // Per beam: argmax of |X| over [0, sr/2) and [N - sr/2, N).  Ties resolve to the
 // lowest bin (the reference resolves to the lowest thread slot).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float cabs_rn(float2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
-
-__device__ void write_peak4(MaxValue* out, const float2* __restrict__ X, uint32_t nfft, uint32_t c, int mirror,
-                            float fs) {
-  const float bw = fs / (float)nfft;
-  const float2 cv = X[c];
-  const float yc = cabs_rn(cv);
-  float2 lv = make_float2(0.f, 0.f), rv = make_float2(0.f, 0.f);
-  float yl = 0.f, yr = 0.f;
-  const int hl = c > 0 ? 1 : 0;
-  const int hr = c < nfft - 1 ? 1 : 0;
-  if (hl) { lv = X[c - 1]; yl = cabs_rn(lv); }
-  if (hr) { rv = X[c + 1]; yr = cabs_rn(rv); }
-  float fo = 0.f;
-  float rf = (float)c * bw;
-  if (hl && hr) {
-    const float den = yl - 2.0f * yc + yr;
-    if (fabsf(den) > 1e-10f) {
-      fo = 0.5f * (yl - yr) / den;
-      if (fo < -0.5f) fo = -0.5f;
-      if (fo > 0.5f) fo = 0.5f;
-      rf = ((float)c + fo) * bw;
-    }
-  }
-  if (mirror) rf = fs - rf;
-  MaxValue m;
-  m.pad = 0;
-  m.index = c; m.real = cv.x; m.imag = cv.y; m.magnitude = yc;
-  m.phase = atan2f(cv.y, cv.x) * 57.29577951f; m.freq_offset = fo; m.refined_frequency = rf;
-  out[0] = m;
-  float rfl = hl ? (float)(c - 1) * bw : 0.f;
-  if (mirror && hl) rfl = fs - rfl;
-  m.index = hl ? c - 1 : 0; m.real = lv.x; m.imag = lv.y; m.magnitude = yl;
-  m.phase = hl ? atan2f(lv.y, lv.x) * 57.29577951f : 0.f; m.freq_offset = 0.f; m.refined_frequency = rfl;
-  out[1] = m;
-  float rfc = (float)c * bw;
-  if (mirror) rfc = fs - rfc;
-  m.index = c; m.real = cv.x; m.imag = cv.y; m.magnitude = yc;
-  m.phase = atan2f(cv.y, cv.x) * 57.29577951f; m.freq_offset = 0.f; m.refined_frequency = rfc;
-  out[2] = m;
-  float rfr = hr ? (float)(c + 1) * bw : 0.f;
-  if (mirror && hr) rfr = fs - rfr;
-  m.index = hr ? c + 1 : 0; m.real = rv.x; m.imag = rv.y; m.magnitude = yr;
-  m.phase = hr ? atan2f(rv.y, rv.x) * 57.29577951f : 0.f; m.freq_offset = 0.f; m.refined_frequency = rfr;
-  out[3] = m;
-}
-
 __global__ void __launch_bounds__(256) k_peaks_top(const float2* __restrict__ spec, MaxValue* __restrict__ out,
                                                    uint32_t batch, uint32_t nfft, uint32_t search_range, float fs,
                                                    int two_peaks) {
@@ -121,10 +71,10 @@ __global__ void __launch_bounds__(256) k_peaks_top(const float2* __restrict__ sp
     if (ri == 0xffffffffu) ri = r2 < nfft ? r2 : nfft - 1;
     if (!two_peaks) {
       const int right = rm > lm ? 1 : 0;
-      write_peak4(out + (uint64_t)beam * 4, X, nfft, right ? ri : li, right, fs);
+      write_peak4(out + (uint64_t)beam * 4, GlobalBins{X}, nfft, right ? ri : li, right, fs);
     } else {
-      write_peak4(out + (uint64_t)beam * 8, X, nfft, li, 0, fs);
-      write_peak4(out + (uint64_t)beam * 8 + 4, X, nfft, ri, 1, fs);
+      write_peak4(out + (uint64_t)beam * 8, GlobalBins{X}, nfft, li, 0, fs);
+      write_peak4(out + (uint64_t)beam * 8 + 4, GlobalBins{X}, nfft, ri, 1, fs);
     }
   }
 }
