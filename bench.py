@@ -246,37 +246,100 @@ def extra_uneven(ctx, K=64):
             "achieved_GBps": alg / (t_batch * 1e-3) / 1e9, "plan": info}
 
 
-def extra_mixed_mlp(ctx, K=32):
+def _rmse_vs_tones(recs, cnt, tf, ta, N):
+    """RMS distance (bins) from each stored peak's refined frequency to the
+    nearest true tone, over peaks within one bin of a tone."""
+    errs = []
+    for b in range(recs.shape[0]):
+        f = recs["refined_frequency"][b, : cnt[b]].astype(np.float64) * N
+        tb = np.mod(tf[b][ta[b] > 0] * N, N)
+        if len(f) and len(tb):
+            dd = (f[:, None] - tb[None, :] + N / 2) % N - N / 2
+            e = dd[np.arange(len(f)), np.argmin(np.abs(dd), axis=1)]
+            errs.append(e[np.abs(e) <= 1.0])
+    e = np.concatenate(errs) if errs else np.zeros(0)
+    return float(np.sqrt(np.mean(e * e))) if len(e) else None, int(len(e))
+
+
+def extra_mixed_mlp(ctx, K=32, reps=30):
+    """C4: 2048 x 16384 stress signal.  The FP16/BF16-MFMA form (learned and
+    parabolic refinement) against the plain FP32 form on complex64 input;
+    spectral error of the MFMA transform against FP64 (transform: FP64 FFT of
+    the stored input; end to end: of the complex64 signal before storage);
+    peak-index agreement over all channels (exact lists vs the FP32 form, and
+    the tie rule vs the oracle on the FP64 spectrum of the stored input);
+    refined-frequency RMSE against the generator's true tones."""
     from spectrum_amd import capi
     from spectrum_amd.refiner import DEFAULT_WEIGHTS, Refiner
     from oracle import oracle as orc
 
     B, N = 2048, 16384
-    x = capi.siggen(capi.GEN_DYNAMIC, 0x5EED0004, B, N)
+    x, tf, ta = capi.siggen(capi.GEN_DYNAMIC, 0x5EED0004, B, N, max_tones=2, with_tones=True)
+    fs = 1.0
     d64 = ctx.to_device(x)
-    u = capi.to_half_storage(x, capi.DTYPE_C32F16)
-    d16 = ctx.to_device(u)
     pk, ct = ctx.alloc(B * K * 32), ctx.alloc(B * 8)
-    t_fp32 = _timer_ms(ctx, lambda: ctx.fused_peaks(d64, B, N, N, pk, ct, max_peaks=K))
+    out = {"workload": f"{B} x {N}, strong tone + tone at -60 dB + noise at -80 dB, Hann, mean+6std, K={K}"}
+    t_fp32 = _timer_ms(ctx, lambda: ctx.fused_peaks(d64, B, N, N, pk, ct, max_peaks=K, window=1), reps)
+    out["fp32_c64_parabolic_ms"] = t_fp32
+    rec32 = pk.download(capi.MAXVALUE_DTYPE, (B, K))
+    cnt32 = ct.download(np.uint32, (B, 2))
+    out["fp32_rmse_bins"], _ = _rmse_vs_tones(rec32, cnt32[:, 0], tf, ta, N)
     r = Refiner(ctx).load(DEFAULT_WEIGHTS)
     r.export()
-    ctx.set_refiner(r.deployed)
-    t_mix = _timer_ms(ctx, lambda: ctx.fused_peaks_mixed(d16, B, N, pk, ct, max_peaks=K,
-                                                         refine=capi.REFINE_LEARNED))
-    t_mix_par = _timer_ms(ctx, lambda: ctx.fused_peaks_mixed(d16, B, N, pk, ct, max_peaks=K))
-    ctx.set_refiner(None)
-    # spectral error on a sample of channels
-    S = 16
-    out = ctx.alloc(S * N * 8)
-    ctx.fft_mixed(d16, out, S, N)
-    ctx.sync()
-    X = out.download(np.complex64, (S, N))
-    ref = orc.spectrum64(capi.from_half_storage(u[:S], capi.DTYPE_C32F16), N, 1)
-    err = np.linalg.norm(X - ref, axis=1) / np.linalg.norm(ref, axis=1)
-    return {"workload": f"{B} x {N}, strong tone + tone at -60 dB + noise at -80 dB, Hann, mean+6std, K={K}",
-            "fp32_c64_parabolic_ms": t_fp32, "mfma_fp16_learned_ms": t_mix, "mfma_fp16_parabolic_ms": t_mix_par,
-            "speedup_vs_fp32": t_fp32 / t_mix, "spectral_rel_l2_max": float(err.max()),
-            "spectral_rel_l2_mean": float(err.mean()), "error_reference": "FP64 FFT of the fp16-stored input"}
+    S = 64  # channels whose spectra are compared against FP64
+    for name, dt in (("fp16", capi.DTYPE_C32F16), ("bf16", capi.DTYPE_C32BF16)):
+        u = capi.to_half_storage(x, dt)
+        d16 = ctx.to_device(u)
+        ctx.set_refiner(r.deployed)
+        t_l = _timer_ms(ctx, lambda: ctx.fused_peaks_mixed(d16, B, N, pk, ct, dtype=dt, max_peaks=K, window=1,
+                                                           refine=capi.REFINE_LEARNED), reps)
+        recl = pk.download(capi.MAXVALUE_DTYPE, (B, K))
+        cntl = ct.download(np.uint32, (B, 2))
+        ctx.set_refiner(None)
+        t_p = _timer_ms(ctx, lambda: ctx.fused_peaks_mixed(d16, B, N, pk, ct, dtype=dt, max_peaks=K, window=1), reps)
+        recp = pk.download(capi.MAXVALUE_DTYPE, (B, K))
+        cntp = ct.download(np.uint32, (B, 2))
+        # the FP32 form on the same stored input: agreement judged on one storage
+        ctx.fused_peaks(d16, B, N, N, pk, ct, dtype=dt, max_peaks=K, window=1)
+        ctx.sync()
+        recf = pk.download(capi.MAXVALUE_DTYPE, (B, K))
+        cntf = ct.download(np.uint32, (B, 2))
+        same_fp32 = sum(list(recp[b, : cntp[b, 0]]["index"]) == list(recf[b, : cntf[b, 0]]["index"])
+                        for b in range(B))
+        xq = capi.from_half_storage(u, dt)
+        Xq = orc.spectrum64(xq, N, 1)
+        ok_oracle = 0
+        for b in range(B):
+            recs, found, g, stats, mags = orc.fused_peaks(Xq[b], fs, nsigma=6.0, kcap=K)
+            ok, _ = orc.compare_peak_lists(recp[b, : cntp[b, 0]]["index"], recs["index"], mags, stats[2],
+                                           2e-3 * mags.max(), K)
+            ok_oracle += int(ok)
+        sp = ctx.alloc(S * N * 8)
+        ctx.fft_mixed(d16, sp, S, N, window=1, dtype=dt)
+        ctx.sync()
+        X = sp.download(np.complex64, (S, N))
+        e_t = np.linalg.norm(X - Xq[:S], axis=1) / np.linalg.norm(Xq[:S], axis=1)
+        ref = orc.spectrum64(x[:S], N, 1)
+        e_e = np.linalg.norm(X - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        e_s = np.linalg.norm(xq[:S] - x[:S], axis=1) / np.linalg.norm(x[:S], axis=1)
+        rl, nl = _rmse_vs_tones(recl, cntl[:, 0], tf, ta, N)
+        rp, _ = _rmse_vs_tones(recp, cntp[:, 0], tf, ta, N)
+        out[name] = {"mfma_learned_ms": t_l, "mfma_parabolic_ms": t_p, "speedup_vs_fp32": t_fp32 / t_l,
+                     "speedup_vs_fp32_parabolic": t_fp32 / t_p,
+                     "transform_rel_l2_max": float(e_t.max()), "end_to_end_rel_l2_max": float(e_e.max()),
+                     "storage_rel_l2_max": float(e_s.max()),
+                     "peak_lists_equal_fp32_form": f"{same_fp32}/{B}",
+                     "peak_lists_tie_rule_vs_oracle": f"{ok_oracle}/{B}",
+                     "learned_rmse_bins": rl, "parabolic_rmse_bins": rp, "rmse_peaks": nl}
+        for buf in (d16, sp):
+            buf.free()
+    out["speedup_vs_fp32"] = out["fp16"]["speedup_vs_fp32"]
+    out["error_reference"] = ("transform: FP64 FFT of the stored input over " + str(S) + " channels; end to end: "
+                              "FP64 FFT of the complex64 signal before storage")
+    mf = os.path.join(ROOT, "profiles", "mixed16k_counters.json")
+    if os.path.exists(mf):
+        out["mfma_util"] = json.load(open(mf)).get("mfma_util")
+    return out
 
 
 def extra_refiner(ctx, K=64):
@@ -321,6 +384,10 @@ def main():
     ap.add_argument("--kpeaks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--extra", choices=["host_stream", "uneven", "mixed_mlp", "refiner"],
+                    help="run only this extra workload (no headline, no CPU baseline) and print its JSON; "
+                         "for profiling one workload")
+    ap.add_argument("--reps", type=int, default=30, help="timed repetitions of each --extra form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, copy the input once device-to-device (a plain copy of known size, "
                          "used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -334,6 +401,15 @@ def main():
 
     B, N, K = args.batch, args.nfft, args.kpeaks
     ctx = capi.Context(0)
+    if args.extra:
+        if args.extra == "host_stream":
+            res = extra_host(ctx, capi.siggen(capi.GEN_MULTI_TONE, 0x5EED0002, B, N), K)
+        elif args.extra == "mixed_mlp":
+            res = extra_mixed_mlp(ctx, reps=args.reps)
+        else:
+            res = {"uneven": extra_uneven, "refiner": extra_refiner}[args.extra](ctx)
+        print(json.dumps({args.extra: res}), flush=True)
+        return
     info = ctx.info()
     print(f"[bench] device {info}", file=sys.stderr, flush=True)
     x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED0002, B, N)
