@@ -27,7 +27,9 @@
 // W_16384^e come from two 128-entry LDS tables (e = 128 e_hi + e_lo), the
 // stage-2 twiddles from a 1024-entry LDS table; every table is generated in
 // FP64 on the host and rounded once.  The window is evaluated in FP32 from
-// its cosine coefficients (hardware cosine in revolutions).
+// its cosine coefficients: a three-term cosine recurrence over each lane's
+// samples, started from the hardware cosine (in revolutions), with the
+// pre-scale folded into the coefficients.
 #include "spx_fused_common.hpp"
 #include "spx_mlp.hpp"
 
@@ -61,6 +63,7 @@ constexpr int TABLE_BYTES = W32_OFF + W32_BYTES;
 }  // namespace mx
 
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 // D += W * x for one 16x16x32 block; W = whi (+ wlo).
 template <int CT>
@@ -142,16 +145,28 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
   auto adr = [](uint32_t k) -> int { return (int)(k + (k >> 5)); };
   auto xat = [&](uint32_t k) -> float2 { const int q = adr(k); return make_float2(xre[q], xim[q]); };
   const uint32_t k0 = (uint32_t)tid * E;
+  const bool full = (a.s0 == 0 && a.s1 == (uint32_t)N);  // uniform
+  // one sweep: sums and argmax over the range, and the bins that are local
+  // maxima (before the threshold is known); the threshold test then
+  // re-reads only those bins
   float fs1 = 0.f, fs2 = 0.f, bm = -1.0f;
   uint32_t bk = 0xffffffffu;
+  uint32_t lmax = 0;
+  {
+    float yl = mag_of(xat((k0 + N - 1) % N));
+    float yc = mag_of(xat(k0));
 #pragma unroll 8
-  for (int e = 0; e < E; ++e) {
-    const uint32_t k = k0 + e;
-    const float m = mag_of(xat(k));
-    if (in_range(k, a)) {
-      fs1 += m;
-      fs2 = fmaf(m, m, fs2);
-      if (m > bm) { bm = m; bk = k; }
+    for (int e = 0; e < E; ++e) {
+      const uint32_t k = k0 + e;
+      const float yr = mag_of(xat((k + 1) % N));
+      if (full || in_range(k, a)) {
+        fs1 += yc;
+        fs2 = fmaf(yc, yc, fs2);
+        if (yc > bm) { bm = yc; bk = k; }
+        if (yc > yl && yc > yr) lmax |= 1u << e;
+      }
+      yl = yc;
+      yc = yr;
     }
   }
   double s1 = fs1, s2 = fs2;
@@ -161,21 +176,14 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
   const double cnt = (double)(a.s1 - a.s0);
   float mu, sd;
   const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
-  // second pass re-reads the magnitudes (LDS) instead of keeping 32 of them live
-  float yl = mag_of(xat((k0 + N - 1) % N));
-  float yc = mag_of(xat(k0));
   uint32_t flags = 0;
   int cpk = 0;
-#pragma unroll 8
-  for (int e = 0; e < E; ++e) {
-    const uint32_t k = k0 + e;
-    const float yr = mag_of(xat((k + 1) % N));
-    if (in_range(k, a) && yc > yl && yc > yr && yc > thr) {
+  for (uint32_t q = lmax; q; q &= q - 1) {
+    const int e = __builtin_ctz(q);
+    if (mag_of(xat(k0 + e)) > thr) {
       flags |= 1u << e;
       ++cpk;
     }
-    yl = yc;
-    yc = yr;
   }
   int total;
   const int off = block_excl_scan<T, false>(cpk, total, red);
@@ -256,16 +264,18 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
     for (int t = 0; t < 8; ++t)
 #pragma unroll
       for (int j = 0; j < 8; ++j) raw[t][j] = (uint32_t)opaque_v((int)raw[t][j]);
-    // ---- per-channel power-of-two pre-scale from the largest component
+    // ---- per-channel power-of-two pre-scale from the largest component:
+    // |re| and |im| as 16-bit patterns (sign cleared) order like their values
+    // for fp16 and bf16 alike, so one packed integer max per word
     const float isc = (a.in_scale != nullptr) ? gp(a.in_scale)[ch] : 1.0f;
-    float mx_ = 0.f;
+    u16x2 mxp = {0, 0};
 #pragma unroll
     for (int t = 0; t < 8; ++t)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const float2 v = unpack_in<DT>(raw[t][j]);
-        mx_ = fmaxf(mx_, fmaxf(fabsf(v.x), fabsf(v.y)));
-      }
+      for (int j = 0; j < 8; ++j)
+        mxp = __builtin_elementwise_max(mxp, __builtin_bit_cast(u16x2, raw[t][j] & 0x7fff7fffu));
+    const uint32_t mxb = (uint32_t)max(mxp[0], mxp[1]);
+    float mx_ = DT == SPX_DTYPE_C32F16 ? (float)__builtin_bit_cast(_Float16, (uint16_t)mxb) : __uint_as_float(mxb << 16);
     mx_ = wave_maxf(mx_);
     if (l == 0) red.m[w] = mx_;
     __syncthreads();
@@ -287,26 +297,37 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
         w0l = gp(W16)[128 + 0 * 64 + l];
         w1l = gp(W16)[128 + 1 * 64 + l];
       }
-      // window by rotation: the lane's 8 samples of block t are n0 + 1024 j, so
-      // e^(i theta_j) = e^(i theta_0) * e^(i j delta) (FP32, 7 steps)
+      // window times pre-scale, w(n) s = c0 s - c1 s cos(th) + c2 s cos(2 th),
+      // th = 2 pi n / (N - 1), as the polynomial P(cos th) = (c0 - c2) s +
+      // cos th (-c1 s + 2 c2 s cos th); the lane's 8 samples of block t are
+      // n0 + 1024 j, so cos th follows the recurrence
+      // cos((j+1) d) = 2 cos(d) cos(j d) - cos((j-1) d) (FP32, 7 steps)
       const float winv = 1.0f / (float)(N - 1);
       const float dr = 1024.0f * winv;  // revolutions per step
-      const float dc = __builtin_amdgcn_cosf(dr), ds = __builtin_amdgcn_sinf(dr);
+      const float two_cd = 2.0f * __builtin_amdgcn_cosf(dr);
+      const bool win = a.wkind != 0;
+      const float p0 = win ? (a.wc0 - a.wc2) * s_in : s_in;
+      const float p1 = win ? -a.wc1 * s_in : 0.0f;
+      const float p2 = win ? 2.0f * a.wc2 * s_in : 0.0f;
+      // the lane's component: re (lanes g < 2) or im (g >= 2) of each word
+      const uint32_t sh = (g < 2) ? 0u : 16u;
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         const int n2 = 16 * (w + 8 * t) + c;
         const float r0 = (float)((8 * (g & 1)) * 1024 + n2) * winv;
-        float cr = __builtin_amdgcn_cosf(r0), ci = __builtin_amdgcn_sinf(r0);
+        float cp = __builtin_amdgcn_cosf(r0 - dr), cc = __builtin_amdgcn_cosf(r0);
         f16x8 b;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          const float2 v = unpack_in<DT>(raw[t][j]);
-          float wn = 1.0f;
-          if (a.wkind != 0) wn = a.wc0 - a.wc1 * cr + a.wc2 * fmaf(2.0f * cr, cr, -1.0f);
-          b[j] = (_Float16)(((g < 2) ? v.x : v.y) * (wn * s_in));
-          const float nr = fmaf(cr, dc, -ci * ds);
-          ci = fmaf(cr, ds, ci * dc);
-          cr = nr;
+          const float ws = fmaf(fmaf(p2, cc, p1), cc, p0);
+          const uint32_t u = raw[t][j] >> sh;
+          float comp;
+          if constexpr (DT == SPX_DTYPE_C32F16) comp = (float)__builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu));
+          else comp = __uint_as_float(u << 16);
+          b[j] = (_Float16)__builtin_fmaf(comp, ws, 0.0f);  // one v_fma_mix (fp16 in, FP32 math, fp16 out)
+          const float cn = fmaf(two_cd, cc, -cp);
+          cp = cc;
+          cc = cn;
         }
         const f32x4 z = {0.f, 0.f, 0.f, 0.f};
         const f32x4 dre = Mma<CT>::run(w0h, w0l, b, z);
@@ -416,7 +437,6 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
           }
       }
       __syncthreads();
-      // the spectrum planes leave 6 KiB at the end of the stage-2 buffer for the refiner staging
       mixed_peaks<LEARN>(a, ch, xre, xim, red);
       __syncthreads();  // the planes are overwritten by the next channel
     }
