@@ -272,32 +272,8 @@ __global__ void __launch_bounds__(256) k_mlp_refine(const unsigned char* dep, co
   const uint32_t n = min(gp(counts)[2 * ch], kcap);
   if (n == 0) return;
   const MlpRegs w = mlp_load(dep);
-  const mlp_h8* src = reinterpret_cast<const mlp_h8*>(feat + (uint64_t)ch * kcap * 16);
-  MaxValue* rec = peaks + (uint64_t)ch * kcap;
-  for (uint32_t r0 = 0; r0 < n; r0 += 16) {
-    // rows r0 .. r0+15 as 32 half-rows of 16 bytes, zeros past the last peak
-    if (l < 32) {
-      const uint32_t row = r0 + (uint32_t)(l >> 1);
-      mlp_h8 v = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (row < n) v = gp(src)[2 * row + (l & 1)];
-      reinterpret_cast<mlp_h8*>(fbuf[wv])[l] = v;
-    }
-    __builtin_amdgcn_s_waitcnt(0);  // LDS writes of this wave complete before mlp_rows16 reads them
-    __builtin_amdgcn_wave_barrier();
-    mlp_rows16(fbuf[wv], hbuf[wv], res[wv], w);
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    if (l < 16 && r0 + l < n) {
-      const float dl = res[wv][l];
-      if (fabsf(dl) <= 0.5f) {
-        MaxValue* r = rec + r0 + l;
-        const uint32_t k = gp(&r->index)[0];
-        gp(&r->freq_offset)[0] = dl;
-        gp(&r->refined_frequency)[0] = ((float)k + dl) * bin_hz;
-      }
-    }
-    __builtin_amdgcn_wave_barrier();  // res and fbuf are rewritten by the next pass
-  }
+  mlp_refine_rows(w, feat + (uint64_t)ch * kcap * 16, peaks + (uint64_t)ch * kcap, n, bin_hz, fbuf[wv], hbuf[wv],
+                  res[wv]);
 }
 
 }  // namespace spx

@@ -149,6 +149,42 @@ __device__ __forceinline__ void mlp_rows16(const _Float16* feat, _Float16* hbuf,
   }
 }
 
+// Learned refinement of one channel's n stored records by one wave, 16 per
+// MFMA pass, from the feature rows the peak kernel wrote
+// (mlp_store_features): a prediction inside [-0.5, 0.5] replaces the
+// record's parabolic offset and refined frequency; anything else (outside,
+// not finite) keeps them.  fbuf: 16 x 16 halves, hbuf: 16 x 64 halves, res:
+// 16 floats of this wave's LDS.
+__device__ __forceinline__ void mlp_refine_rows(const MlpRegs& w, const _Float16* feat, spx_max_value* rec, uint32_t n,
+                                                float bin_hz, _Float16* fbuf, _Float16* hbuf, float* res) {
+  const int l = threadIdx.x & 63;
+  const mlp_h8* src = reinterpret_cast<const mlp_h8*>(feat);
+  for (uint32_t r0 = 0; r0 < n; r0 += 16) {
+    // rows r0 .. r0+15 as 32 half-rows of 16 bytes, zeros past the last peak
+    if (l < 32) {
+      const uint32_t row = r0 + (uint32_t)(l >> 1);
+      mlp_h8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (row < n) v = gp(src)[2 * row + (l & 1)];
+      reinterpret_cast<mlp_h8*>(fbuf)[l] = v;
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // LDS writes of this wave complete before mlp_rows16 reads them
+    __builtin_amdgcn_wave_barrier();
+    mlp_rows16(fbuf, hbuf, res, w);
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (l < 16 && r0 + l < n) {
+      const float dl = res[l];
+      if (fabsf(dl) <= 0.5f) {
+        spx_max_value* r = rec + r0 + l;
+        const uint32_t k = gp(&r->index)[0];
+        gp(&r->freq_offset)[0] = dl;
+        gp(&r->refined_frequency)[0] = ((float)k + dl) * bin_hz;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // res and fbuf are rewritten by the next pass
+  }
+}
+
 // Same, loading the weights itself (callers with nothing to overlap).
 __device__ __forceinline__ void mlp_rows16(const _Float16* feat, _Float16* hbuf, float* out,
                                            const unsigned char* wts) {
