@@ -21,7 +21,8 @@ with --no-extras):
              the plain FP32 form (complex64, parabolic), with its spectral
              error against FP64;
   refiner    4096 x 8192 fractional tones: learned vs parabolic refinement
-             (time per launch, frequency RMSE against the true tones).
+             (time per launch, frequency RMSE against the true tones);
+  refiner_split  the same at the headline shape, 1024 x 65536 (split path).
 """
 import argparse
 import ctypes as C
@@ -372,6 +373,32 @@ def extra_refiner(ctx, K=64):
     return out
 
 
+def extra_refiner_split(ctx, K=64, reps=20):
+    """Learned refinement on the split path at the headline shape: 1024 x 65536
+    complex64 fractional-bin tones (1-16 per channel), Hann, mean+6std, K=64;
+    time per call and frequency RMSE against the true tones, learned vs
+    parabolic."""
+    from spectrum_amd import capi
+    from spectrum_amd.refiner import DEFAULT_WEIGHTS, Refiner
+
+    B, N = 1024, 65536
+    x, tf, ta = capi.siggen(capi.GEN_FRACTIONAL, 0x5EED0006, B, N, max_tones=16, with_tones=True)
+    d = ctx.to_device(x)
+    pk, ct = ctx.alloc(B * K * 32), ctx.alloc(B * 8)
+    r = Refiner(ctx).load(DEFAULT_WEIGHTS)
+    r.export()
+    out = {"workload": f"{B} x {N} complex64, 1-16 fractional-bin tones, Hann, mean+6std, K={K} (split path)"}
+    for name, refine in (("learned", capi.REFINE_LEARNED), ("parabolic", capi.REFINE_PARABOLIC)):
+        ctx.set_refiner(r.deployed if refine == capi.REFINE_LEARNED else None)
+        out[f"{name}_ms"] = _timer_ms(ctx, lambda: ctx.fused_peaks(d, B, N, N, pk, ct, max_peaks=K, refine=refine),
+                                      reps)
+        recs = pk.download(capi.MAXVALUE_DTYPE, (B, K))
+        cnt = ct.download(np.uint32, (B, 2))[:, 0]
+        out[f"{name}_rmse_bins"], out["rmse_peaks"] = _rmse_vs_tones(recs, cnt, tf, ta, N)
+    ctx.set_refiner(None)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,7 +411,7 @@ def main():
     ap.add_argument("--kpeaks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extra", choices=["host_stream", "uneven", "mixed_mlp", "refiner"],
+    ap.add_argument("--extra", choices=["host_stream", "uneven", "mixed_mlp", "refiner", "refiner_split"],
                     help="run only this extra workload (no headline, no CPU baseline) and print its JSON; "
                          "for profiling one workload")
     ap.add_argument("--reps", type=int, default=30, help="timed repetitions of each --extra form")
@@ -407,7 +434,7 @@ def main():
         elif args.extra == "mixed_mlp":
             res = extra_mixed_mlp(ctx, reps=args.reps)
         else:
-            res = {"uneven": extra_uneven, "refiner": extra_refiner}[args.extra](ctx)
+            res = {"uneven": extra_uneven, "refiner": extra_refiner, "refiner_split": extra_refiner_split}[args.extra](ctx)
         print(json.dumps({args.extra: res}), flush=True)
         return
     info = ctx.info()
@@ -487,7 +514,8 @@ def main():
         except Exception as e:
             out["host_stream"] = {"error": repr(e)}
         del d_in, x
-        for name, fn in (("uneven", extra_uneven), ("mixed_mlp", extra_mixed_mlp), ("refiner", extra_refiner)):
+        for name, fn in (("uneven", extra_uneven), ("mixed_mlp", extra_mixed_mlp), ("refiner", extra_refiner),
+                         ("refiner_split", extra_refiner_split)):
             print(f"[bench] {name} ...", file=sys.stderr, flush=True)
             try:
                 out[name] = fn(ctx)

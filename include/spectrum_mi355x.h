@@ -71,7 +71,10 @@ typedef struct spx_ctx spx_ctx;
  * work at once take a context each; several contexts may share a device. */
 int spx_create(int device, spx_ctx** out);
 int spx_destroy(spx_ctx* ctx);
-/* borrow a caller stream (hipStream_t); NULL selects the context's own stream */
+/* borrow a caller stream (hipStream_t); NULL selects the context's own stream.
+ * Work already queued on the previous stream is ordered before anything
+ * queued on the new one (the new stream waits on an event of the old), so
+ * the context's workspace is never shared by calls in flight. */
 int spx_set_stream(spx_ctx* ctx, void* hip_stream);
 void* spx_get_stream(spx_ctx* ctx);
 int spx_synchronize(spx_ctx* ctx);
@@ -172,8 +175,8 @@ int spx_fused_peaks(spx_ctx* ctx, const spx_fused_desc* desc);
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
  * workspace is a fixed 64 MiB device array per GPU, held until the process
  * ends; long channels (nfft >= 32768) run in launch groups that fit it, and a
- * learned-refinement call needs batch * max_peaks * 32 bytes of it (more:
- * SPX_ECAP).  The context's tables for the call's nfft, n_point and window
+ * learned-refinement call on the short path needs batch * max_peaks * 32
+ * bytes of it (more: SPX_ECAP).  The context's tables for the call's nfft, n_point and window
  * must exist already (any earlier spx_fused_peaks call with them creates
  * them; otherwise SPX_EINVAL, nothing allocated).  Results equal
  * spx_fused_peaks' bit for bit.  All calls on one GPU share the array:
@@ -216,10 +219,12 @@ int spx_host_free(spx_ctx* ctx, void* ptr);
  * GPU (own forward/backward and Adam kernels, FP32 master weights); the
  * deployed block holds FP16 weights for MFMA inference with FP32
  * accumulation.  With spx_set_refiner, spx_fused_peaks accepts
- * refine = SPX_REFINE_LEARNED (threshold search, nfft 1024..16384): the
+ * refine = SPX_REFINE_LEARNED (threshold search, nfft 1024..262144): the
  * fused kernel stores the parabolic records and the features of each stored
- * peak, and a second MFMA kernel replaces the offset wherever the network's
- * prediction lies in [-0.5, 0.5]. */
+ * peak (on the split path, nfft >= 32768, the row kernel keeps each
+ * candidate's complex neighbourhood and the finish kernel writes the
+ * features), and a second MFMA kernel replaces the offset wherever the
+ * network's prediction lies in [-0.5, 0.5]. */
 int spx_mlp_param_count(void);    /* 4929 FP32 master parameters */
 int spx_mlp_deployed_size(void);  /* bytes of the deployed block */
 int spx_mlp_init(spx_ctx* ctx, float* params, uint64_t seed);
@@ -250,7 +255,8 @@ int spx_mlp_dataset(spx_ctx* ctx, const void* spec, const float* mag16, const sp
  * operands are fp16 (compute F16) or split bf16 hi+lo (compute BF16).
  * spx_fft_mixed writes the FP32 spectrum (complex64); spx_fused_peaks_mixed
  * runs the spx_fused_peaks rules on it (desc->dtype fp16/bf16, n_point =
- * nfft = 16384; refine NONE or PARABOLIC). */
+ * nfft = 16384; refine NONE, PARABOLIC or LEARNED (the refiner set with
+ * spx_set_refiner, one extra launch)). */
 #define SPX_COMPUTE_F16 0
 #define SPX_COMPUTE_BF16 1
 int spx_fft_mixed(spx_ctx* ctx, const void* in, void* out_c64, uint32_t batch, uint32_t nfft, uint32_t window,
@@ -293,6 +299,8 @@ typedef struct spx_batch_desc {
   uint32_t flags;             /* SPX_BATCH_* */
 } spx_batch_desc;
 
+/* A plan keeps pointers to its context (stream, twiddle and window tables):
+ * destroy every plan before the context it was created on. */
 int spx_batch_plan_create(spx_ctx* ctx, const spx_batch_desc* desc, spx_batch_plan** out);
 int spx_batch_execute(spx_batch_plan* plan);
 int spx_batch_plan_destroy(spx_batch_plan* plan);

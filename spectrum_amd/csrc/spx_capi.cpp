@@ -252,13 +252,17 @@ int learned_refine(spx_ctx* c, const FusedArgs& a) {
 }
 
 // Channels per launch group of the split long-channel peak search: bounds the
-// workspace (Z, candidate lists, edge rows) to about 2 GiB.
-uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch) {
+// workspace (Z, candidate lists, edge rows) to about 2 GiB (6 GiB with the
+// learned-refinement buffers).
+uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch, int learn, uint32_t kcap) {
   const char* env = std::getenv("SPX_FS_CHUNK");
   uint32_t ch = env ? (uint32_t)std::strtoul(env, nullptr, 10) : 0;
   if (ch == 0) {
-    const uint64_t per = spx_fs_workspace_bytes(nfft, 1);
-    ch = (uint32_t)std::max<uint64_t>(64, (2ull << 30) / std::max<uint64_t>(per, 1));
+    // learned refinement keeps each candidate's neighbourhood (CAND_CAP x 32 B
+    // per channel): a larger budget keeps a 1024-channel batch in one group
+    const uint64_t per = spx_fs_workspace_bytes_ex(nfft, 1, learn, kcap);
+    const uint64_t budget = learn ? (6ull << 30) : (2ull << 30);
+    ch = (uint32_t)std::max<uint64_t>(64, budget / std::max<uint64_t>(per, 1));
   }
   return std::min(ch, batch);
 }
@@ -297,15 +301,18 @@ int run_fused(spx_ctx* c, FusedArgs& a) {
       SPX_HIP(c, spx_launch_fourstep_dense(&a, c->scratch, chunk, c->stream));
       return SPX_OK;
     }
-    uint32_t chunk = fourstep_chunk(a.nfft, a.batch);
+    // learned refinement: the refiner's features and the refiner run per chunk
+    const int learn = (a.refine == 2 && a.mlp != nullptr) ? 1 : 0;
+    auto wsb = [&](uint32_t n) { return spx_fs_workspace_bytes_ex(a.nfft, n, learn, a.kcap); };
+    uint32_t chunk = fourstep_chunk(a.nfft, a.batch, learn, a.kcap);
     if (c->ws_fixed) {
       // as many channels per launch group as the fixed workspace holds
-      const uint64_t per = spx_fs_workspace_bytes(a.nfft, 1) - spx_fs_workspace_bytes(a.nfft, 0);
-      const uint64_t fit = (c->ws_fixed_bytes - spx_fs_workspace_bytes(a.nfft, 0)) / per;
-      if (fit == 0) return ensure_scratch(c, spx_fs_workspace_bytes(a.nfft, 1));  // reports the shortfall
+      const uint64_t per = wsb(1) - wsb(0);
+      const uint64_t fit = c->ws_fixed_bytes > wsb(0) ? (c->ws_fixed_bytes - wsb(0)) / per : 0;
+      if (fit == 0) return ensure_scratch(c, wsb(1));  // reports the shortfall
       chunk = (uint32_t)std::min<uint64_t>(fit, a.batch);
     }
-    if ((rc = ensure_scratch(c, spx_fs_workspace_bytes(a.nfft, chunk)))) return rc;
+    if ((rc = ensure_scratch(c, wsb(chunk)))) return rc;
     SPX_HIP(c, spx_launch_fourstep(&a, c->ws_fixed ? c->ws_fixed : c->scratch, chunk, c->stream));
     return SPX_OK;
   }
@@ -397,7 +404,18 @@ int spx_destroy(spx_ctx* c) {
 
 int spx_set_stream(spx_ctx* c, void* s) {
   if (!c) return SPX_EINVAL;
-  c->stream = s ? (hipStream_t)s : c->own_stream;
+  hipStream_t ns = s ? (hipStream_t)s : c->own_stream;
+  if (ns == c->stream) return SPX_OK;
+  // calls still in flight on the old stream share the context's workspace
+  // with the next ones: the new stream waits for them
+  SPX_HIP(c, hipSetDevice(c->device));
+  hipEvent_t e;
+  SPX_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  hipError_t r = hipEventRecord(e, c->stream);
+  if (r == hipSuccess) r = hipStreamWaitEvent(ns, e, 0);
+  (void)hipEventDestroy(e);
+  SPX_HIP(c, r);
+  c->stream = ns;
   return SPX_OK;
 }
 
@@ -606,8 +624,8 @@ int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a) {
   if (d->refine > SPX_REFINE_LEARNED) return fail(c, SPX_EINVAL, "unknown refine mode");
   if (d->refine == SPX_REFINE_LEARNED) {
     if (!c->refiner) return fail(c, SPX_EINVAL, "learned refinement requested but no refiner is set (spx_set_refiner)");
-    if (d->peak_mode != SPX_PEAKS_THRESHOLD || d->nfft < 1024 || d->nfft > 16384)
-      return fail(c, SPX_ENOTSUP, "learned refinement: threshold search and nfft 1024..16384 only");
+    if (d->peak_mode != SPX_PEAKS_THRESHOLD || d->nfft < 1024 || d->nfft > (1u << 18))
+      return fail(c, SPX_ENOTSUP, "learned refinement: threshold search and nfft 1024..262144 only");
   }
   uint32_t s0 = d->search_start, s1 = d->search_end;
   if (d->peak_mode == SPX_PEAKS_ALL_MAXIMA) {

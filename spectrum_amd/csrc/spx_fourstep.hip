@@ -12,6 +12,7 @@
 // no waits), and several resident workgroups per CU overlap one workgroup's
 // loads with another's arithmetic.
 #include "spx_fs_common.hpp"
+#include "spx_mlp.hpp"
 
 #include <cstring>
 
@@ -59,15 +60,12 @@ __device__ __forceinline__ void fs_cols_multi(const FusedArgs& a, const FsWork& 
   for (int m = 0; m < MC; ++m) {
     const uint32_t ci = cg * MC + m;
     if (ci >= w.nch) break;  // uniform over the workgroup
-    uint64_t ich = (uint64_t)w.ch0 + ci;
-    const char* cb;
-    if (w.chan != nullptr) {
-      cb = reinterpret_cast<const char*>(a.in) + gp(w.inoff)[gp(w.chan)[ich]] * ES;
-      ich = 0;
-    } else {
-      cb = reinterpret_cast<const char*>(a.in) + ich * (uint64_t)a.n_point * ES;
-    }
-    const float sc = (a.in_scale != nullptr) ? gp(a.in_scale)[ich] : 1.0f;
+    const uint64_t ich = (uint64_t)w.ch0 + ci;
+    // the caller's channel id: through the indirection when there is one
+    const uint64_t uch = (w.chan != nullptr) ? (uint64_t)gp(w.chan)[ich] : ich;
+    const char* cb = (w.chan != nullptr) ? reinterpret_cast<const char*>(a.in) + gp(w.inoff)[uch] * ES
+                                         : reinterpret_cast<const char*>(a.in) + ich * (uint64_t)a.n_point * ES;
+    const float sc = (a.in_scale != nullptr) ? gp(a.in_scale)[uch] : 1.0f;
     float2 v[PPT];
 #pragma unroll
     for (int r = 0; r < PPT; ++r) {
@@ -121,12 +119,23 @@ k_fs_cols(FusedArgs a, FsWork w) {
 
 
 // ---------------------------------------------------------------------------
-template <int N2, int TC>
+// LEARN: also the complex neighbourhood of every candidate (w.candx) and the
+// complex rows 0, 1, RB-2, RB-1 of the block (w.edgex), for the refiner's
+// features in the finish.
+template <int N2, int TC, bool LEARN = false>
 __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
   using C = FsCfg<N2, TC>;
   constexpr int N1 = C::N1, RB = C::RB, NT2 = C::NT2, PPT = C::PPT, T = C::TR;
   using LAY2 = typename C::LAY2;
-  __shared__ __align__(16) float ldsf[LAY2::SIZE];
+  // LEARN: the transform's float tile aliases the complex tile, which then
+  // holds X of the block (|X| is recomputed from it, bit for bit the same)
+  __shared__ __align__(16) float ldsf[LEARN ? 1 : LAY2::SIZE];
+  __shared__ __align__(16) float2 cxt[LEARN ? LAY2::SIZE : 1];
+  float* const tile = LEARN ? reinterpret_cast<float*>(cxt) : ldsf;
+  auto mt = [&](int row, int k1) -> float {
+    if constexpr (LEARN) return mag_of(cxt[LAY2::idx(row, k1)]);
+    else return ldsf[LAY2::idx(row, k1)];
+  };
   __shared__ __align__(16) float2 twl[Plan<256, 0>::TWSIZE_LDS];
   __shared__ BlockRed red;
   const int tid = threadIdx.x;
@@ -143,9 +152,20 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
   const float lb = bnd.lb;
   copy_tw_to_lds<256, 0, T>(a.tw_a, twl, tid);
   __syncthreads();
-  fft256_rows_halfx<RB, T, true>(ldsf, v, twl, tid);
+  fft256_rows_halfx<RB, T, true>(tile, v, twl, tid);
   // v[r] = X[k2 = t*RB + s_row][k1 = jj_row + 16 r]  ->  bin k2 + N2*k1 (ascending in r)
   const uint32_t k2 = t * RB + s_row;
+  if constexpr (LEARN) {
+    __syncthreads();  // every wave is done with the float tile
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) cxt[LAY2::idx(s_row, jj_row + 16 * r)] = v[r];
+    const int er = s_row < 2 ? s_row : (s_row >= RB - 2 ? s_row - (RB - 4) : -1);
+    if (er >= 0) {
+      float2* ex = w.edgex + (((uint64_t)ci * NT2 + t) * 4 + er) * N1;
+#pragma unroll
+      for (int r = 0; r < PPT; ++r) gp(ex)[jj_row + 16 * r] = v[r];
+    }
+  }
   float m[PPT];
   float fs1 = 0.f, fs2 = 0.f;
   float bm = -1.f;  // largest in-band |X| of this thread (first r on ties: lowest bin)
@@ -173,9 +193,11 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
       }
     }
   }
-  wave_lds_sync();  // the wave's rows of the tile are free again
+  if constexpr (!LEARN) {
+    wave_lds_sync();  // the wave's rows of the tile are free again
 #pragma unroll
-  for (int r = 0; r < PPT; ++r) ldsf[LAY2::idx(s_row, jj_row + 16 * r)] = m[r];
+    for (int r = 0; r < PPT; ++r) ldsf[LAY2::idx(s_row, jj_row + 16 * r)] = m[r];
+  }
   if (s_row == 0 || s_row == RB - 1) {
     float* er = w.edge + (((uint64_t)ci * NT2 + t) * 2 + (s_row == 0 ? 0 : 1)) * N1;
 #pragma unroll
@@ -199,9 +221,9 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
       const int r = __builtin_ctz(pend);
       pend &= pend - 1;
       const int k1 = jj_row + 16 * r;
-      const float mc = ldsf[LAY2::idx(s_row, k1)];
-      const float yl = ldsf[LAY2::idx(rl, k1)];
-      const float yr = ldsf[LAY2::idx(rr, k1)];
+      const float mc = mt(s_row, k1);
+      const float yl = mt(rl, k1);
+      const float yr = mt(rr, k1);
       const bool ok = full_band || in_range(k2 + (uint32_t)N2 * (uint32_t)k1, a);
       if (ok && (top || mc > yl) && (bot || mc > yr)) {
         if (mc >= bnd.hi) mhigh |= 1u << r;
@@ -229,10 +251,19 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
           cd.k = k2 + (uint32_t)N2 * (uint32_t)k1;
           cd.re = x.x;
           cd.im = x.y;
-          cd.ml = top ? qnan : ldsf[LAY2::idx(rl, k1)];
+          cd.ml = top ? qnan : mt(rl, k1);
           cd.mc = mc;
-          cd.mr = bot ? qnan : ldsf[LAY2::idx(rr, k1)];
-          gp(cand)[high ? p : (uint32_t)C::CAND_CAP - 1 - p] = cd;
+          cd.mr = bot ? qnan : mt(rr, k1);
+          const uint32_t slot = high ? p : (uint32_t)C::CAND_CAP - 1 - p;
+          gp(cand)[slot] = cd;
+          if constexpr (LEARN) {
+            float2* cx4 = w.candx + ((uint64_t)ci * C::CAND_CAP + slot) * 4;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int row = s_row + (j < 2 ? j - 2 : j - 1);
+              gp(cx4)[j] = (row >= 0 && row < RB) ? cxt[LAY2::idx(row, k1)] : make_float2(qnan, qnan);
+            }
+          }
         }
       }
     };
@@ -258,8 +289,8 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
     const int k1 = jj_row + 16 * rb;
     gp(&pp->xr)[0] = x.x;
     gp(&pp->xi)[0] = x.y;
-    gp(&pp->ml)[0] = top ? qnan : ldsf[LAY2::idx(rl, k1)];
-    gp(&pp->mr)[0] = bot ? qnan : ldsf[LAY2::idx(rr, k1)];
+    gp(&pp->ml)[0] = top ? qnan : mt(rl, k1);
+    gp(&pp->mr)[0] = bot ? qnan : mt(rr, k1);
   }
 }
 
@@ -281,7 +312,20 @@ struct FsFinishShared {
 // looked up here, and the candidate dropped (mc = NaN) when it is no maximum.
 // Up to 64 candidates (one per lane) are selected with ballots, without
 // barriers or LDS; longer lists go through the bitmap selection (select_first_k).
-template <int N2, int TC>
+// X at bin (k + d) mod N (|d| <= 2) from the complex edge rows 0, 1, RB-2,
+// RB-1 of its row block (a neighbour that lies outside the candidate's block)
+template <int N2, int RB>
+__device__ __forceinline__ float2 fs_outer_x(const float2* edgex, uint32_t k, int d) {
+  constexpr uint32_t N = 256u * N2;
+  const uint32_t kk = (k + N + (uint32_t)d) % N;
+  const uint32_t k2 = kk & (N2 - 1), k1 = kk / N2, row = k2 % RB;
+  const uint32_t e = row < 2 ? row : row - (RB - 4);
+  return gp(edgex)[((k2 / RB) * 4 + e) * 256 + k1];
+}
+
+// LEARN: every stored record also gets its refiner feature row (a.mlp_feat,
+// row ci * kcap + rank of the chunk), from the candidate's neighbourhood.
+template <int N2, int TC, bool LEARN = false>
 __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
   using C = FsCfg<N2, TC>;
   constexpr int NT1 = C::NT1, NT2 = C::NT2, T = C::TF;
@@ -343,6 +387,26 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
     if (__builtin_isnan(br)) br = fs_outer<N2, TC>(edge, bk, 1);
     write_record(a.gmax + och, bk, bx, bl, bm, br, a);
   }
+  // refiner features of candidate i (list position), stored at record `rank`
+  auto features = [&](uint32_t i, uint32_t rank, const Cand& q) {
+    if constexpr (LEARN) {
+      const uint32_t slot = i < nh ? i : (uint32_t)C::CAND_CAP - 1 - (i - nh);
+      const float2* cx4 = w.candx + ((uint64_t)ci * C::CAND_CAP + slot) * 4;
+      const float2* ex = w.edgex + (uint64_t)ci * NT2 * 4 * C::N1;
+      float2 x[5];
+      x[2] = make_float2(q.re, q.im);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int d = j < 2 ? j - 2 : j - 1;
+        float2 v = gp(cx4)[j];
+        if (__builtin_isnan(v.x)) v = fs_outer_x<N2, C::RB>(ex, q.k, d);
+        x[d + 2] = v;
+      }
+      const uint32_t k0 = q.k;
+      mlp_store_features([&](uint32_t qq) { return x[(qq + (uint32_t)C::N + 2u - k0) % (uint32_t)C::N]; }, q.k,
+                         (uint32_t)C::N, mlp_feat_row(a, ci, (int)rank));
+    }
+  };
   if (nc <= 64u) {
     const bool surv = (uint32_t)lane < nc && c.mc > thr;
     uint64_t smask = __ballot(surv);
@@ -356,8 +420,10 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
       rank += (kj < c.k) ? 1 : 0;
     }
     const int kcap = (int)a.kcap;
-    if (surv && rank < kcap)
+    if (surv && rank < kcap) {
       write_record(a.peaks + och * a.kcap + rank, c.k, make_float2(c.re, c.im), c.ml, c.mc, c.mr, a);
+      features((uint32_t)lane, (uint32_t)rank, c);
+    }
     if (lane == 0) {
       if (a.counts) {
         gp(a.counts)[2 * och + 0] = (uint32_t)min(total, kcap);
@@ -375,7 +441,7 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
       Cand q = at(i);
       resolve(q);
       return q;
-    }, nc, thr, mu, sd, P, och, sh.bits, sh.pre, sh.base, sh.red);
+    }, nc, thr, mu, sd, P, och, sh.bits, sh.pre, sh.base, sh.red, features);
   }
 }
 
@@ -688,8 +754,20 @@ static hipError_t launch_fs(const FusedArgs& a, const FsWork& w0, uint32_t chunk
     w.nch = min(chunk, a.batch - c0);
     const uint32_t groups = (w.nch + C::MC - 1) / C::MC;
     hipLaunchKernelGGL((k_fs_cols<N2, TC>), dim3(groups * C::NT1), dim3(TC), 0, st, a, w);
-    hipLaunchKernelGGL((k_fs_rows<N2, TC>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);
-    hipLaunchKernelGGL((k_fs_finish<N2, TC>), dim3(w.nch), dim3(C::TF), 0, st, a, w);
+    if (w.candx != nullptr) {
+      // learned refinement: feature rows of the chunk's records, then the
+      // refiner over them (before the chunk's workspace is reused)
+      hipLaunchKernelGGL((k_fs_rows<N2, TC, true>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);
+      hipLaunchKernelGGL((k_fs_finish<N2, TC, true>), dim3(w.nch), dim3(C::TF), 0, st, a, w);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      e = spx_launch_mlp_refine(a.mlp, a.mlp_feat, a.peaks + (uint64_t)c0 * a.kcap, a.counts + 2 * (uint64_t)c0,
+                                w.nch, a.kcap, a.fs / (float)a.nfft, st);
+      if (e != hipSuccess) return e;
+    } else {
+      hipLaunchKernelGGL((k_fs_rows<N2, TC>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);
+      hipLaunchKernelGGL((k_fs_finish<N2, TC>), dim3(w.nch), dim3(C::TF), 0, st, a, w);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -697,12 +775,19 @@ static hipError_t launch_fs(const FusedArgs& a, const FsWork& w0, uint32_t chunk
 }
 
 
+// learned refinement on the split path: threshold search with a refiner set
+__host__ __device__ inline bool fs_learn(const FusedArgs& a) {
+  return a.refine == 2 && a.peak_mode == 1 && a.mlp != nullptr && a.kcap > 0;
+}
+
 template <int N2>
-static uint64_t fs_channel_bytes() {
+static uint64_t fs_channel_bytes(bool learn, uint32_t kcap) {
   using C = FsCfg<N2, FsShape<N2>::TC>;
   auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
-  return r((uint64_t)C::N * 8) + r(C::CAND_CAP * sizeof(Cand)) + 256 + r((uint64_t)C::NT1 * 8) +
-         r((uint64_t)C::NT2 * sizeof(FsRowPart)) + r((uint64_t)C::NT2 * 2 * 256 * 4);
+  uint64_t b = r((uint64_t)C::N * 8) + r(C::CAND_CAP * sizeof(Cand)) + 256 + r((uint64_t)C::NT1 * 8) +
+               r((uint64_t)C::NT2 * sizeof(FsRowPart)) + r((uint64_t)C::NT2 * 2 * 256 * 4);
+  if (learn) b += r(C::CAND_CAP * 4 * 8) + r((uint64_t)C::NT2 * 4 * 256 * 8) + (uint64_t)kcap * 32;
+  return b;
 }
 
 template <int N2>
@@ -717,12 +802,20 @@ static hipError_t launch_fs_n(const FusedArgs& a, void* ws, uint32_t chunk, cons
   w.ncand = reinterpret_cast<uint32_t*>(p);      p += r((uint64_t)chunk * 8);
   w.pe = reinterpret_cast<double*>(p);           p += r((uint64_t)chunk * C::NT1 * 8);
   w.part = reinterpret_cast<FsRowPart*>(p);      p += r((uint64_t)chunk * C::NT2 * sizeof(FsRowPart));
-  w.edge = reinterpret_cast<float*>(p);
+  w.edge = reinterpret_cast<float*>(p);          p += r((uint64_t)chunk * C::NT2 * 2 * 256 * 4);
+  w.candx = nullptr;
+  w.edgex = nullptr;
+  FusedArgs al = a;
+  if (fs_learn(a)) {
+    w.candx = reinterpret_cast<float2*>(p);      p += r((uint64_t)chunk * C::CAND_CAP * 4 * 8);
+    w.edgex = reinterpret_cast<float2*>(p);      p += r((uint64_t)chunk * C::NT2 * 4 * 256 * 8);
+    al.mlp_feat = p;  // chunk feature rows: [chunk][kcap][16 halves]
+  }
   w.ch0 = 0;
   w.nch = 0;
   w.chan = chan;
   w.inoff = inoff;
-  return launch_fs<N2, FsShape<N2>::TC>(a, w, chunk, st);
+  return launch_fs<N2, FsShape<N2>::TC>(al, w, chunk, st);
 }
 
 }  // namespace spx
@@ -731,17 +824,20 @@ extern "C" {
 
 // Workspace of the split long-channel peak search for `chunk` channels of
 // length nfft (0 when nfft is not 2^15 .. 2^18).
-uint64_t spx_fs_workspace_bytes(uint32_t nfft, uint32_t chunk) {
+uint64_t spx_fs_workspace_bytes(uint32_t nfft, uint32_t chunk) { return spx_fs_workspace_bytes_ex(nfft, chunk, 0, 0); }
+
+// ... with the learned-refinement buffers when learn != 0 (kcap records per channel)
+uint64_t spx_fs_workspace_bytes_ex(uint32_t nfft, uint32_t chunk, int learn, uint32_t kcap) {
   using namespace spx;
   uint64_t per = 0;
   switch (nfft) {
-    case 1u << 15: per = fs_channel_bytes<128>(); break;
-    case 1u << 16: per = fs_channel_bytes<256>(); break;
-    case 1u << 17: per = fs_channel_bytes<512>(); break;
-    case 1u << 18: per = fs_channel_bytes<1024>(); break;
+    case 1u << 15: per = fs_channel_bytes<128>(learn != 0, kcap); break;
+    case 1u << 16: per = fs_channel_bytes<256>(learn != 0, kcap); break;
+    case 1u << 17: per = fs_channel_bytes<512>(learn != 0, kcap); break;
+    case 1u << 18: per = fs_channel_bytes<1024>(learn != 0, kcap); break;
     default: return 0;
   }
-  return per * (uint64_t)chunk + 6 * 256;
+  return per * (uint64_t)chunk + 9 * 256;
 }
 
 // Runs the split path over a.batch channels in chunks of `chunk` channels,

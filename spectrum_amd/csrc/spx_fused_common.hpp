@@ -322,12 +322,16 @@ struct Cand {
 // land in order with no sort and for any K.  Thread t owns words
 // [t*WPT, (t+1)*WPT): base[t] holds the survivors below its first word and
 // pre[w] the survivors in its earlier words (at most 32*WPT, 16 bits).
-// cand(i) returns candidate i (mc = NaN: no survivor).  Every thread of the
+// cand(i) returns candidate i (mc = NaN: no survivor); hook(i, rank, q) runs
+// after the record of candidate i is written at `rank`.  Every thread of the
 // T-thread workgroup calls it; bits, pre, base are LDS.
-template <int T, int NW, class SRC>
+struct NoRecordHook {
+  __device__ __forceinline__ void operator()(uint32_t, uint32_t, const Cand&) const {}
+};
+template <int T, int NW, class SRC, class HOOK = NoRecordHook>
 __device__ __forceinline__ void select_first_k(const FusedArgs& a, SRC cand, uint32_t ncand, float thr, float mu,
                                                float sd, double P, uint64_t och, uint32_t* bits, uint16_t* pre,
-                                               int* base, BlockRed& red) {
+                                               int* base, BlockRed& red, HOOK hook = HOOK()) {
   constexpr int WPT = NW / T;
   static_assert(NW % T == 0 && 32 * WPT < 65536, "bitmap words per thread");
   const int tid = threadIdx.x;
@@ -354,7 +358,10 @@ __device__ __forceinline__ void select_first_k(const FusedArgs& a, SRC cand, uin
       if (q.mc > thr) {
         const uint32_t w = q.k >> 5;
         const uint32_t rank = (uint32_t)base[w / WPT] + pre[w] + __popc(bits[w] & ((1u << (q.k & 31)) - 1u));
-        if (rank < kcap) write_record(out + rank, q.k, make_float2(q.re, q.im), q.ml, q.mc, q.mr, a);
+        if (rank < kcap) {
+          write_record(out + rank, q.k, make_float2(q.re, q.im), q.ml, q.mc, q.mr, a);
+          hook(i, rank, q);
+        }
       }
     }
   }

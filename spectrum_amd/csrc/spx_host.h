@@ -54,7 +54,7 @@ int window_ptr(spx_ctx* c, uint32_t window, uint32_t n_point, const float** w);
 // Releases the host-streaming buffers, streams and events.
 void host_stage_free(spx_ctx* c);
 // Channels per launch group of the split long-channel path (bounds its workspace).
-uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch);
+uint32_t fourstep_chunk(uint32_t nfft, uint32_t batch, int learn = 0, uint32_t kcap = 0);
 // Validates a fused descriptor into kernel arguments; runs them.
 int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a);
 int run_fused(spx_ctx* c, FusedArgs& a);

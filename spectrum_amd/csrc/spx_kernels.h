@@ -93,6 +93,7 @@ hipError_t spx_launch_batch(const BatchArgs* b, int grid, hipStream_t st);
 hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st);
 int spx_fused_is_large(uint32_t nfft);
 uint64_t spx_fs_workspace_bytes(uint32_t nfft, uint32_t chunk);
+uint64_t spx_fs_workspace_bytes_ex(uint32_t nfft, uint32_t chunk, int learn, uint32_t kcap);
 hipError_t spx_launch_fourstep(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st);
 hipError_t spx_launch_fourstep_ex(const FusedArgs* a, void* ws, uint32_t chunk, const uint32_t* chan,
                                   const uint64_t* inoff, hipStream_t st);

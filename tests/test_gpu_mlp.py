@@ -161,3 +161,71 @@ def test_fused_learned_many_peaks_matches_predict(ctx):
     assert np.max(np.abs(got[inside] - pred[inside])) < 2e-3
     assert got[outside].tobytes() == P["freq_offset"][outside].tobytes()
     assert np.allclose(L["refined_frequency"], (L["index"] + got) / N, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("N", [1 << 15, 1 << 16, 1 << 17, 1 << 18])
+def test_split_learned_matches_predict(ctx, N):
+    """Learned refinement on the split long-channel path (features from the
+    row kernel's candidate neighbourhoods and complex block-edge rows, the
+    refiner once per chunk).  Records equal the parabolic ones except the
+    offset, which is the network's prediction on the features of the
+    spectrum (the split dense transform, same arithmetic) wherever it lies in
+    [-0.5, 0.5].  Two regimes: tones on row-block edges (k2 = k mod N2 in
+    {0, 1, 14, 15} mod 16 and bins 0, 1, N-2, N-1: neighbours in other blocks),
+    threshold mean + 6 std (few candidates); and nsigma = 0 on noise, more
+    than 64 candidates (the bitmap selection)."""
+    from spectrum_amd.refiner import DEFAULT_WEIGHTS, Refiner
+
+    N2 = N // 256
+    r = Refiner(ctx).load(DEFAULT_WEIGHTS)
+    r.export()
+    n = np.arange(N)
+    rng = np.random.default_rng(N + 7)
+    bins = [[5 * N2 + 0, 9 * N2 + 1, 200 * N2 + 14], [3 * N2 + 15, 0, N // 2 + 7], [N - 1, 60 * N2 + 1, 120 * N2 + 15],
+            [1, 31 * N2 + 17, 250 * N2 + 30], [N - 2, 100 * N2 + 31, 150 * N2 + 16]]
+    B = len(bins)
+    x = 0.01 * (rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N)))
+    for b, ks in enumerate(bins):
+        for j, k in enumerate(ks):
+            x[b] += (1.0 - 0.2 * j) * np.exp(2j * np.pi * (k + 0.1 + 0.15 * j) * n / N)
+    noise = (rng.standard_normal((2, N)) + 1j * rng.standard_normal((2, N))).astype(np.complex64)
+    for name, sig, nsigma, K in (("edges", x.astype(np.complex64), 6.0, 64), ("noise", noise, 0.0, 100)):
+        Bc = sig.shape[0]
+        print(f"[case] split learned N={N} {name}", flush=True)
+        d_in = ctx.to_device(sig)
+        out = {}
+        for rname, refine in (("learned", capi.REFINE_LEARNED), ("parabolic", capi.REFINE_PARABOLIC)):
+            pk, ct = ctx.alloc(Bc * K * 32), ctx.alloc(Bc * 8)
+            ctx.set_refiner(r.deployed if refine == capi.REFINE_LEARNED else None)
+            ctx.fused_peaks(d_in, Bc, N, N, pk, ct, window=1, nsigma=nsigma, max_peaks=K, refine=refine)
+            ctx.sync()
+            out[rname] = (pk.download(capi.MAXVALUE_DTYPE, (Bc, K)), ct.download(np.uint32, (Bc, 2)))
+        ctx.set_refiner(None)
+        (L, cl), (P, cp) = out["learned"], out["parabolic"]
+        assert np.array_equal(cl, cp)
+        if name == "noise":
+            assert np.all(cl[:, 0] == K)
+        else:
+            for b, ks in enumerate(bins):
+                assert set(ks) <= set(int(k) for k in L["index"][b, : cl[b, 0]]), (b, ks)
+        for b in range(Bc):
+            for f in ("index", "real", "imag", "magnitude", "phase"):
+                assert L[f][b, : cl[b, 0]].tobytes() == P[f][b, : cl[b, 0]].tobytes(), (b, f)
+        spec = ctx.alloc(Bc * N * 8)
+        ctx.fft_c2c(d_in, spec, Bc, N, N, 1)
+        ctx.sync()
+        X = spec.download(np.complex64, (Bc, N)).astype(np.complex128)
+        rows = [(c, s) for c in range(Bc) for s in range(int(cl[c, 0]))]
+        feats = np.array([omlp.features(X[c], int(L["index"][c, s])) for c, s in rows])
+        pred = r.predict(feats.astype(np.float32))
+        got = np.array([L["freq_offset"][c, s] for c, s in rows])
+        par = np.array([P["freq_offset"][c, s] for c, s in rows])
+        inside = np.abs(pred) <= 0.5 - 2e-3
+        outside = np.abs(pred) > 0.5 + 2e-3
+        assert inside.sum() >= len(rows) // 2
+        assert np.max(np.abs(got[inside] - pred[inside])) < 2e-3
+        assert np.array_equal(got[outside], par[outside])
+        for b in range(Bc):
+            m = int(cl[b, 0])
+            assert np.allclose(L["refined_frequency"][b, :m], (L["index"][b, :m] + L["freq_offset"][b, :m]) / N,
+                               rtol=1e-5, atol=1e-7)
