@@ -22,7 +22,9 @@ with --no-extras):
              error against FP64;
   refiner    4096 x 8192 fractional tones: learned vs parabolic refinement
              (time per launch, frequency RMSE against the true tones);
-  refiner_split  the same at the headline shape, 1024 x 65536 (split path).
+  refiner_split  the same at the headline shape, 1024 x 65536 (split path);
+  top_peaks  the reference ONE/TWO_PEAKS search inside the transform vs
+             transform-then-search, 1024 x 65536.
 """
 import argparse
 import ctypes as C
@@ -399,6 +401,36 @@ def extra_refiner_split(ctx, K=64, reps=20):
     return out
 
 
+def extra_top_peaks(ctx, reps=20):
+    """The reference ONE_PEAK / TWO_PEAKS search of SpectrumProcessorROCm at
+    1024 x 65536 complex64 (search range nfft/4): inside the transform
+    (spx_fused_peaks_top: three launches, no spectrum in HBM) against the
+    reference's chain, transform to a stored spectrum then the search
+    (spx_fft_c2c + spx_peaks_top)."""
+    from spectrum_amd import capi
+
+    B, N = 1024, 65536
+    x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED0007, B, N)
+    d = ctx.to_device(x)
+    spec = ctx.alloc(B * N * 8)
+    out = {"workload": f"{B} x {N} complex64, search range {N // 4}, fs 1"}
+    for name, two in (("one_peak", False), ("two_peaks", True)):
+        R = 8 if two else 4
+        o1, o2 = ctx.alloc(B * R * 32), ctx.alloc(B * R * 32)
+        t_f = _timer_ms(ctx, lambda: ctx.fused_peaks_top(d, o1, B, N, N, N // 4, 1.0, two_peaks=two), reps)
+
+        def chain():
+            ctx.fft_c2c(d, spec, B, N, N)
+            ctx.peaks_top(spec, o2, B, N, N // 4, 1.0, two_peaks=two)
+
+        t_c = _timer_ms(ctx, chain, reps)
+        a = o1.download(capi.MAXVALUE_DTYPE, (B, R))
+        b = o2.download(capi.MAXVALUE_DTYPE, (B, R))
+        out[name] = {"fused_ms": t_f, "fft_then_search_ms": t_c, "speedup": t_c / t_f,
+                     "records_equal": bool(np.array_equal(a["index"], b["index"]))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,7 +443,7 @@ def main():
     ap.add_argument("--kpeaks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extra", choices=["host_stream", "uneven", "mixed_mlp", "refiner", "refiner_split"],
+    ap.add_argument("--extra", choices=["host_stream", "uneven", "mixed_mlp", "refiner", "refiner_split", "top_peaks"],
                     help="run only this extra workload (no headline, no CPU baseline) and print its JSON; "
                          "for profiling one workload")
     ap.add_argument("--reps", type=int, default=30, help="timed repetitions of each --extra form")
@@ -434,7 +466,8 @@ def main():
         elif args.extra == "mixed_mlp":
             res = extra_mixed_mlp(ctx, reps=args.reps)
         else:
-            res = {"uneven": extra_uneven, "refiner": extra_refiner, "refiner_split": extra_refiner_split}[args.extra](ctx)
+            res = {"uneven": extra_uneven, "refiner": extra_refiner, "refiner_split": extra_refiner_split,
+                   "top_peaks": extra_top_peaks}[args.extra](ctx)
         print(json.dumps({args.extra: res}), flush=True)
         return
     info = ctx.info()
@@ -515,7 +548,7 @@ def main():
             out["host_stream"] = {"error": repr(e)}
         del d_in, x
         for name, fn in (("uneven", extra_uneven), ("mixed_mlp", extra_mixed_mlp), ("refiner", extra_refiner),
-                         ("refiner_split", extra_refiner_split)):
+                         ("refiner_split", extra_refiner_split), ("top_peaks", extra_top_peaks)):
             print(f"[bench] {name} ...", file=sys.stderr, flush=True)
             try:
                 out[name] = fn(ctx)

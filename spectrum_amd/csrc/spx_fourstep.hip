@@ -558,15 +558,22 @@ __global__ void __launch_bounds__(256) k_fs_rows_top(FusedArgs a, FsWork w) {
 #pragma unroll
     for (int r = 0; r < PPT; ++r) gp(edge)[(s_row == 0 ? 0 : N1) + jj_row + 16 * r] = v[r];
   }
+  // per side the largest |X| of the thread (ascending k: the first wins
+  // ties), with X there; v is only ever indexed with unrolled constants
   float lm = -1.f, rm = -1.f;
   uint32_t li = 0xffffffffu, ri = 0xffffffffu;
-  int lr = 0, rr = 0;
+  float2 lx = make_float2(0.f, 0.f), rx = lx;
 #pragma unroll
   for (int r = 0; r < PPT; ++r) {
     const uint32_t k = k2 + (uint32_t)N2 * (uint32_t)(jj_row + 16 * r);
     const float m = cabs_rn(v[r]);
-    if (k < a.s0 && (m > lm || (m == lm && k < li))) { lm = m; li = k; lr = r; }
-    if (k >= a.s1 && (m > rm || (m == rm && k < ri))) { rm = m; ri = k; rr = r; }
+    const bool ul = k < a.s0 && m > lm, ur = k >= a.s1 && m > rm;
+    lm = ul ? m : lm;
+    li = ul ? k : li;
+    lx = ul ? v[r] : lx;
+    rm = ur ? m : rm;
+    ri = ur ? k : ri;
+    rx = ur ? v[r] : rx;
   }
   float glm = lm, grm = rm;
   uint32_t gli = li, gri = ri;
@@ -575,15 +582,12 @@ __global__ void __launch_bounds__(256) k_fs_rows_top(FusedArgs a, FsWork w) {
   block_stats<T, false>(d0, d1, grm, gri, red);
   FsTopPart* pp = reinterpret_cast<FsTopPart*>(w.part) + (uint64_t)ci * NT2 + t;
   const float qnan = __builtin_nanf("");
-  auto put = [&](int side, float m, uint32_t k, int r) {
+  auto put = [&](int side, float m, uint32_t k, float2 x) {
     FsTopSide sd;
     sd.m = m;
     sd.k = k;
-    float2 x = v[0];
-#pragma unroll
-    for (int q = 1; q < PPT; ++q) x = (q == r) ? v[q] : x;
     sd.x = x;
-    const int k1 = jj_row + 16 * r;
+    const int k1 = (int)(k / (uint32_t)N2);  // the bin's column: its row in the block is s_row
     sd.xl = s_row > 0 ? cx[LAY2::idx(s_row - 1, k1)] : make_float2(qnan, qnan);
     sd.xr = s_row < RB - 1 ? cx[LAY2::idx(s_row + 1, k1)] : make_float2(qnan, qnan);
     gp(&pp->s[side])[0] = sd;
@@ -598,12 +602,12 @@ __global__ void __launch_bounds__(256) k_fs_rows_top(FusedArgs a, FsWork w) {
   if (gli == 0xffffffffu) {
     if (tid == 0) none(0);
   } else if (li == gli) {
-    put(0, lm, li, lr);
+    put(0, lm, li, lx);
   }
   if (gri == 0xffffffffu) {
     if (tid == 0) none(1);
   } else if (ri == gri) {
-    put(1, rm, ri, rr);
+    put(1, rm, ri, rx);
   }
 }
 
@@ -640,34 +644,29 @@ __global__ void __launch_bounds__(64) k_fs_top_finish(FusedArgs a, FsWork w) {
   const FsTopPart* parts = reinterpret_cast<const FsTopPart*>(w.part) + (uint64_t)ci * NT2;
   const float2* edge = reinterpret_cast<const float2*>(w.edge) + (uint64_t)ci * NT2 * 2 * 256;
   const uint32_t N = a.nfft;
-  float m[2] = {-1.f, -1.f};
-  uint32_t k[2] = {0xffffffffu, 0xffffffffu};
-  FsTopSide sd[2];
+  FsTopSide s0, s1;
+  s0.m = s1.m = -1.f;
+  s0.k = s1.k = 0xffffffffu;
+  s0.x = s0.xl = s0.xr = s1.x = s1.xl = s1.xr = make_float2(0.f, 0.f);
   if (lane < NT2) {
-    sd[0] = gp(&parts[lane].s[0])[0];
-    sd[1] = gp(&parts[lane].s[1])[0];
-    m[0] = sd[0].m;
-    k[0] = sd[0].k;
-    m[1] = sd[1].m;
-    k[1] = sd[1].k;
+    s0 = gp(&parts[lane].s[0])[0];
+    s1 = gp(&parts[lane].s[1])[0];
   }
-  TripletBins tb[2];
-  float gm[2];
-  uint32_t gk[2];
-#pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    gm[side] = m[side];
-    gk[side] = k[side];
-    wave_argmax(gm[side], gk[side]);
-    const uint64_t own = __ballot(lane < NT2 && k[side] == gk[side] && gk[side] != 0xffffffffu);
+  // the side's winner over the row blocks (lowest bin on ties), its values
+  // from the owning lane, a neighbour in another block from the edge rows
+  auto side_bins = [&](const FsTopSide& sd, int side, float& gm) {
+    gm = sd.m;
+    uint32_t gk = sd.k;
+    wave_argmax(gm, gk);
+    const uint64_t own = __ballot(lane < NT2 && sd.k == gk && gk != 0xffffffffu);
     const int src = own ? __builtin_ctzll(own) : 0;
     auto rd = [&](float f) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, f), src)); };
     TripletBins t3;
-    t3.c = gk[side];
-    t3.xc = make_float2(rd(sd[side].x.x), rd(sd[side].x.y));
-    t3.xl = make_float2(rd(sd[side].xl.x), rd(sd[side].xl.y));
-    t3.xr = make_float2(rd(sd[side].xr.x), rd(sd[side].xr.y));
-    if (gk[side] != 0xffffffffu) {
+    t3.c = gk;
+    t3.xc = make_float2(rd(sd.x.x), rd(sd.x.y));
+    t3.xl = make_float2(rd(sd.xl.x), rd(sd.xl.y));
+    t3.xr = make_float2(rd(sd.xr.x), rd(sd.xr.y));
+    if (gk != 0xffffffffu) {
       if (__builtin_isnan(t3.xl.x) && t3.c > 0) t3.xl = fs_outer_cx<N2, RB>(edge, t3.c, -1);
       if (__builtin_isnan(t3.xr.x) && t3.c < N - 1) t3.xr = fs_outer_cx<N2, RB>(edge, t3.c, 1);
     } else {
@@ -677,16 +676,19 @@ __global__ void __launch_bounds__(64) k_fs_top_finish(FusedArgs a, FsWork w) {
       t3.c = side == 0 ? 0u : min(a.s1, N - 1);
       t3.xc = t3.xl = t3.xr = make_float2(qnan, qnan);
     }
-    tb[side] = t3;
-  }
+    return t3;
+  };
+  float gm0, gm1;
+  const TripletBins t0 = side_bins(s0, 0, gm0);
+  const TripletBins t1 = side_bins(s1, 1, gm1);
   if (lane == 0) {
     MaxValue* out = a.peaks + och * (a.peak_mode == 3 ? 4 : 8);
     if (a.peak_mode == 3) {
-      const int right = gm[1] > gm[0] ? 1 : 0;
-      write_peak4(out, tb[right], N, tb[right].c, right, a.fs);
+      if (gm1 > gm0) write_peak4(out, t1, N, t1.c, 1, a.fs);
+      else write_peak4(out, t0, N, t0.c, 0, a.fs);
     } else {
-      write_peak4(out, tb[0], N, tb[0].c, 0, a.fs);
-      write_peak4(out + 4, tb[1], N, tb[1].c, 1, a.fs);
+      write_peak4(out, t0, N, t0.c, 0, a.fs);
+      write_peak4(out + 4, t1, N, t1.c, 1, a.fs);
     }
   }
 }

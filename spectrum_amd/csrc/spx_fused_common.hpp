@@ -174,6 +174,9 @@ struct LdsBins {
 // right-side peak, frequencies reported as fs - f.
 template <class BINS>
 __device__ void write_peak4(MaxValue* out, BINS X, uint32_t nfft, uint32_t c, int mirror, float fs) {
+  // no fused multiply-adds: fs - (c + d) bw cancels, and every caller (the
+  // search on a stored spectrum and the fused kernels) must round alike
+#pragma clang fp contract(off)
   const float bw = fs / (float)nfft;
   const float2 cv = X(c);
   const float yc = cabs_rn(cv);
