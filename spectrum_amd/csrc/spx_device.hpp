@@ -108,6 +108,28 @@ __device__ __forceinline__ float2 tw16(float2 a) {
   }
 }
 
+// tw16<E> for an exponent that is a constant only after loop unrolling
+__device__ __forceinline__ float2 tw16_rt(int e, float2 a) {
+  switch (e & 15) {
+    case 0: return a;
+    case 1: return tw16<1>(a);
+    case 2: return tw16<2>(a);
+    case 3: return tw16<3>(a);
+    case 4: return tw16<4>(a);
+    case 5: return tw16<5>(a);
+    case 6: return tw16<6>(a);
+    case 7: return tw16<7>(a);
+    case 8: return tw16<8>(a);
+    case 9: return tw16<9>(a);
+    case 10: return tw16<10>(a);
+    case 11: return tw16<11>(a);
+    case 12: return tw16<12>(a);
+    case 13: return tw16<13>(a);
+    case 14: return tw16<14>(a);
+    default: return tw16<15>(a);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // DFT butterflies, natural-order in and out, in registers.
 // ---------------------------------------------------------------------------
@@ -202,12 +224,27 @@ struct Plan {
   static constexpr int NP = (LOG + 3) / 4;
   static constexpr int RX = (LOG % 4) ? (1 << (LOG % 4)) : 16;
   static constexpr int radix(int p) { return NP == 1 ? RX : (V == 0 ? (p == NP - 1 ? RX : 16) : (p == 0 ? RX : 16)); }
-  static constexpr int ns(int p) { return p <= 0 ? 1 : ns(p - 1) * radix(p - 1); }
-  static constexpr int twoff(int p) { return p <= 1 ? 0 : twoff(p - 1) + ns(p - 1) * radix(p - 1); }
+  // loops, not recursion: a recursive constexpr function evaluated outside a
+  // constant expression is emitted as a real device call (s_swappc plus the
+  // caller's register saves around it)
+  static constexpr int ns(int p) {
+    int n = 1;
+    for (int i = 0; i < p; ++i) n *= radix(i);
+    return n;
+  }
+  static constexpr int twoff(int p) {
+    int o = 0;
+    for (int i = 1; i < p; ++i) o += ns(i) * radix(i);
+    return o;
+  }
   static constexpr int TWSIZE = NP <= 1 ? 1 : twoff(NP - 1) + ns(NP - 1) * radix(NP - 1);
   // LDS copies pad every [k][r] row by two entries (a 16-byte multiple that
   // spreads the rows of neighbouring k over the banks)
-  static constexpr int twoff_lds(int p) { return p <= 1 ? 0 : twoff_lds(p - 1) + ns(p - 1) * (radix(p - 1) + 2); }
+  static constexpr int twoff_lds(int p) {
+    int o = 0;
+    for (int i = 1; i < p; ++i) o += ns(i) * (radix(i) + 2);
+    return o;
+  }
   static constexpr int TWSIZE_LDS = NP <= 1 ? 1 : twoff_lds(NP - 1) + ns(NP - 1) * (radix(NP - 1) + 2);
   static constexpr int R0 = (NP == 1) ? RX : (V == 0 ? 16 : RX);   // first-pass radix
   static constexpr int RL = (NP == 1) ? RX : (V == 0 ? RX : 16);   // last-pass radix
@@ -361,9 +398,158 @@ template <class LAY, int L, int NSEQ, int T, int PPT, int P, int V, bool TWLDS>
 struct FftPasses {
   __device__ __forceinline__ static void run(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
     using PL = Plan<L, V>;
-    fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1, TWLDS>(
-        lds, v, tw + (TWLDS ? PL::twoff_lds(P) : PL::twoff(P)), tid);
+    constexpr int OFF = TWLDS ? PL::twoff_lds(P) : PL::twoff(P);
+    fft_pass<LAY, L, NSEQ, PL::radix(P), PL::ns(P), T, PPT, P == 0, P == PL::NP - 1, TWLDS>(lds, v, tw + OFF, tid);
     if constexpr (P + 1 < PL::NP) FftPasses<LAY, L, NSEQ, T, PPT, P + 1, V, TWLDS>::run(lds, v, tw, tid);
+  }
+};
+
+// The same passes with each pass's twiddles loaded one step early: right
+// after the previous pass has written its outputs to LDS (its VGPRs are free
+// then) and before the barrier, so the table reads' latency runs under the
+// barrier wait and the LDS reads instead of after them.  Fewer bytes are read
+// than the R-entry rows hold (the vector-memory path, not the table, limits):
+//  TW_ROWS  w[b*R + r] = W_(NS*R)^(r*(jj%NS)), the table rows as they are;
+//  TW_LAST  last pass of one sequence with T = L/16 threads: butterfly
+//           j = tid + b*T has k = j, and W_L^(r*j) = W_L^(r*tid) W16^(r*b),
+//           so only the row of k = tid is read (w[r]) and the W16 factors
+//           are compile-time rotations;
+//  TW_FACT  radix 16, one butterfly per thread: W^(r k) = W^((r&3) k)
+//           W^((r&12) k), so entries 1, 2, 3, 4, 8, 12 of the row are read
+//           (w[0..5] = W^(k), W^(2k), W^(3k), W^(4k), W^(8k), W^(12k)) and the
+//           other nine are products.
+enum { TW_ROWS = 0, TW_LAST = 1, TW_FACT = 2 };
+
+template <class LAY, int L, int NSEQ, int R, int NS, int T, int PPT, bool LAST>
+constexpr int tw_mode() {
+  if constexpr (NS <= 1) return TW_ROWS;
+  else if constexpr (LAST && NSEQ == 1 && PPT / R > 1 && T * 16 == L && NS * R == L) return TW_LAST;
+  else if constexpr (R == 16 && PPT == 16) return TW_FACT;
+  else return TW_ROWS;
+}
+
+template <class LAY, int L, int NSEQ, int R, int NS, int T, int PPT, bool TWLDS, int TWM>
+__device__ __forceinline__ void pass_twiddles(const float2* __restrict__ twp, int tid, float2 (&w)[PPT]) {
+  constexpr int BPT = PPT / R;
+  constexpr int NB = NSEQ * L / R;
+  constexpr bool FULL = (NB == T * BPT);
+  constexpr int RW = TWLDS ? R + 2 : R;  // row pitch
+  if constexpr (TWM == TW_LAST) {
+    load_tw_row<R, TWLDS>(twp + (uint32_t)(tid * RW), w);
+  } else if constexpr (TWM == TW_FACT) {
+    int s, jj;
+    LAY::template map<R>(tid, s, jj);
+    const float2* row = twp + (uint32_t)((jj % NS) * RW);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = TWLDS ? row[1 + i] : *gp(row + 1 + i);
+    w[4] = TWLDS ? row[8] : *gp(row + 8);
+    w[5] = TWLDS ? row[12] : *gp(row + 12);
+  } else {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = tid + b * T;
+      if (FULL || j < NB) {
+        int s, jj;
+        LAY::template map<R>(j, s, jj);
+        load_tw_row<R, TWLDS>(twp + (uint32_t)((jj % NS) * RW), &w[b * R]);
+      }
+    }
+  }
+}
+
+// twiddle of element r of butterfly b from what pass_twiddles read
+template <int R, int TWM>
+__device__ __forceinline__ float2 pass_tw(const float2* w, int b, int r) {
+  if constexpr (TWM == TW_LAST) {
+    return tw16_rt(r * b, w[r]);
+  } else if constexpr (TWM == TW_FACT) {
+    const int lo = r & 3, hi = r >> 2;  // w[lo - 1] = W^(lo k), w[2 + hi] = W^(4 hi k)
+    if (hi == 0) return w[lo - 1];
+    if (lo == 0) return w[2 + hi];
+    return c_mul(w[lo - 1], w[2 + hi]);
+  } else {
+    return w[b * R + r];
+  }
+}
+
+// fft_pass with the twiddles given and without the closing barrier (the
+// caller issues the next pass's twiddle loads first).
+template <class LAY, int L, int NSEQ, int R, int NS, int T, int PPT, bool FIRST, bool LAST, int TWM>
+__device__ __forceinline__ void fft_pass_w(float2* __restrict__ lds, float2 (&v)[PPT], const float2 (&w)[PPT], int tid) {
+  constexpr int BPT = PPT / R;
+  constexpr int NB = NSEQ * L / R;
+  constexpr bool FULL = (NB == T * BPT);
+  constexpr int RS = LAY::rstride(R);
+  constexpr int WS = LAY::wstride(NS);
+  if constexpr (!FIRST) {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = tid + b * T;
+      if (FULL || j < NB) {
+        int s, jj;
+        LAY::template map<R>(j, s, jj);
+        if constexpr (RS > 0) {
+          const float2* base = lds + LAY::idx(s, jj);
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[b * R + r] = base[r * RS];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[b * R + r] = lds[LAY::idx(s, jj + r * (L / R))];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = tid + b * T;
+    if (FULL || j < NB) {
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], pass_tw<R, TWM>(w, b, r));
+      }
+      Dft<R>::run(&v[b * R]);
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = tid + b * T;
+      if (FULL || j < NB) {
+        int s, jj;
+        LAY::template map<R>(j, s, jj);
+        const int d0 = (jj / NS) * NS * R + (jj % NS);
+        if constexpr (WS > 0) {
+          float2* base = lds + LAY::idx(s, d0);
+#pragma unroll
+          for (int r = 0; r < R; ++r) base[r * WS] = v[b * R + r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) lds[LAY::idx(s, d0 + r * NS)] = v[b * R + r];
+        }
+      }
+    }
+  }
+}
+
+template <class LAY, int L, int NSEQ, int T, int PPT, int P, int V, bool TWLDS>
+struct FftPassesPF {
+  __device__ __forceinline__ static void run(float2* lds, float2 (&v)[PPT], const float2 (&w)[PPT], const float2* tw,
+                                             int tid) {
+    using PL = Plan<L, V>;
+    constexpr int R = PL::radix(P), NS = PL::ns(P);
+    constexpr bool LAST = P == PL::NP - 1;
+    fft_pass_w<LAY, L, NSEQ, R, NS, T, PPT, P == 0, LAST, tw_mode<LAY, L, NSEQ, R, NS, T, PPT, LAST>()>(lds, v, w, tid);
+    if constexpr (P + 1 < PL::NP) {
+      constexpr int RN = PL::radix(P + 1), NSN = PL::ns(P + 1);
+      constexpr bool LASTN = P + 1 == PL::NP - 1;
+      constexpr int OFFN = TWLDS ? PL::twoff_lds(P + 1) : PL::twoff(P + 1);
+      float2 wn[PPT];
+      pass_twiddles<LAY, L, NSEQ, RN, NSN, T, PPT, TWLDS, tw_mode<LAY, L, NSEQ, RN, NSN, T, PPT, LASTN>()>(
+          tw + OFFN, tid, wn);
+      __syncthreads();
+      FftPassesPF<LAY, L, NSEQ, T, PPT, P + 1, V, TWLDS>::run(lds, v, wn, tw, tid);
+    }
   }
 };
 
@@ -375,6 +561,14 @@ struct FftPasses {
 template <class LAY, int L, int NSEQ, int T, int PPT, int V = 0, bool TWLDS = false>
 __device__ __forceinline__ void fft_tile(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
   FftPasses<LAY, L, NSEQ, T, PPT, 0, V, TWLDS>::run(lds, v, tw, tid);
+}
+
+// fft_tile with every pass's twiddles loaded before the barrier that starts
+// the pass (same arguments and result).
+template <class LAY, int L, int NSEQ, int T, int PPT, int V = 0, bool TWLDS = false>
+__device__ __forceinline__ void fft_tile_pf(float2* lds, float2 (&v)[PPT], const float2* tw, int tid) {
+  float2 w0[PPT];  // pass 0 has sub-transform length 1: no twiddles
+  FftPassesPF<LAY, L, NSEQ, T, PPT, 0, V, TWLDS>::run(lds, v, w0, tw, tid);
 }
 
 // ---------------------------------------------------------------------------

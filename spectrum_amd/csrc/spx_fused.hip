@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
         load_run<R0, MODE == 0>(a, ch, (uint32_t)j, (uint32_t)(L / R0), &v[b * R0]);
       }
     }
-    fft_tile<LAY, L, 1, T, PPT>(lds, v, a.tw_a, tid);
+    fft_tile_pf<LAY, L, 1, T, PPT>(lds, v, a.tw_a, tid);
     // v[b*RL + r] = X[j + r*L/RL], j = tid + b*T < L/RL
 
     const uint64_t cbase = (uint64_t)ch * L;
@@ -104,10 +104,49 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
       float bm = -1.0f;
       uint32_t bk = 0xffffffffu;
       const bool owner = tid < C::OWN;
-      if (owner) {
+      const uint32_t k0 = (uint32_t)tid * E;
+      // whole band (the default): no per-bin range test
+      const bool full = a.s0 == 0 && a.s1 >= (uint32_t)L;
+      float ml = 0.f, mr = 0.f;
+      if constexpr (E == 16) {
+        // every thread owns a run of 16 bins, which shares one pad slot and
+        // so is contiguous in LDS; the run and both circular neighbours are
+        // read before any of them is used
+        const float2* run = lds + LAY::idx(0, k0);
+        float2 xv[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) xv[e] = run[e];
+        const float2 xl = lds[LAY::idx(0, (k0 + L - 1) % L)];
+        const float2 xr = lds[LAY::idx(0, (k0 + E) % L)];
+#pragma unroll
+        for (int e = 0; e < E; ++e) m[e] = mag_of(xv[e]);
+        if (full) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            fs1 += m[e];
+            fs2 = fmaf(m[e], m[e], fs2);
+            const bool up = m[e] > bm;
+            bm = up ? m[e] : bm;
+            bk = up ? k0 + e : bk;
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const bool in = in_range(k0 + e, a);
+            const float mv = in ? m[e] : 0.0f;
+            fs1 += mv;
+            fs2 = fmaf(mv, mv, fs2);
+            const bool up = in && m[e] > bm;
+            bm = up ? m[e] : bm;
+            bk = up ? k0 + e : bk;
+          }
+        }
+        ml = mag_of(xl);
+        mr = mag_of(xr);
+      } else if (owner) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const uint32_t k = tid * E + e;
+          const uint32_t k = k0 + e;
           m[e] = mag_of(lds[LAY::idx(0, k)]);
           if (in_range(k, a)) {
             fs1 += m[e];
@@ -115,6 +154,8 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
             if (m[e] > bm) { bm = m[e]; bk = k; }
           }
         }
+        ml = mag_of(lds[LAY::idx(0, (k0 + L - 1) % L)]);
+        mr = mag_of(lds[LAY::idx(0, (k0 + E) % L)]);
       }
       s1 = fs1;
       s2 = fs2;
@@ -125,24 +166,22 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
       float mu, sd;
       const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
 
-      // neighbours of the owned run (circular)
-      float ml = 0.f, mr = 0.f;
+      // local maxima of the owned run (circular neighbours)
       uint32_t flags = 0;
-      int c = 0;
       if (owner) {
-        ml = mag_of(lds[LAY::idx(0, (tid * E + L - 1) % L)]);
-        mr = mag_of(lds[LAY::idx(0, (tid * E + E) % L)]);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const uint32_t k = tid * E + e;
           const float yl = (e == 0) ? ml : m[e - 1];
           const float yr = (e == E - 1) ? mr : m[e + 1];
-          if (in_range(k, a) && m[e] > yl && m[e] > yr && m[e] > thr) {
-            flags |= 1u << e;
-            ++c;
-          }
+          const bool pk = m[e] > yl && m[e] > yr && m[e] > thr;
+          flags |= pk ? (1u << e) : 0u;
+        }
+        if (!full) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) flags &= in_range(k0 + e, a) ? ~0u : ~(1u << e);
         }
       }
+      const int c = __builtin_popcount(flags);
       int total;
       int off = block_excl_scan<T, false>(c, total, red);
       if (owner && flags && off < (int)a.kcap) {

@@ -16,16 +16,36 @@ using MaxValue = spx_max_value;
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
+// Global-memory element at base + a 32-bit byte offset.  The base is cast to
+// the global address space before the offset is added, so the load takes the
+// scalar-base form (global_load v, v_off, s[base]) with one offset VGPR; an
+// element index, or a cast after the add, needs a 64-bit address per load (a
+// channel and a window table are far below 4 GiB).
+template <class T, bool STREAM = false>
+__device__ __forceinline__ T ld_at(const void* base, uint32_t off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(1))) const char gchar;
+  typedef __attribute__((address_space(1))) const T gT;
+  gT* p = (gT*)((gchar*)base + off);
+  if constexpr (STREAM) return __builtin_nontemporal_load(p);
+  else return *p;
+#else
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+#endif
+}
+
 // STREAM: non-temporal complex64 loads
 template <int DT, bool STREAM = false>
 __device__ __forceinline__ float2 ld_raw(const void* chan_base, uint32_t i) {
   if constexpr (DT == SPX_DTYPE_C64) {
-    return ld_f2<STREAM>(reinterpret_cast<const float2*>(chan_base) + i);
+    const stream_float2 v = ld_at<stream_float2, STREAM>(chan_base, i * 8u);
+    return make_float2(v.x, v.y);
   } else if constexpr (DT == SPX_DTYPE_C32F16) {
-    const __half2 h = gp(reinterpret_cast<const __half2*>(chan_base))[i];
-    return __half22float2(h);
+    const uint32_t u = ld_at<uint32_t>(chan_base, i * 4u);
+    return make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu)),
+                       (float)__builtin_bit_cast(_Float16, (uint16_t)(u >> 16)));
   } else {
-    const uint32_t u = gp(reinterpret_cast<const uint32_t*>(chan_base))[i];
+    const uint32_t u = ld_at<uint32_t>(chan_base, i * 4u);
     return make_float2(bf16_to_f((uint16_t)(u & 0xffffu)), bf16_to_f((uint16_t)(u >> 16)));
   }
 }
@@ -45,7 +65,7 @@ __device__ __forceinline__ void load_run_t(const FusedArgs& a, uint64_t ch, uint
   float wv[R];
   if (a.window != nullptr) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) wv[r] = gp(a.window)[PAD ? min(n0 + r * stride, last) : n0 + r * stride];
+    for (int r = 0; r < R; ++r) wv[r] = ld_at<float>(a.window, 4u * (PAD ? min(n0 + r * stride, last) : n0 + r * stride));
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) wv[r] = 1.0f;
@@ -278,10 +298,11 @@ __device__ __forceinline__ void block_stats(double& s1, double& s2, float& bm, u
   for (int i = 0; i < NW; ++i) {
     s1 += red.s1[i];
     s2 += red.s2[i];
-    if (red.m[i] > bm || (red.m[i] == bm && red.k[i] < bk)) {
-      bm = red.m[i];
-      bk = red.k[i];
-    }
+    const float mi = red.m[i];
+    const uint32_t ki = red.k[i];
+    const bool up = (mi > bm) | ((mi == bm) & (ki < bk));
+    bm = up ? mi : bm;
+    bk = up ? ki : bk;
   }
   if constexpr (REUSE) __syncthreads();
 }
