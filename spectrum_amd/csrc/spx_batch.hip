@@ -76,7 +76,7 @@ __device__ __forceinline__ void batch_small_tile(const BatchArgs& b, const uint4
       for (int r = 0; r < R0; ++r) v[q * R0 + r] = make_float2(0.f, 0.f);
     }
   }
-  fft_tile<LAY, L, G, T, PPT, V>(lds, v, tw, tid);
+  fft_tile_pf<LAY, L, G, T, PPT, V>(lds, v, tw, tid);
   __syncthreads();  // the last pass read LDS
 #pragma unroll
   for (int q = 0; q < PPT / RL; ++q) {

@@ -263,6 +263,7 @@ struct Plan {
 // ---------------------------------------------------------------------------
 template <int L, int NSEQ>
 struct LayContig {
+  static constexpr bool CONTIG = true;
   static constexpr int SIZE = L * NSEQ + (L * NSEQ) / 16;
   __device__ __forceinline__ static int idx(int s, int i) {
     int a = s * L + i;
@@ -279,6 +280,7 @@ struct LayContig {
 
 template <int L, int NSEQ>
 struct LayInter {
+  static constexpr bool CONTIG = false;
   static constexpr int SIZE = L * NSEQ;
   __device__ __forceinline__ static int idx(int s, int i) { return i * NSEQ + s; }
   static constexpr int rstride(int R) { return (L / R) * NSEQ; }
@@ -410,10 +412,12 @@ struct FftPasses {
 // barrier wait and the LDS reads instead of after them.  Fewer bytes are read
 // than the R-entry rows hold (the vector-memory path, not the table, limits):
 //  TW_ROWS  w[b*R + r] = W_(NS*R)^(r*(jj%NS)), the table rows as they are;
-//  TW_LAST  last pass of one sequence with T = L/16 threads: butterfly
-//           j = tid + b*T has k = j, and W_L^(r*j) = W_L^(r*tid) W16^(r*b),
-//           so only the row of k = tid is read (w[r]) and the W16 factors
-//           are compile-time rotations;
+//  TW_LAST  last pass (NS = L/R) of a contiguous tile whose butterfly
+//           index j = tid + b*T gives k = j mod L/R = tid mod L/R + q_b with
+//           q_b = (b*T) mod L/R a multiple of L/16 (T and L/R powers of two,
+//           L <= 16 T): W_L^(r k) = W_L^(r (tid mod L/R)) W16^(r q_b 16/L), so
+//           one row is read (w[r]) and the W16 factors are compile-time
+//           rotations;
 //  TW_FACT  radix 16, one butterfly per thread: W^(r k) = W^((r&3) k)
 //           W^((r&12) k), so entries 1, 2, 3, 4, 8, 12 of the row are read
 //           (w[0..5] = W^(k), W^(2k), W^(3k), W^(4k), W^(8k), W^(12k)) and the
@@ -423,7 +427,9 @@ enum { TW_ROWS = 0, TW_LAST = 1, TW_FACT = 2 };
 template <class LAY, int L, int NSEQ, int R, int NS, int T, int PPT, bool LAST>
 constexpr int tw_mode() {
   if constexpr (NS <= 1) return TW_ROWS;
-  else if constexpr (LAST && NSEQ == 1 && PPT / R > 1 && T * 16 == L && NS * R == L) return TW_LAST;
+  else if constexpr (LAST && LAY::CONTIG && PPT / R > 1 && NS * R == L && (16 * T) % L == 0 &&
+                     ((L / R) % T == 0 || T % (L / R) == 0))
+    return TW_LAST;
   else if constexpr (R == 16 && PPT == 16) return TW_FACT;
   else return TW_ROWS;
 }
@@ -435,7 +441,7 @@ __device__ __forceinline__ void pass_twiddles(const float2* __restrict__ twp, in
   constexpr bool FULL = (NB == T * BPT);
   constexpr int RW = TWLDS ? R + 2 : R;  // row pitch
   if constexpr (TWM == TW_LAST) {
-    load_tw_row<R, TWLDS>(twp + (uint32_t)(tid * RW), w);
+    load_tw_row<R, TWLDS>(twp + (uint32_t)((tid % (L / R)) * RW), w);
   } else if constexpr (TWM == TW_FACT) {
     int s, jj;
     LAY::template map<R>(tid, s, jj);
@@ -458,10 +464,10 @@ __device__ __forceinline__ void pass_twiddles(const float2* __restrict__ twp, in
 }
 
 // twiddle of element r of butterfly b from what pass_twiddles read
-template <int R, int TWM>
+template <int L, int R, int T, int TWM>
 __device__ __forceinline__ float2 pass_tw(const float2* w, int b, int r) {
   if constexpr (TWM == TW_LAST) {
-    return tw16_rt(r * b, w[r]);
+    return tw16_rt(r * (((b * T) % (L / R)) * 16 / L), w[r]);
   } else if constexpr (TWM == TW_FACT) {
     const int lo = r & 3, hi = r >> 2;  // w[lo - 1] = W^(lo k), w[2 + hi] = W^(4 hi k)
     if (hi == 0) return w[lo - 1];
@@ -505,7 +511,7 @@ __device__ __forceinline__ void fft_pass_w(float2* __restrict__ lds, float2 (&v)
     if (FULL || j < NB) {
       if constexpr (NS > 1) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], pass_tw<R, TWM>(w, b, r));
+        for (int r = 1; r < R; ++r) v[b * R + r] = c_mul(v[b * R + r], pass_tw<L, R, T, TWM>(w, b, r));
       }
       Dft<R>::run(&v[b * R]);
     }
