@@ -123,14 +123,16 @@ __device__ __forceinline__ void load_ops(const uint32_t* p, f16x8& re, f16x8& im
 }
 
 // one channel's stage-1 input for this lane: x[1024 n1 + n2], n1 = 8(g&1)+j, n2 = 16(w+8t)+c
+// (32-bit byte offsets from the channel's scalar base: one offset VGPR, the
+// (t, j) steps as immediates or one add each)
 __device__ __forceinline__ void load_raw(const FusedArgs& a, uint32_t ch, int w, int g, int c, uint32_t (&raw)[8][8]) {
-  const uint32_t* in = reinterpret_cast<const uint32_t*>(a.in) + (uint64_t)ch * mx::N;
+  const char* base = reinterpret_cast<const char*>(a.in) + (uint64_t)ch * (mx::N * 4u);
+  const uint32_t lo = 4u * ((uint32_t)(g & 1) * 8192u + 16u * (uint32_t)w + (uint32_t)c);
+  // the (t, j) step goes into the (scalar) base, the lane part is the offset
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const int n2 = 16 * (w + 8 * t) + c;
+  for (int t = 0; t < 8; ++t)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) raw[t][j] = gp(in)[(8 * (g & 1) + j) * 1024 + n2];
-  }
+    for (int j = 0; j < 8; ++j) raw[t][j] = ld_at<uint32_t>(base + (4096u * j + 512u * t), lo);
 }
 
 // Threshold peak search over the FP32 spectrum planes (bin k at k + k/32);
@@ -154,20 +156,39 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
   uint32_t bk = 0xffffffffu;
   uint32_t lmax = 0;
   {
-    float yl = mag_of(xat((k0 + N - 1) % N));
-    float yc = mag_of(xat(k0));
-#pragma unroll 8
-    for (int e = 0; e < E; ++e) {
-      const uint32_t k = k0 + e;
-      const float yr = mag_of(xat((k + 1) % N));
-      if (full || in_range(k, a)) {
-        fs1 += yc;
-        fs2 = fmaf(yc, yc, fs2);
-        if (yc > bm) { bm = yc; bk = k; }
-        if (yc > yl && yc > yr && yc > lb) lmax |= 1u << e;
+    // the thread's 32 bins sit at 33 t + e in both planes (one pad slot per
+    // 32 bins): all reads are issued before the first use, then the sums,
+    // argmax and local-maximum bits are selects, with no per-bin branch on
+    // the (uniform) whole-band case
+    float m[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) m[e] = mag_of(make_float2(xre[adr(k0) + e], xim[adr(k0) + e]));
+    const float ml = mag_of(xat((k0 + N - 1) % N));
+    const float mr = mag_of(xat((k0 + E) % N));
+    if (full) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float yl = e == 0 ? ml : m[e - 1], yr = e == E - 1 ? mr : m[e + 1];
+        fs1 += m[e];
+        fs2 = fmaf(m[e], m[e], fs2);
+        const bool up = m[e] > bm;
+        bm = up ? m[e] : bm;
+        bk = up ? k0 + e : bk;
+        lmax |= (m[e] > yl && m[e] > yr && m[e] > lb) ? 1u << e : 0u;
       }
-      yl = yc;
-      yc = yr;
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float yl = e == 0 ? ml : m[e - 1], yr = e == E - 1 ? mr : m[e + 1];
+        const bool in = in_range(k0 + e, a);
+        const float mv = in ? m[e] : 0.0f;
+        fs1 += mv;
+        fs2 = fmaf(mv, mv, fs2);
+        const bool up = in && m[e] > bm;
+        bm = up ? m[e] : bm;
+        bk = up ? k0 + e : bk;
+        lmax |= (in && m[e] > yl && m[e] > yr && m[e] > lb) ? 1u << e : 0u;
+      }
     }
   }
   double s1 = fs1, s2 = fs2;
@@ -337,17 +358,25 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
           cp = cc;
           cc = cn;
         }
+        // the block's four twiddles W_16384^(n2 k1), k1 = 4g + i, read from
+        // the LDS tables before the MFMAs and before any store of the block
+        // (the stores could alias the tables for all the compiler knows)
+        float2 tw[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int e = n2 * (4 * g + i);  // < 16384
+          tw[i] = c_mul(tal[e & 127], tah[e >> 7]);
+        }
         const f32x4 z = {0.f, 0.f, 0.f, 0.f};
         const f32x4 dre = Mma<CT>::run(w0h, w0l, b, z);
         const f32x4 dim = Mma<CT>::run(w1h, w1l, b, z);
         const int m1 = n2 >> 5, m2 = n2 & 31;
+        // row k1 = 4g + i of [k1][m2][m1]: base + i * 32 * PSTR (an immediate)
+        uint32_t* pb = ping + (128 * g + m2) * PSTR + m1;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int k1 = 4 * g + i;
-          const int e = n2 * k1;  // < 16384
-          const float2 tw = c_mul(tal[e & 127], tah[e >> 7]);
-          const float2 y = c_mul(make_float2(dre[i], dim[i]), tw);
-          ping[(k1 * 32 + m2) * PSTR + m1] = pack_h2(y.x, y.y);
+          const float2 y = c_mul(make_float2(dre[i], dim[i]), tw[i]);
+          pb[i * 32 * PSTR] = pack_h2(y.x, y.y);
         }
       }
       if constexpr (MODE == 1) {
@@ -395,6 +424,11 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
       const int k1 = cb >> 1, m2 = 16 * (cb & 1) + c;
       f16x8 b0, b1;
       load_ops(ping + (k1 * 32 + m2) * PSTR + 8 * g, b0, b1);
+      float2 twb[8];  // stage-2 twiddles of the block's outputs, read before its stores
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) twb[4 * h + i] = tb[(16 * h + 4 * g + i) * 32 + m2];
       f32x4 d[4];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
@@ -402,13 +436,14 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
         d[rb] = Mma<CT>::run(wh[rb * 2 + 0], wl[rb * 2 + 0], b0, z);
         d[rb] = Mma<CT>::run(wh[rb * 2 + 1], wl[rb * 2 + 1], b1, d[rb]);
       }
+      // row j1 = 16h + 4g + i of [j1][k1][m2]: base + (16h + i) * 16 * PSTR
+      uint32_t* qb = pong + (64 * g + k1) * PSTR + m2;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int j1 = 16 * h + 4 * g + i;
-          const float2 y = c_mul(make_float2(d[h][i], d[2 + h][i]), tb[j1 * 32 + m2]);  // tb carries the 2^-5
-          pong[(j1 * 16 + k1) * PSTR + m2] = pack_h2(y.x, y.y);
+          const float2 y = c_mul(make_float2(d[h][i], d[2 + h][i]), twb[4 * h + i]);  // tb carries the 2^-5
+          qb[(16 * h + i) * 16 * PSTR] = pack_h2(y.x, y.y);
         }
     }
     __syncthreads();
@@ -438,8 +473,8 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
             gp(a.spec_out + (uint64_t)ch * N)[c + 16 * j1 + 512 * j2] =
                 make_float2(d[h][i] * s_out, d[2 + h][i] * s_out);
           } else {
-            const int k = c + 16 * j1 + 512 * j2;
-            xre[k + (k >> 5)] = d[h][i] * s_out;
+            // k = c + 16 j1 + 512 j2 sits at k + k/32 = c + 16 j1 + j1/2 + 528 j2
+            xre[c + 16 * j1 + (j1 >> 1) + 528 * 4 * g + 528 * (16 * h + i)] = d[h][i] * s_out;
           }
         }
       dim3[t][0] = d[2];
@@ -457,11 +492,8 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int j2 = 16 * h + 4 * g + i;
-            const int k = c + 16 * j1 + 512 * j2;
-            xim[k + (k >> 5)] = dim3[t][h][i] * s_out;
-          }
+          for (int i = 0; i < 4; ++i)
+            xim[c + 16 * j1 + (j1 >> 1) + 528 * 4 * g + 528 * (16 * h + i)] = dim3[t][h][i] * s_out;
       }
       __syncthreads();
       mixed_peaks<LEARN>(a, ch, xre, xim, lb, red);
