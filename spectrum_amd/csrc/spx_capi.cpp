@@ -704,12 +704,15 @@ std::vector<float> make_mixed_tables(uint32_t compute) {
       *lo = f32_to_bf16_bits((float)(v - (double)bf16_bits_to_f32(h)));
     }
   };
+  // DFT-16: the K = 32 operand rows are ordered (n1, re/im), K row q holds
+  // component q & 1 of input n1 = q / 2 (matrix column q / 2 or 16 + q / 2)
   uint16_t* W16 = reinterpret_cast<uint16_t*>(p + 1024 * 8 + 256 * 8);
   for (int rb = 0; rb < 2; ++rb)
     for (int l = 0; l < 64; ++l)
       for (int j = 0; j < 8; ++j) {
         const size_t i = ((size_t)rb * 64 + l) * 8 + j;
-        put(W16 + i, W16 + 2 * 64 * 8 + i, M(16, 16 * rb + l % 16, 8 * (l / 16) + j));
+        const int q = 8 * (l / 16) + j;
+        put(W16 + i, W16 + 2 * 64 * 8 + i, M(16, 16 * rb + l % 16, (q & 1) * 16 + q / 2));
       }
   uint16_t* W32 = W16 + 2 * 2 * 64 * 8;
   for (int rb = 0; rb < 4; ++rb)

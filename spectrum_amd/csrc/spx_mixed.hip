@@ -122,17 +122,17 @@ __device__ __forceinline__ void load_ops(const uint32_t* p, f16x8& re, f16x8& im
   }
 }
 
-// one channel's stage-1 input for this lane: x[1024 n1 + n2], n1 = 8(g&1)+j, n2 = 16(w+8t)+c
-// (32-bit byte offsets from the channel's scalar base: one offset VGPR, the
-// (t, j) steps as immediates or one add each)
-__device__ __forceinline__ void load_raw(const FusedArgs& a, uint32_t ch, int w, int g, int c, uint32_t (&raw)[8][8]) {
+// one channel's stage-1 input for this lane: words x[1024 n1 + n2] for
+// n1 = 4g + m (m < 4), n2 = 16(w + 8t) + c, as 32-bit byte offsets from the
+// channel's scalar base (the (t, m) step in the base, the lane part in one
+// offset VGPR); every word is loaded by exactly one lane
+__device__ __forceinline__ void load_raw(const FusedArgs& a, uint32_t ch, int w, int g, int c, uint32_t (&raw)[8][4]) {
   const char* base = reinterpret_cast<const char*>(a.in) + (uint64_t)ch * (mx::N * 4u);
-  const uint32_t lo = 4u * ((uint32_t)(g & 1) * 8192u + 16u * (uint32_t)w + (uint32_t)c);
-  // the (t, j) step goes into the (scalar) base, the lane part is the offset
+  const uint32_t lo = 4u * ((uint32_t)g * 4096u + 16u * (uint32_t)w + (uint32_t)c);
 #pragma unroll
   for (int t = 0; t < 8; ++t)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) raw[t][j] = ld_at<uint32_t>(base + (4096u * j + 512u * t), lo);
+    for (int m = 0; m < 4; ++m) raw[t][m] = ld_at<uint32_t>(base + (4096u * m + 512u * t), lo);
 }
 
 // Threshold peak search over the FP32 spectrum planes (bin k at k + k/32);
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
   }
   uint32_t ch = blockIdx.x;
   if (ch >= a.batch) return;
-  uint32_t raw[8][8];
+  uint32_t raw[8][4];
   load_raw(a, ch, w, g, c, raw);
   __syncthreads();
 
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
 #pragma unroll
     for (int t = 0; t < 8; ++t)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) raw[t][j] = (uint32_t)opaque_v((int)raw[t][j]);
+      for (int m = 0; m < 4; ++m) raw[t][m] = (uint32_t)opaque_v((int)raw[t][m]);
     // ---- per-channel power-of-two pre-scale from the largest component:
     // |re| and |im| as 16-bit patterns (sign cleared) order like their values
     // for fp16 and bf16 alike, so one packed integer max per word
@@ -294,8 +294,8 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
 #pragma unroll
     for (int t = 0; t < 8; ++t)
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        mxp = __builtin_elementwise_max(mxp, __builtin_bit_cast(u16x2, raw[t][j] & 0x7fff7fffu));
+      for (int m = 0; m < 4; ++m)
+        mxp = __builtin_elementwise_max(mxp, __builtin_bit_cast(u16x2, raw[t][m] & 0x7fff7fffu));
     const uint32_t mxb = (uint32_t)max(mxp[0], mxp[1]);
     float mx_ = DT == SPX_DTYPE_C32F16 ? (float)__builtin_bit_cast(_Float16, (uint16_t)mxb) : __uint_as_float(mxb << 16);
     mx_ = wave_maxf(mx_);
@@ -321,9 +321,9 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
       }
       // window times pre-scale, w(n) s = c0 s - c1 s cos(th) + c2 s cos(2 th),
       // th = 2 pi n / (N - 1), as the polynomial P(cos th) = (c0 - c2) s +
-      // cos th (-c1 s + 2 c2 s cos th); the lane's 8 samples of block t are
-      // n0 + 1024 j, so cos th follows the recurrence
-      // cos((j+1) d) = 2 cos(d) cos(j d) - cos((j-1) d) (FP32, 7 steps)
+      // cos th (-c1 s + 2 c2 s cos th); the lane's 4 samples of block t are
+      // n0 + 1024 m, so cos th follows the recurrence
+      // cos((m+1) d) = 2 cos(d) cos(m d) - cos((m-1) d) (FP32, 3 steps)
       const float winv = 1.0f / (float)(N - 1);
       const float dr = 1024.0f * winv;  // revolutions per step
       const float two_cd = 2.0f * __builtin_amdgcn_cosf(dr);
@@ -331,33 +331,28 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
       const float p0 = win ? (a.wc0 - a.wc2) * s_in : s_in;
       const float p1 = win ? -a.wc1 * s_in : 0.0f;
       const float p2 = win ? 2.0f * a.wc2 * s_in : 0.0f;
-      // the lane's component: re (lanes g < 2) or im (g >= 2) of each word
-      const uint32_t sh = (g < 2) ? 0u : 16u;
-      float en = 0.f;  // energy of the scaled, windowed operands (MODE 1)
+      float en = 0.f;  // energy of the fp16 operands (MODE 1)
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         const int n2 = 16 * (w + 8 * t) + c;
-        const float r0 = (float)((8 * (g & 1)) * 1024 + n2) * winv;
+        const float r0 = (float)((4 * g) * 1024 + n2) * winv;
         float cp = __builtin_amdgcn_cosf(r0 - dr), cc = __builtin_amdgcn_cosf(r0);
-        f16x8 b;
+        // K = 32 operand rows ordered (n1, re/im): the lane's rows 8g .. 8g+7
+        // are re, im of its four samples, i.e. its four words times w(n) s
+        // (FP32 product, one rounding to fp16)
+        uint32_t bw[4];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int m = 0; m < 4; ++m) {
           const float ws = fmaf(fmaf(p2, cc, p1), cc, p0);
-          const uint32_t u = raw[t][j] >> sh;
-          float comp;
-          if constexpr (DT == SPX_DTYPE_C32F16) comp = (float)__builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu));
-          else comp = __uint_as_float(u << 16);
-          if constexpr (MODE == 1) {
-            const float pv = comp * ws;
-            b[j] = (_Float16)pv;
-            en = fmaf(pv, pv, en);
-          } else {
-            b[j] = (_Float16)__builtin_fmaf(comp, ws, 0.0f);  // one v_fma_mix (fp16 in, FP32 math, fp16 out)
-          }
+          const float2 x = unpack_in<DT>(raw[t][m]);
+          const f16x2 h = {(_Float16)__builtin_fmaf(x.x, ws, 0.0f), (_Float16)__builtin_fmaf(x.y, ws, 0.0f)};
+          bw[m] = __builtin_bit_cast(uint32_t, h);
+          if constexpr (MODE == 1) en = __builtin_amdgcn_fdot2(h, h, en, false);
           const float cn = fmaf(two_cd, cc, -cp);
           cp = cc;
           cc = cn;
         }
+        const f16x8 b = __builtin_bit_cast(f16x8, make_uint4(bw[0], bw[1], bw[2], bw[3]));
         // the block's four twiddles W_16384^(n2 k1), k1 = 4g + i, read from
         // the LDS tables before the MFMAs and before any store of the block
         // (the stores could alias the tables for all the compiler knows)
