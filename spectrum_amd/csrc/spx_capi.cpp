@@ -714,13 +714,16 @@ std::vector<float> make_mixed_tables(uint32_t compute) {
         const int q = 8 * (l / 16) + j;
         put(W16 + i, W16 + 2 * 64 * 8 + i, M(16, 16 * rb + l % 16, (q & 1) * 16 + q / 2));
       }
+  // DFT-32: K = 64 rows ordered (m, re/im) as well; k-step ks row q holds
+  // component q & 1 of input m = 16 ks + q / 2
   uint16_t* W32 = W16 + 2 * 2 * 64 * 8;
   for (int rb = 0; rb < 4; ++rb)
     for (int ks = 0; ks < 2; ++ks)
       for (int l = 0; l < 64; ++l)
         for (int j = 0; j < 8; ++j) {
           const size_t i = (((size_t)(rb * 2 + ks)) * 64 + l) * 8 + j;
-          put(W32 + i, W32 + 8 * 64 * 8 + i, M(32, 16 * rb + l % 16, 32 * ks + 8 * (l / 16) + j));
+          const int q = 8 * (l / 16) + j;
+          put(W32 + i, W32 + 8 * 64 * 8 + i, M(32, 16 * rb + l % 16, (q & 1) * 32 + 16 * ks + q / 2));
         }
   return t;
 }

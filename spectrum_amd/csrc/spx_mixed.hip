@@ -12,7 +12,10 @@
 //   X[k]           = sum_m2 B'[k1][j1][m2] W32^(m2 j2)            (stage 3, DFT-32)
 // A complex DFT-R is the real 2R x 2R matrix [[C, S], [-S, C]] (C = cos,
 // S = sin of 2 pi r q / R) acting on [Re; Im]; the data of 16 sequences are
-// the B operand (K = 32 real inputs per MFMA), the DFT matrix the A operand.
+// the B operand (K = 32 real inputs per MFMA), the DFT matrix the A operand,
+// with its K columns permuted to the order (input, re/im): a lane's eight K
+// rows are then four complex inputs exactly as they sit in memory (half2
+// words), with no splitting into re and im vectors.
 // MFMA 16x16x32 lane layout (checked on the device): lane l holds
 // A[l%16][8(l/16) + j], B[8(l/16) + j][l%16], D[4(l/16) + i][l%16].
 //
@@ -109,17 +112,13 @@ __device__ __forceinline__ uint32_t pack_h2(float re, float im) {
   return __builtin_bit_cast(uint32_t, h);
 }
 
-// 8 consecutive half2 {re, im} at p (16-byte aligned) -> re and im operand vectors
-__device__ __forceinline__ void load_ops(const uint32_t* p, f16x8& re, f16x8& im) {
-  const uint4 q0 = *reinterpret_cast<const uint4*>(p);
-  const uint4 q1 = *reinterpret_cast<const uint4*>(p + 4);
-  const uint32_t u[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const f16x2 h = __builtin_bit_cast(f16x2, u[j]);
-    re[j] = h[0];
-    im[j] = h[1];
-  }
+// DFT-32 operands of lane group g for one 16-column block: the K = 64 rows
+// are ordered (m, re/im), so k-step 0 holds inputs 4g .. 4g+3 and k-step 1
+// inputs 16 + 4g .. 16 + 4g + 3 of the row, each as its half2 {re, im} word
+// (row: 32 words, 16-byte aligned)
+__device__ __forceinline__ void load_ops(const uint32_t* row, int g, f16x8& b0, f16x8& b1) {
+  b0 = __builtin_bit_cast(f16x8, *reinterpret_cast<const uint4*>(row + 4 * g));
+  b1 = __builtin_bit_cast(f16x8, *reinterpret_cast<const uint4*>(row + 16 + 4 * g));
 }
 
 // one channel's stage-1 input for this lane: words x[1024 n1 + n2] for
@@ -418,7 +417,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
       const int cb = w + 8 * t;
       const int k1 = cb >> 1, m2 = 16 * (cb & 1) + c;
       f16x8 b0, b1;
-      load_ops(ping + (k1 * 32 + m2) * PSTR + 8 * g, b0, b1);
+      load_ops(ping + (k1 * 32 + m2) * PSTR, g, b0, b1);
       float2 twb[8];  // stage-2 twiddles of the block's outputs, read before its stores
 #pragma unroll
       for (int h = 0; h < 2; ++h)
@@ -451,7 +450,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
     for (int t = 0; t < 4; ++t) {
       const int j1 = w + 8 * t;
       f16x8 b0, b1;
-      load_ops(pong + (j1 * 16 + c) * PSTR + 8 * g, b0, b1);
+      load_ops(pong + (j1 * 16 + c) * PSTR, g, b0, b1);
       f32x4 d[4];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
