@@ -3,6 +3,7 @@
 // helpers, block reductions, the first-K survivor selection and the
 // ONE_PEAK / TWO_PEAKS record writers.
 #pragma once
+#include <type_traits>
 #include "spx_device.hpp"
 #include "spx_types.h"
 #include "spx_kernels.h"
@@ -273,12 +274,36 @@ __device__ __forceinline__ float threshold_of(const FusedArgs& a, double s1, dou
   return -__builtin_inff();
 }
 
+// Cross-wave combine of the NW <= 16 per-wave partials in LDS, done by every
+// wave lane-parallel instead of an NW-step loop in every lane: lane i < NW
+// takes wave i's partial and a prefix pass over lanes 0-15 (DPP row shifts
+// 1, 2, 4, 8 with the identity shifted in) leaves the combined value of
+// lanes 0 .. p in lane p; lane NW - 1 holds the total.
+namespace rowred {
+template <int CTRL, class V>
+__device__ __forceinline__ V shr(V id, V x) {
+  if constexpr (sizeof(V) == 8) return dpp::mov<CTRL, 0x1>(id, x);
+  else if constexpr (__is_same(V, float)) return dpp::mov<CTRL, 0x1>(id, x);
+  else return (V)dpp::mov<CTRL, 0x1>((int)id, (int)x);
+}
+template <class V>
+__device__ __forceinline__ V lane_of(V x, int i) {
+  if constexpr (sizeof(V) == 8) {
+    const int2 a = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(V, make_int2(__builtin_amdgcn_readlane(a.x, i), __builtin_amdgcn_readlane(a.y, i)));
+  } else {
+    return __builtin_bit_cast(V, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), i));
+  }
+}
+}  // namespace rowred
+
 // Block-wide sum of two doubles and argmax (lowest index wins ties).
 // Result valid in all threads on return.  REUSE: end with a barrier so that
 // `red` may be written again (not needed when it is the last use).
 template <int T, bool REUSE = true>
 __device__ __forceinline__ void block_stats(double& s1, double& s2, float& bm, uint32_t& bk, BlockRed& red) {
   constexpr int NW = (T + 63) / 64;
+  static_assert(NW <= 16, "at most 16 waves");
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   s1 = wave_sum(s1);
   s2 = wave_sum(s2);
@@ -290,20 +315,28 @@ __device__ __forceinline__ void block_stats(double& s1, double& s2, float& bm, u
     red.k[w] = bk;
   }
   __syncthreads();
-  s1 = 0;
-  s2 = 0;
-  bm = -1.0f;
-  bk = 0xffffffffu;
-#pragma unroll
-  for (int i = 0; i < NW; ++i) {
-    s1 += red.s1[i];
-    s2 += red.s2[i];
-    const float mi = red.m[i];
-    const uint32_t ki = red.k[i];
-    const bool up = (mi > bm) | ((mi == bm) & (ki < bk));
-    bm = up ? mi : bm;
-    bk = up ? ki : bk;
-  }
+  const bool in = lane < NW;
+  double a1 = in ? red.s1[lane] : 0.0, a2 = in ? red.s2[lane] : 0.0;
+  float m = in ? red.m[lane] : -1.0f;
+  uint32_t k = in ? red.k[lane] : 0xffffffffu;
+  auto step = [&](auto ctrl) {
+    constexpr int C = decltype(ctrl)::value;
+    a1 += rowred::shr<C>(0.0, a1);
+    a2 += rowred::shr<C>(0.0, a2);
+    const float om = rowred::shr<C>(-1.0f, m);
+    const uint32_t ok = rowred::shr<C>(0xffffffffu, k);
+    const bool up = (om > m) | ((om == m) & (ok < k));
+    m = up ? om : m;
+    k = up ? ok : k;
+  };
+  if constexpr (NW > 1) step(std::integral_constant<int, dpp::kShr1>{});
+  if constexpr (NW > 2) step(std::integral_constant<int, dpp::kShr2>{});
+  if constexpr (NW > 4) step(std::integral_constant<int, dpp::kShr4>{});
+  if constexpr (NW > 8) step(std::integral_constant<int, dpp::kShr8>{});
+  s1 = rowred::lane_of(a1, NW - 1);
+  s2 = rowred::lane_of(a2, NW - 1);
+  bm = rowred::lane_of(m, NW - 1);
+  bk = rowred::lane_of(k, NW - 1);
   if constexpr (REUSE) __syncthreads();
 }
 
@@ -313,16 +346,19 @@ __device__ __forceinline__ void block_stats(double& s1, double& s2, float& bm, u
 template <int T, bool REUSE = true>
 __device__ __forceinline__ int block_excl_scan(int c, int& total, BlockRed& red) {
   constexpr int NW = (T + 63) / 64;
+  static_assert(NW <= 16, "at most 16 waves");
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int inc = wave_incl_scan(c);
   if (lane == 63) red.c[w] = inc;
   __syncthreads();
-  int off = 0, tot = 0;
-#pragma unroll
-  for (int i = 0; i < NW; ++i) {
-    if (i < w) off += red.c[i];
-    tot += red.c[i];
-  }
+  // inclusive scan of the wave totals over lanes 0 .. NW - 1
+  int x = lane < NW ? red.c[lane] : 0;
+  if constexpr (NW > 1) x += rowred::shr<dpp::kShr1>(0, x);
+  if constexpr (NW > 2) x += rowred::shr<dpp::kShr2>(0, x);
+  if constexpr (NW > 4) x += rowred::shr<dpp::kShr4>(0, x);
+  if constexpr (NW > 8) x += rowred::shr<dpp::kShr8>(0, x);
+  const int tot = __builtin_amdgcn_readlane(x, NW - 1);
+  const int off = w > 0 ? __builtin_amdgcn_readlane(x, w - 1) : 0;
   if constexpr (REUSE) __syncthreads();
   total = tot;
   return off + inc - c;
