@@ -704,19 +704,27 @@ std::vector<float> make_mixed_tables(uint32_t compute) {
       *lo = f32_to_bf16_bits((float)(v - (double)bf16_bits_to_f32(h)));
     }
   };
-  // DFT-16: the K = 32 operand rows are ordered (n1, re/im), K row q holds
-  // component q & 1 of input n1 = q / 2 (matrix column q / 2 or 16 + q / 2)
+  // DFT-16 of column block t (t < 8) with the twiddle factor W_128^(t k1) of
+  // its output rows folded in: complex entry W_128^(k1 (8 n1 + t)).  The K = 32
+  // operand rows are ordered (n1, re/im): K row q holds component q & 1 of
+  // input n1 = q / 2.
+  auto M16t = [](int t, int r, int q) -> double {
+    const int k1 = r % 16, n1 = q / 2;
+    const double a = 2.0 * M_PI * (double)((k1 * (8 * n1 + t)) % 128) / 128.0;
+    if (r < 16) return (q & 1) ? std::sin(a) : std::cos(a);
+    return (q & 1) ? std::cos(a) : -std::sin(a);
+  };
   uint16_t* W16 = reinterpret_cast<uint16_t*>(p + 1024 * 8 + 256 * 8);
-  for (int rb = 0; rb < 2; ++rb)
-    for (int l = 0; l < 64; ++l)
-      for (int j = 0; j < 8; ++j) {
-        const size_t i = ((size_t)rb * 64 + l) * 8 + j;
-        const int q = 8 * (l / 16) + j;
-        put(W16 + i, W16 + 2 * 64 * 8 + i, M(16, 16 * rb + l % 16, (q & 1) * 16 + q / 2));
-      }
+  for (int t = 0; t < 8; ++t)
+    for (int rb = 0; rb < 2; ++rb)
+      for (int l = 0; l < 64; ++l)
+        for (int j = 0; j < 8; ++j) {
+          const size_t i = (((size_t)t * 2 + rb) * 64 + l) * 8 + j;
+          put(W16 + i, W16 + 8 * 2 * 64 * 8 + i, M16t(t, 16 * rb + l % 16, 8 * (l / 16) + j));
+        }
   // DFT-32: K = 64 rows ordered (m, re/im) as well; k-step ks row q holds
   // component q & 1 of input m = 16 ks + q / 2
-  uint16_t* W32 = W16 + 2 * 2 * 64 * 8;
+  uint16_t* W32 = W16 + 2 * 8 * 2 * 64 * 8;
   for (int rb = 0; rb < 4; ++rb)
     for (int ks = 0; ks < 2; ++ks)
       for (int l = 0; l < 64; ++l)

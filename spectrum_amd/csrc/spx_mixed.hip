@@ -59,7 +59,7 @@ constexpr int TB_OFF = 0;
 constexpr int TAL_OFF = TB_OFF + 1024 * 8;
 constexpr int TAH_OFF = TAL_OFF + 128 * 8;
 constexpr int W16_OFF = TAH_OFF + 128 * 8;
-constexpr int W16_BYTES = 2 * 2 * 64 * 16;      // [hi|lo][rb 2][lane 64][8 x 16 bit]
+constexpr int W16_BYTES = 2 * 8 * 2 * 64 * 16;  // [hi|lo][block t 8][rb 2][lane 64][8 x 16 bit]
 constexpr int W32_OFF = W16_OFF + W16_BYTES;
 constexpr int W32_BYTES = 2 * 8 * 64 * 16;      // [hi|lo][rb 4 x ks 2][lane 64][8 x 16 bit]
 constexpr int TABLE_BYTES = W32_OFF + W32_BYTES;
@@ -312,11 +312,16 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
 
     // ---- stage 1: DFT-16 over n1, twiddle W_16384^(n2 k1), store half2 rows [k1][m2][m1]
     {
-      const WT w0h = gp(W16)[0 * 64 + l], w1h = gp(W16)[1 * 64 + l];
-      WT w0l = w0h, w1l = w1h;
-      if constexpr (Mma<CT>::SPLIT) {
-        w0l = gp(W16)[128 + 0 * 64 + l];
-        w1l = gp(W16)[128 + 1 * 64 + l];
+      // The twiddle W_16384^(n2 k1), n2 = 128 t + 16 w + c, is W_128^(t k1)
+      // (one factor per output row k1 of block t: folded into block t's
+      // DFT-16 matrix, W16^(n1 k1) W_128^(t k1) = W_128^(k1 (8 n1 + t)))
+      // times the lane's column factor pc[i] = W_16384^((16 w + c)(4 g + i)),
+      // the same for every block.
+      float2 pc[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int e = (16 * w + c) * (4 * g + i);  // < 2048
+        pc[i] = c_mul(tal[e & 127], tah[e >> 7]);
       }
       // window times pre-scale, w(n) s = c0 s - c1 s cos(th) + c2 s cos(2 th),
       // th = 2 pi n / (N - 1), as the polynomial P(cos th) = (c0 - c2) s +
@@ -352,14 +357,11 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
           cc = cn;
         }
         const f16x8 b = __builtin_bit_cast(f16x8, make_uint4(bw[0], bw[1], bw[2], bw[3]));
-        // the block's four twiddles W_16384^(n2 k1), k1 = 4g + i, read from
-        // the LDS tables before the MFMAs and before any store of the block
-        // (the stores could alias the tables for all the compiler knows)
-        float2 tw[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int e = n2 * (4 * g + i);  // < 16384
-          tw[i] = c_mul(tal[e & 127], tah[e >> 7]);
+        const WT w0h = gp(W16)[(2 * t + 0) * 64 + l], w1h = gp(W16)[(2 * t + 1) * 64 + l];
+        WT w0l = w0h, w1l = w1h;
+        if constexpr (Mma<CT>::SPLIT) {
+          w0l = gp(W16)[1024 + (2 * t + 0) * 64 + l];
+          w1l = gp(W16)[1024 + (2 * t + 1) * 64 + l];
         }
         const f32x4 z = {0.f, 0.f, 0.f, 0.f};
         const f32x4 dre = Mma<CT>::run(w0h, w0l, b, z);
@@ -369,7 +371,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
         uint32_t* pb = ping + (128 * g + m2) * PSTR + m1;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const float2 y = c_mul(make_float2(dre[i], dim[i]), tw[i]);
+          const float2 y = c_mul(make_float2(dre[i], dim[i]), pc[i]);
           pb[i * 32 * PSTR] = pack_h2(y.x, y.y);
         }
       }
