@@ -266,10 +266,28 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
     const float2* gt = reinterpret_cast<const float2*>(tabs);   // TB | TAL | TAH are contiguous
     for (int i = tid; i < 1024 + 256; i += T) tb[i] = gp(gt)[i];
   }
+  // per-wave largest |re|, |im| of the words in raw: |re| and |im| as 16-bit
+  // patterns (sign cleared) order like their values for fp16 and bf16
+  // alike, so one packed integer max per word.  Posted for the next channel
+  // ahead of the barrier that ends the current one, so the pre-scale needs no
+  // barrier of its own.
+  __shared__ float pmax[T / 64];
+  auto post_max = [&](const uint32_t (&rw)[8][4], int wv, int ln) {
+    u16x2 mxp = {0, 0};
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) mxp = __builtin_elementwise_max(mxp, __builtin_bit_cast(u16x2, rw[t][m] & 0x7fff7fffu));
+    const uint32_t mxb = (uint32_t)max(mxp[0], mxp[1]);
+    float mv = DT == SPX_DTYPE_C32F16 ? (float)__builtin_bit_cast(_Float16, (uint16_t)mxb) : __uint_as_float(mxb << 16);
+    mv = wave_maxf(mv);
+    if (ln == 0) pmax[wv] = mv;
+  };
   uint32_t ch = blockIdx.x;
   if (ch >= a.batch) return;
   uint32_t raw[8][4];
   load_raw(a, ch, w, g, c, raw);
+  post_max(raw, w, l);
   __syncthreads();
 
   for (; ch < a.batch; ch += gridDim.x) {
@@ -285,24 +303,13 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
     for (int t = 0; t < 8; ++t)
 #pragma unroll
       for (int m = 0; m < 4; ++m) raw[t][m] = (uint32_t)opaque_v((int)raw[t][m]);
-    // ---- per-channel power-of-two pre-scale from the largest component:
-    // |re| and |im| as 16-bit patterns (sign cleared) order like their values
-    // for fp16 and bf16 alike, so one packed integer max per word
+    // ---- per-channel power-of-two pre-scale from the largest component
+    // (the waves' maxima were posted before the barrier that ended the
+    // previous channel)
     const float isc = (a.in_scale != nullptr) ? gp(a.in_scale)[ch] : 1.0f;
-    u16x2 mxp = {0, 0};
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-        mxp = __builtin_elementwise_max(mxp, __builtin_bit_cast(u16x2, raw[t][m] & 0x7fff7fffu));
-    const uint32_t mxb = (uint32_t)max(mxp[0], mxp[1]);
-    float mx_ = DT == SPX_DTYPE_C32F16 ? (float)__builtin_bit_cast(_Float16, (uint16_t)mxb) : __uint_as_float(mxb << 16);
-    mx_ = wave_maxf(mx_);
-    if (l == 0) red.m[w] = mx_;
-    __syncthreads();
     float M = 0.f;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) M = fmaxf(M, red.m[i]);
+    for (int i = 0; i < T / 64; ++i) M = fmaxf(M, pmax[i]);
     M *= fabsf(isc);
     int e2 = 0;
     if (M > 0.f && M < __builtin_inff()) (void)frexpf(M, &e2);
@@ -476,6 +483,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
       dim3[t][0] = d[2];
       dim3[t][1] = d[3];
     }
+    if constexpr (MODE == 0) post_max(raw, w, l);  // the next channel's words (loaded after stage 1)
     __syncthreads();  // every stage-3 operand has been read
     if constexpr (MODE == 1) {
       {
@@ -493,6 +501,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
       }
       __syncthreads();
       mixed_peaks<LEARN>(a, ch, xre, xim, lb, red);
+      post_max(raw, w, l);  // the next channel's words (loaded after stage 3)
       __syncthreads();  // the planes are overwritten by the next channel
     }
   }
