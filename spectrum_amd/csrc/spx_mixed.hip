@@ -47,8 +47,11 @@ constexpr int N = 16384;
 constexpr int T = 512;             // 8 waves
 constexpr int PSTR = 36;           // stage-2 operand row stride in complex (144 B: conflict-free stage-1 stores)
 constexpr int PING_BYTES = 512 * PSTR * 4;     // [k1][m2] rows of 32 (+4) half2 {re, im}
-constexpr int PONG_BYTES = 512 * PSTR * 4;     // [j1][k1] rows of 32 (+4) half2
-constexpr int XPAD = N + N / 32;               // spectrum plane, bin k at k + k/32 (conflict-free per-thread runs)
+// [j1][k1] rows of 32 (+4) half2, row r shifted by 16 (r >> 6) words
+constexpr int PONG_BYTES = (512 * PSTR + 112) * 4;
+// spectrum plane, bin k at k + k/32 + 16 (k >> 11): conflict-free per-thread
+// runs for the sweep and conflict-free stage-3 stores
+constexpr int XPAD = N + N / 32 + 112;
 constexpr int TB_LDS = PING_BYTES + PONG_BYTES;   // 2^-5 W_1024^(m2 j1), [j1][m2] float2
 constexpr int TAL_LDS = TB_LDS + 1024 * 8;        // W_16384^e, e < 128
 constexpr int TAH_LDS = TAL_LDS + 128 * 8;        // W_128^e, e < 128
@@ -134,7 +137,8 @@ __device__ __forceinline__ void load_raw(const FusedArgs& a, uint32_t ch, int w,
     for (int m = 0; m < 4; ++m) raw[t][m] = ld_at<uint32_t>(base + (4096u * m + 512u * t), lo);
 }
 
-// Threshold peak search over the FP32 spectrum planes (bin k at k + k/32);
+// Threshold peak search over the FP32 spectrum planes (bin k at k + k/32 +
+// 16 (k >> 11));
 // thread t owns natural bins 32t .. 32t+31.  LEARN: the records get the
 // parabolic offset and the features of every stored peak go to a.mlp_feat,
 // for the learned refinement that runs after the kernel.
@@ -143,7 +147,7 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
                                             float lb, BlockRed& red) {
   constexpr int N = mx::N, T = mx::T, E = N / T;
   const int tid = threadIdx.x;
-  auto adr = [](uint32_t k) -> int { return (int)(k + (k >> 5)); };
+  auto adr = [](uint32_t k) -> int { return (int)(k + (k >> 5) + 16 * (k >> 11)); };
   auto xat = [&](uint32_t k) -> float2 { const int q = adr(k); return make_float2(xre[q], xim[q]); };
   const uint32_t k0 = (uint32_t)tid * E;
   const bool full = (a.s0 == 0 && a.s1 == (uint32_t)N);  // uniform
@@ -440,13 +444,15 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
         d[rb] = Mma<CT>::run(wh[rb * 2 + 1], wl[rb * 2 + 1], b1, d[rb]);
       }
       // row j1 = 16h + 4g + i of [j1][k1][m2]: base + (16h + i) * 16 * PSTR
-      uint32_t* qb = pong + (64 * g + k1) * PSTR + m2;
+      // row j1 * 16 + k1 shifted by 16 (j1 >> 2) = 64 h + 16 g words: the lane
+      // groups g land 16 banks apart (4-way conflicts without the shift)
+      uint32_t* qb = pong + (64 * g + k1) * PSTR + m2 + 16 * g;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const float2 y = c_mul(make_float2(d[h][i], d[2 + h][i]), twb[4 * h + i]);  // tb carries the 2^-5
-          qb[(16 * h + i) * 16 * PSTR] = pack_h2(y.x, y.y);
+          qb[(16 * h + i) * 16 * PSTR + 64 * h] = pack_h2(y.x, y.y);
         }
     }
     __syncthreads();
@@ -459,7 +465,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
     for (int t = 0; t < 4; ++t) {
       const int j1 = w + 8 * t;
       f16x8 b0, b1;
-      load_ops(pong + (j1 * 16 + c) * PSTR, g, b0, b1);
+      load_ops(pong + (j1 * 16 + c) * PSTR + 16 * (j1 >> 2), g, b0, b1);
       f32x4 d[4];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
@@ -476,8 +482,9 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
             gp(a.spec_out + (uint64_t)ch * N)[c + 16 * j1 + 512 * j2] =
                 make_float2(d[h][i] * s_out, d[2 + h][i] * s_out);
           } else {
-            // k = c + 16 j1 + 512 j2 sits at k + k/32 = c + 16 j1 + j1/2 + 528 j2
-            xre[c + 16 * j1 + (j1 >> 1) + 528 * 4 * g + 528 * (16 * h + i)] = d[h][i] * s_out;
+            // k = c + 16 j1 + 512 j2 sits at k + k/32 + 16 (k >> 11)
+            //   = c + 16 j1 + j1/2 + 528 j2 + 64 h + 16 g  (j2 = 16 h + 4 g + i)
+            xre[c + 16 * j1 + (j1 >> 1) + (528 * 4 + 16) * g + 528 * (16 * h + i) + 64 * h] = d[h][i] * s_out;
           }
         }
       dim3[t][0] = d[2];
@@ -497,7 +504,7 @@ __global__ void __launch_bounds__(512) k_mixed16k(FusedArgs a) {
         for (int h = 0; h < 2; ++h)
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            xim[c + 16 * j1 + (j1 >> 1) + 528 * 4 * g + 528 * (16 * h + i)] = dim3[t][h][i] * s_out;
+            xim[c + 16 * j1 + (j1 >> 1) + (528 * 4 + 16) * g + 528 * (16 * h + i) + 64 * h] = dim3[t][h][i] * s_out;
       }
       __syncthreads();
       mixed_peaks<LEARN>(a, ch, xre, xim, lb, red);
