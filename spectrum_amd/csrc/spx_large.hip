@@ -12,14 +12,18 @@
 // with hipFFT), not at the rate of the fused lengths.
 //
 // The peak searches (spx_fused_peaks at these lengths) run on the stored
-// spectrum with the same rules and records as the fused kernels:
-//   k_ls_stats   many blocks per channel: sums of |X| and |X|^2 over the
-//                search range and the largest |X| (lowest bin on ties)
-//   k_ls_select  one block per channel: statistics, threshold, global-maximum
-//                record, then every bin in ascending order: strict local
-//                maxima (circular neighbours) above the threshold are
-//                counted and the first K written in order (block scan of
-//                per-thread counts, a running total across tiles)
+// spectrum with the same rules and records as the fused kernels, in four
+// launches that each spread a channel over many workgroups except the two
+// per-channel steps that touch a few values:
+//   k_ls_stats   blocks of 16384 bins: sums of |X| and |X|^2 over the search
+//                range and the largest |X| (lowest bin on ties)
+//   k_ls_thresh  one block per channel: statistics, threshold, the
+//                global-maximum record and the stats output
+//   k_ls_cand    blocks of 16384 bins: strict local maxima (circular
+//                neighbours) above the threshold, their bins in ascending
+//                order per block (block scans of per-thread counts)
+//   k_ls_gather  one block per channel: offsets of the blocks' lists, then
+//                the first K bins overall as records, in ascending order
 #include "spx_fused_common.hpp"
 
 namespace spx {
@@ -139,14 +143,19 @@ __global__ void __launch_bounds__(LS_T) k_ls_stats(FusedArgs a, const float2* sp
 // |X| of bin k of a channel's stored spectrum (k taken modulo N)
 __device__ __forceinline__ float ls_mag(const float2* X, uint32_t k, uint32_t N) { return mag_of(gp(X)[k & (N - 1)]); }
 
-__global__ void __launch_bounds__(LS_T) k_ls_select(FusedArgs a, const float2* spec, const LsPart* part,
-                                                    uint32_t nblk) {
+constexpr int LS_CAP = LS_BLK / 2;  // candidate slots per block: strict maxima are never adjacent
+
+struct LsChan {
+  float thr;
+};
+
+__global__ void __launch_bounds__(LS_T) k_ls_thresh(FusedArgs a, const float2* spec, const LsPart* part,
+                                                    uint32_t nblk, LsChan* chs) {
   __shared__ BlockRed red;
   const uint64_t c = blockIdx.x;  // channel of the chunk (the host offsets the output pointers)
   const uint32_t N = a.nfft;
   const float2* X = spec + c * (uint64_t)N;
   const int tid = threadIdx.x;
-  // statistics and the global maximum from the block partials
   double s1 = 0.0, s2 = 0.0;
   float bm = -1.f;
   uint32_t bk = 0xffffffffu;
@@ -159,55 +168,103 @@ __global__ void __launch_bounds__(LS_T) k_ls_select(FusedArgs a, const float2* s
       bk = p.k;
     }
   }
-  block_stats<LS_T, true>(s1, s2, bm, bk, red);
+  block_stats<LS_T, false>(s1, s2, bm, bk, red);
   const double cnt = (double)(a.s1 - a.s0);
   float mu, sd;
   const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
-  // global-maximum record
-  if (a.gmax && tid == 0 && bk != 0xffffffffu)
-    write_record(a.gmax + c, bk, gp(X)[bk], ls_mag(X, bk + N - 1, N), bm, ls_mag(X, bk + 1, N), a);
-  // ascending sweep: 16 consecutive bins per thread per tile
-  constexpr int E = 16;
-  uint32_t total = 0;
-  const uint32_t kcap = a.kcap;
-  spx_max_value* out = a.peaks + c * (uint64_t)kcap;
-  for (uint32_t base = 0; base < N; base += LS_T * E) {
-    const uint32_t k0 = base + (uint32_t)tid * E;
-    float m[E + 2];
-#pragma unroll
-    for (int e = 0; e < E + 2; ++e) m[e] = ls_mag(X, k0 + N - 1 + (uint32_t)e, N);
-    uint32_t mask = 0;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const uint32_t k = k0 + e;
-      const bool ok = k < N && in_range(k, a) && m[e + 1] > m[e] && m[e + 1] > m[e + 2] && m[e + 1] > thr;
-      mask |= (ok ? 1u : 0u) << e;
-    }
-    int tot;
-    const int off = block_excl_scan<LS_T>(__popc(mask), tot, red);
-    uint32_t rank = total + (uint32_t)off;
-    while (mask) {
-      const int e = __builtin_ctz(mask);
-      mask &= mask - 1;
-      if (rank < kcap) {
-        const uint32_t k = k0 + e;
-        write_record(out + rank, k, gp(X)[k], m[e], m[e + 1], m[e + 2], a);
-      }
-      ++rank;
-    }
-    total += (uint32_t)tot;
-  }
   if (tid == 0) {
-    if (a.counts) {
-      gp(a.counts)[2 * c + 0] = min(total, kcap);
-      gp(a.counts)[2 * c + 1] = total;
-    }
+    gp(chs)[c].thr = thr;
+    if (a.gmax && bk != 0xffffffffu)
+      write_record(a.gmax + c, bk, gp(X)[bk], ls_mag(X, bk + N - 1, N), bm, ls_mag(X, bk + 1, N), a);
     if (a.stats) {
       gp(a.stats)[4 * c + 0] = mu;
       gp(a.stats)[4 * c + 1] = sd;
       gp(a.stats)[4 * c + 2] = thr;
       gp(a.stats)[4 * c + 3] = (float)(cnt > 0 ? s2 / cnt : 0.0);
     }
+  }
+}
+
+// cnt[c * nblk + b]: the number of bins listed by block b; idx[(c * nblk +
+// b) * LS_CAP + i]: its i-th bin, ascending
+__global__ void __launch_bounds__(LS_T) k_ls_cand(FusedArgs a, const float2* spec, const LsChan* chs,
+                                                  uint32_t nblk, uint32_t* cnt, uint32_t* idx) {
+  __shared__ BlockRed red;
+  const uint64_t c = blockIdx.y;
+  const uint32_t N = a.nfft;
+  const float2* X = spec + c * (uint64_t)N;
+  const int tid = threadIdx.x;
+  const float thr = gp(chs)[c].thr;
+  const uint64_t slot = c * nblk + blockIdx.x;
+  uint32_t* out = idx + slot * LS_CAP;
+  constexpr int E = 16;
+  uint32_t total = 0;
+  for (uint32_t tile = 0; tile < (uint32_t)LS_BLK; tile += LS_T * E) {
+    const uint32_t k0 = blockIdx.x * LS_BLK + tile + (uint32_t)tid * E;  // N is a multiple of LS_BLK
+    float m[E + 2];
+#pragma unroll
+    for (int e = 0; e < E + 2; ++e) m[e] = ls_mag(X, k0 + N - 1 + (uint32_t)e, N);
+    uint32_t mask = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const bool ok = in_range(k0 + e, a) && m[e + 1] > m[e] && m[e + 1] > m[e + 2] && m[e + 1] > thr;
+      mask |= (ok ? 1u : 0u) << e;
+    }
+    int tot;
+    uint32_t rank = total + (uint32_t)block_excl_scan<LS_T>(__popc(mask), tot, red);
+    while (mask) {
+      const int e = __builtin_ctz(mask);
+      mask &= mask - 1;
+      gp(out)[rank++] = k0 + e;
+    }
+    total += (uint32_t)tot;
+  }
+  if (tid == 0) gp(cnt)[slot] = total;
+}
+
+constexpr int LS_MAXBLK = (1 << 24) / LS_BLK;
+
+__global__ void __launch_bounds__(LS_T) k_ls_gather(FusedArgs a, const float2* spec, const uint32_t* cnt,
+                                                    const uint32_t* idx, uint32_t nblk) {
+  __shared__ BlockRed red;
+  __shared__ uint32_t off[LS_MAXBLK];
+  const uint64_t c = blockIdx.x;
+  const uint32_t N = a.nfft;
+  const float2* X = spec + c * (uint64_t)N;
+  const int tid = threadIdx.x;
+  // exclusive prefix of the block counts: thread t owns blocks [t*PER, (t+1)*PER)
+  constexpr int PER = LS_MAXBLK / LS_T;
+  uint32_t own[PER], run = 0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const uint32_t b = (uint32_t)tid * PER + i;
+    own[i] = b < nblk ? gp(cnt)[c * nblk + b] : 0u;
+    run += own[i];
+  }
+  int tot;
+  uint32_t base = (uint32_t)block_excl_scan<LS_T>((int)run, tot, red);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const uint32_t b = (uint32_t)tid * PER + i;
+    if (b < nblk) off[b] = base;
+    base += own[i];
+  }
+  __syncthreads();
+  const uint32_t total = (uint32_t)tot, kcap = a.kcap;
+  const uint32_t nout = min(total, kcap);
+  spx_max_value* out = a.peaks + c * (uint64_t)kcap;
+  for (uint32_t s = (uint32_t)tid; s < nout; s += LS_T) {
+    // the listing block: the last b with off[b] <= s (blocks without bins
+    // share their successor's offset and are passed over)
+    uint32_t b = 0;
+    for (uint32_t step = LS_MAXBLK / 2; step > 0; step >>= 1)
+      if (b + step < nblk && off[b + step] <= s) b += step;
+    const uint32_t k = gp(idx)[(c * nblk + b) * LS_CAP + (s - off[b])];
+    write_record(out + s, k, gp(X)[k], ls_mag(X, k + N - 1, N), ls_mag(X, k, N), ls_mag(X, k + 1, N), a);
+  }
+  if (tid == 0 && a.counts) {
+    gp(a.counts)[2 * c + 0] = nout;
+    gp(a.counts)[2 * c + 1] = total;
   }
 }
 
@@ -222,7 +279,8 @@ int spx_large_is(uint32_t nfft) { return nfft >= (1u << 19) && nfft <= (1u << 24
 uint64_t spx_large_workspace_bytes(uint32_t nfft, uint32_t chunk) {
   using namespace spx;
   const uint64_t nblk = (nfft + LS_BLK - 1) / LS_BLK;
-  return 2ull * chunk * nfft * 8 + (uint64_t)chunk * nblk * sizeof(LsPart) + 512;
+  return 2ull * chunk * nfft * 8 + (uint64_t)chunk * nblk * sizeof(LsPart) + (uint64_t)chunk * sizeof(LsChan) +
+         (uint64_t)chunk * nblk * 4 * (1 + LS_CAP) + 1024;
 }
 
 // Transform (and, for a->peak_mode != 0, the peak search) of a->batch
@@ -239,7 +297,14 @@ hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, void* ws, uin
   const int rx = (lg % 4) ? (1 << (lg % 4)) : 16;
   const uint64_t nblk = (N + LS_BLK - 1) / LS_BLK;
   float2* buf[2] = {reinterpret_cast<float2*>(ws), reinterpret_cast<float2*>(ws) + (uint64_t)chunk * N};
-  LsPart* part = reinterpret_cast<LsPart*>(reinterpret_cast<unsigned char*>(ws) + 2ull * chunk * N * 8);
+  unsigned char* tail = reinterpret_cast<unsigned char*>(ws) + 2ull * chunk * N * 8;
+  LsPart* part = reinterpret_cast<LsPart*>(tail);
+  tail += (uint64_t)chunk * nblk * sizeof(LsPart);
+  LsChan* chs = reinterpret_cast<LsChan*>(tail);
+  tail += ((uint64_t)chunk * sizeof(LsChan) + 255) & ~255ull;
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(tail);
+  tail += ((uint64_t)chunk * nblk * 4 + 255) & ~255ull;
+  uint32_t* idx = reinterpret_cast<uint32_t*>(tail);
   const bool peaks = a0->peak_mode != 0;
   for (uint32_t c0 = 0; c0 < a0->batch; c0 += chunk) {
     const uint32_t nch = min(chunk, a0->batch - c0);
@@ -280,7 +345,10 @@ hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, void* ws, uin
       b.gmax = a0->gmax ? a0->gmax + c0 : nullptr;
       b.stats = a0->stats ? a0->stats + 4ull * c0 : nullptr;
       hipLaunchKernelGGL(k_ls_stats, dim3((uint32_t)nblk, nch), dim3(LS_T), 0, st, b, spec, part, (uint32_t)nblk);
-      hipLaunchKernelGGL(k_ls_select, dim3(nch), dim3(LS_T), 0, st, b, spec, part, (uint32_t)nblk);
+      hipLaunchKernelGGL(k_ls_thresh, dim3(nch), dim3(LS_T), 0, st, b, spec, part, (uint32_t)nblk, chs);
+      hipLaunchKernelGGL(k_ls_cand, dim3((uint32_t)nblk, nch), dim3(LS_T), 0, st, b, spec, chs, (uint32_t)nblk, cnt,
+                         idx);
+      hipLaunchKernelGGL(k_ls_gather, dim3(nch), dim3(LS_T), 0, st, b, spec, cnt, idx, (uint32_t)nblk);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
