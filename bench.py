@@ -218,6 +218,8 @@ def extra_host(ctx, x, K=64, reps=5):
 
 
 def extra_uneven(ctx, K=64):
+    import tempfile
+
     from spectrum_amd import capi
 
     n = 4096
@@ -226,27 +228,40 @@ def extra_uneven(ctx, K=64):
     x = capi.siggen_uneven(capi.GEN_MULTI_TONE, 0x5EED0003, L, off, total)
     d = ctx.to_device(x)
     pk, ct = ctx.alloc(n * K * 32), ctx.alloc(n * 8)
-    plan = ctx.batch_plan(ctx.batch_desc(d, off, L, pk, ct, max_peaks=K))
+    # the autotuner times every candidate of every length on this batch and
+    # writes its cache; the measured plan reads it (the untuned plan beside it)
+    cache = os.path.join(tempfile.mkdtemp(prefix="spx_tune_"), "uneven.json")
+    t0 = time.perf_counter()
+    ctx.batch_autotune(ctx.batch_desc(d, off, L, pk, ct, max_peaks=K), cache)
+    tune_s = time.perf_counter() - t0
+    tuned = json.load(open(cache))
+    plan = ctx.batch_plan(ctx.batch_desc(d, off, L, pk, ct, max_peaks=K, tune_cache=cache))
+    plan0 = ctx.batch_plan(ctx.batch_desc(d, off, L, pk, ct, max_peaks=K))
     bufs = {Lx: (ctx.alloc(len(ids) * K * 32), ctx.alloc(len(ids) * 8)) for Lx, (o, ids) in groups.items()}
 
     def seq():
         for Lx, (o, ids) in sorted(groups.items()):
             ctx.fused_peaks(d.ptr + 8 * o, len(ids), Lx, Lx, bufs[Lx][0], bufs[Lx][1], max_peaks=K)
 
-    # the two forms alternate over three rounds (the first timed round of a
-    # run reads slow), medians reported
-    tb, ts = [], []
+    # the forms alternate over three rounds (the first timed round of a run
+    # reads slow), medians reported
+    tb, t0s, ts = [], [], []
     for _ in range(3):
         tb.append(_timer_ms(ctx, plan.execute))
+        t0s.append(_timer_ms(ctx, plan0.execute))
         ts.append(_timer_ms(ctx, seq))
-    t_batch, t_seq = float(np.median(tb)), float(np.median(ts))
+    t_batch, t_untuned, t_seq = float(np.median(tb)), float(np.median(t0s)), float(np.median(ts))
     info = plan.info()
     plan.close()
+    plan0.close()
     alg = total * 8 + n * (K * 32 + 8)
     return {"workload": f"{n} channels, lengths 2^10..2^17 log-uniform (complex64, {total * 8 / 1e9:.2f} GB), "
-                        f"Hann, mean+6std, K={K}", "batch_graph_ms": t_batch, "per_length_launches_ms": t_seq,
-            "speedup": t_seq / t_batch, "channels_per_s": n / (t_batch * 1e-3),
-            "achieved_GBps": alg / (t_batch * 1e-3) / 1e9, "plan": info}
+                        f"Hann, mean+6std, K={K}", "batch_graph_ms": t_batch, "untuned_graph_ms": t_untuned,
+            "per_length_launches_ms": t_seq, "speedup": t_seq / t_batch, "channels_per_s": n / (t_batch * 1e-3),
+            "achieved_GBps": alg / (t_batch * 1e-3) / 1e9, "autotune_s": tune_s,
+            "tuned": [{k: e[k] for k in ("length", "order", "tile", "groups", "us")} for e in tuned["lengths"]],
+            "tile_sets_us": [{"us": t["us"], "tiles": t["tiles"]} for t in tuned.get("tile_sets", [])],
+            "plan": info}
 
 
 def _rmse_vs_tones(recs, cnt, tf, ta, N):

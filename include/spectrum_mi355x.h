@@ -269,13 +269,18 @@ int spx_fused_peaks_mixed(spx_ctx* ctx, const spx_fused_desc* desc, uint32_t com
  * (src/fft_func/src/spectrum_processor_rocm.cpp:1129-1150).  Channels of
  * lengths 1024 .. 131072 (powers of two, any mix, any order) are searched with
  * the SPX_PEAKS_THRESHOLD rules of spx_fused_peaks (whole band, nfft = length):
- * lengths up to 16384 in one launch of a persistent kernel over a cost-sorted
- * work list, longer lengths by the split four-step launches of spx_fused_peaks
- * reading each channel at its own offset; the plan captures all of it in one
- * hipGraph.  The radix order per length up to 16384 comes from an autotuner
- * cache (JSON, written by spx_batch_autotune); without one every length uses
- * the default order.  A plan binds its buffers: execute re-runs the captured
- * graph on whatever the input buffer holds at that time. */
+ * lengths up to 16384 in persistent launches over cost-sorted work lists of
+ * LDS tiles (one launch per tile shape in use: 16384, 8192 or 4096 points per
+ * workgroup, i.e. 1024, 512 or 256 threads and 16384/tile workgroups per CU,
+ * each tile holding tile/length whole channels), longer lengths by the split
+ * four-step launches of spx_fused_peaks reading each channel at its own
+ * offset, in one or more launch groups; the plan captures all of it in one
+ * hipGraph.  The radix order and tile shape of every length up to 16384 and
+ * the launch groups of every longer length come from an autotuner cache
+ * (JSON, written by spx_batch_autotune); without one every length uses the
+ * default order, the 16384-point tile and one launch group.  A plan binds its
+ * buffers: execute re-runs the captured graph on whatever the input buffer
+ * holds at that time. */
 typedef struct spx_batch_plan spx_batch_plan;
 
 #define SPX_BATCH_NO_GRAPH 1u   /* launch the kernels directly on every execute */
@@ -304,12 +309,14 @@ typedef struct spx_batch_desc {
 int spx_batch_plan_create(spx_ctx* ctx, const spx_batch_desc* desc, spx_batch_plan** out);
 int spx_batch_execute(spx_batch_plan* plan);
 int spx_batch_plan_destroy(spx_batch_plan* plan);
-/* JSON description of the plan (items per length, radix order, grid, scratch). */
+/* JSON description of the plan (per tile launch: shape, grid, items; per
+ * length: radix order, tile shape or launch groups, items, scratch). */
 int spx_batch_plan_info(spx_batch_plan* plan, char* buf, size_t len);
-/* Times every radix order of every length present in desc (on desc's own
- * buffers, each length alone), keeps the fastest and writes the choices with
- * the measured times to cache_path as JSON.  Lengths whose transforms have a
- * single radix order are recorded with order 0. */
+/* Times, for every length present in desc (on desc's own buffers, each length
+ * alone), every candidate: up to 16384 each radix order (where the length has
+ * two) in each tile shape that holds the length, above 16384 one, two or four
+ * launch groups; keeps the fastest and writes the choices with every measured
+ * time to cache_path as JSON. */
 int spx_batch_autotune(spx_ctx* ctx, const spx_batch_desc* desc, const char* cache_path);
 
 /* ---- synthetic signals (host side, deterministic) --------------------------

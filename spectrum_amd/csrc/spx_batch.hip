@@ -1,18 +1,20 @@
 // Persistent uneven-length batch: every channel of a batch whose length is
 // 2^10 .. 2^14 runs through the fused window -> FFT -> |X| -> threshold peak
-// search in ONE launch (the host plan captures it in a hipGraph together with
-// the split-path launches of the longer lengths).
+// search in persistent launches (the host plan captures them in a hipGraph
+// together with the split-path launches of the longer lengths).
 //
-// The host builds a work list sorted by decreasing cost (tiles of the longest
-// length first: a longest-processing-time-first schedule) and the grid is one
-// 1024-thread workgroup per CU.  Each workgroup takes the next item from a
-// global cursor (one relaxed atomic per item, no other cross-workgroup
-// traffic); an item is a tile of G = 16384/L whole channels in LDS: the same
-// Stockham passes as the single-channel kernel but over G sequences, then
-// per-channel (segmented) statistics, peak flags and scan, so every channel's
-// records equal the single-channel kernel's.  The radix order of each length
-// (Plan<L, V>) is chosen per bucket by the host autotuner and carried in the
-// item word.
+// The host builds a work list per tile shape, sorted by decreasing cost (tiles
+// of the longest length first: a longest-processing-time-first schedule).  A
+// tile shape is TILE complex points in LDS held by TILE/16 threads: 16384
+// (1024 threads, one workgroup per CU), 8192 (512 threads, two per CU) or
+// 4096 (256 threads, four per CU); the grid of a shape fills every CU.  Each
+// workgroup takes the next item from its launch's cursor (one relaxed atomic
+// per item, no other cross-workgroup traffic); an item is a tile of
+// G = TILE/L whole channels: the same Stockham passes as the single-channel
+// kernel but over G sequences, then per-channel (segmented) statistics, peak
+// flags and scan, so every channel's records equal the single-channel
+// kernel's.  The tile shape and radix order (Plan<L, V>) of each length are
+// chosen by the host autotuner; the order is carried in the item word.
 #include "spx_fused_common.hpp"
 
 namespace spx {
@@ -47,11 +49,11 @@ __device__ __forceinline__ const void* chan_base(const BatchArgs& b, uint32_t ch
   return reinterpret_cast<const char*>(b.in) + gp(b.offsets)[ch] * es;
 }
 
-// G = 16384/L channels of length L, 1024 threads, 16 points per thread.
-template <int L, int V>
+// G = TILE/L channels of length L, TILE/16 threads, 16 points per thread.
+template <int L, int V, int TILE>
 __device__ __forceinline__ void batch_small_tile(const BatchArgs& b, const uint4 it, float2* lds, BlockRed& red) {
   using PL = Plan<L, V>;
-  constexpr int T = 1024, PPT = 16, G = 16384 / L;
+  constexpr int T = TILE / 16, PPT = 16, G = TILE / L;
   constexpr int TPC = L / 16;      // threads per channel in the peak stage (16 bins each)
   constexpr int WPC = TPC / 64;    // waves per channel
   constexpr int R0 = PL::R0, RL = PL::RL;
@@ -200,17 +202,30 @@ __global__ void k_batch_reset(uint32_t* counter) {
   if (threadIdx.x == 0) gp(counter)[0] = 0u;
 }
 
-// LDS of a tile: every length's LayContig<L, 16384/L> has the same size
+// LDS of a tile: every length's LayContig<L, TILE/L> has the same size
+template <int TILE>
 struct BatchShared {
-  float2 tile[LayContig<1024, 16>::SIZE];
+  float2 tile[LayContig<TILE, 1>::SIZE];
   BlockRed red;
 };
 static_assert(LayContig<16384, 1>::SIZE == LayContig<1024, 16>::SIZE, "one tile size for every length");
 
-__global__ void __launch_bounds__(1024) k_batch(BatchArgs b) {
+// one tile item of length L (item word: bucket | variant << 8)
+template <int TILE, int L>
+__device__ __forceinline__ void batch_item_len(const BatchArgs& b, const uint4 it, BatchShared<TILE>& sh) {
+  if constexpr (L <= TILE) {
+    if ((it.x >> 8) & 1u) batch_small_tile<L, 1, TILE>(b, it, sh.tile, sh.red);
+    else batch_small_tile<L, 0, TILE>(b, it, sh.tile, sh.red);
+  }
+}
+
+// Every TILE/16-thread shape keeps four waves per SIMD (128 VGPRs): one
+// 1024-thread, two 512-thread or four 256-thread workgroups per CU.
+template <int TILE>
+__global__ void __launch_bounds__(TILE / 16) __attribute__((amdgpu_waves_per_eu(4))) k_batch(BatchArgs b) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ uint32_t item_s;
-  BatchShared& sh = *reinterpret_cast<BatchShared*>(smem_raw);
+  BatchShared<TILE>& sh = *reinterpret_cast<BatchShared<TILE>*>(smem_raw);
   for (;;) {
     if (threadIdx.x == 0) item_s = atomicAdd(b.counter, 1u);
     __syncthreads();
@@ -221,17 +236,15 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs b) {
     it.x = __builtin_amdgcn_readfirstlane(it.x);
     it.y = __builtin_amdgcn_readfirstlane(it.y);
     it.z = __builtin_amdgcn_readfirstlane(it.z);
-    switch (it.x & 0xffffu) {
-      case 0x000: batch_small_tile<1024, 0>(b, it, sh.tile, sh.red); break;
-      case 0x100: batch_small_tile<1024, 1>(b, it, sh.tile, sh.red); break;
-      case 0x001: batch_small_tile<2048, 0>(b, it, sh.tile, sh.red); break;
-      case 0x101: batch_small_tile<2048, 1>(b, it, sh.tile, sh.red); break;
-      case 0x002: batch_small_tile<4096, 0>(b, it, sh.tile, sh.red); break;
-      case 0x003: batch_small_tile<8192, 0>(b, it, sh.tile, sh.red); break;
-      case 0x103: batch_small_tile<8192, 1>(b, it, sh.tile, sh.red); break;
-      case 0x004: batch_small_tile<16384, 0>(b, it, sh.tile, sh.red); break;
-      case 0x104: batch_small_tile<16384, 1>(b, it, sh.tile, sh.red); break;
-      default: break;  // the host validates every item word
+    // only orders the plan has tables for are ever put in an item (the host
+    // validates every item word: variant 1 only where the length has one)
+    switch (it.x & 0xffu) {
+      case 0: batch_item_len<TILE, 1024>(b, it, sh); break;
+      case 1: batch_item_len<TILE, 2048>(b, it, sh); break;
+      case 2: batch_item_len<TILE, 4096>(b, it, sh); break;
+      case 3: batch_item_len<TILE, 8192>(b, it, sh); break;
+      case 4: batch_item_len<TILE, 16384>(b, it, sh); break;
+      default: break;
     }
   }
 }
@@ -240,22 +253,39 @@ __global__ void __launch_bounds__(1024) k_batch(BatchArgs b) {
 
 extern "C" {
 
-// Dynamic LDS of the batch kernel (one tile).
-static size_t batch_smem() { return sizeof(spx::BatchShared); }
-
-// Allows the batch kernel its dynamic LDS; call once per process before the
-// first launch and outside any stream capture.
-hipError_t spx_batch_kernel_init(void) {
+// Dynamic LDS of the batch kernel of a tile shape (one tile).
+static size_t batch_smem(int tile) {
   using namespace spx;
-  return hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)batch_smem());
+  return tile == 16384 ? sizeof(BatchShared<16384>) : tile == 8192 ? sizeof(BatchShared<8192>) : sizeof(BatchShared<4096>);
 }
 
-hipError_t spx_launch_batch(const BatchArgs* b, int grid, hipStream_t st) {
+// Allows the batch kernels their dynamic LDS; call once per process before
+// the first launch and outside any stream capture.
+hipError_t spx_batch_kernel_init(void) {
   using namespace spx;
+  hipError_t e = hipFuncSetAttribute((const void*)k_batch<16384>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)batch_smem(16384));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_batch<8192>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)batch_smem(8192));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_batch<4096>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)batch_smem(4096));
+  return e;
+}
+
+int spx_batch_tile_ok(int tile) { return tile == 16384 || tile == 8192 || tile == 4096; }
+
+// b->counter must point at a word of its own per launch (reset here).
+hipError_t spx_launch_batch(const BatchArgs* b, int tile, int grid, hipStream_t st) {
+  using namespace spx;
+  if (!spx_batch_tile_ok(tile) || grid <= 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->counter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(1024), batch_smem(), st, *b);
+  if (tile == 16384) hipLaunchKernelGGL(k_batch<16384>, dim3(grid), dim3(1024), batch_smem(tile), st, *b);
+  else if (tile == 8192) hipLaunchKernelGGL(k_batch<8192>, dim3(grid), dim3(512), batch_smem(tile), st, *b);
+  else hipLaunchKernelGGL(k_batch<4096>, dim3(grid), dim3(256), batch_smem(tile), st, *b);
   return hipGetLastError();
 }
 

@@ -1,8 +1,10 @@
 // Host side of the uneven-length batch: work-list planning for the persistent
-// LDS-tile launch (lengths 2^10 .. 2^14), the split-path launch sequences of
-// the longer lengths (2^15 .. 2^17, one per length, channel indirection
-// through the plan's permutation and offsets), the radix-order autotuner and
-// its JSON cache, and hipGraph capture of the whole sequence.
+// LDS-tile launches (lengths 2^10 .. 2^14, one launch per tile shape in use),
+// the split-path launch sequences of the longer lengths (2^15 .. 2^17, one per
+// length, in launch groups, channel indirection through the plan's
+// permutation and offsets), the autotuner (radix order and tile shape per
+// LDS-tile length, launch groups per split length) and its JSON cache, and
+// hipGraph capture of the whole sequence.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,23 +27,44 @@ struct SplitGroup {
   uint32_t chunk;    // channels per launch sequence
 };
 
-struct spx_batch_plan {
-  spx_ctx* c = nullptr;
+// Tile shapes of the persistent LDS-tile launches: TILE complex points in LDS,
+// TILE/16 threads per workgroup, 16384/TILE workgroups per CU.
+constexpr int kTiles[3] = {16384, 8192, 4096};
+
+// one persistent launch: every item of one tile shape
+struct TileLaunch {
+  int tile = 0;
   BatchArgs args{};
   int grid = 0;
+  uint32_t n_items = 0;
+};
+
+// What the autotuner chooses per length: the radix order and tile shape of an
+// LDS-tile length, the number of launch groups a split length's channels are
+// divided into (each group one column / row / finish sequence).
+struct Choice {
+  int order = 0;
+  int tile = 16384;
+  int groups = 1;
+};
+
+struct spx_batch_plan {
+  spx_ctx* c = nullptr;
+  std::vector<TileLaunch> tl;  // shapes in use, largest tile first
   void* d_items = nullptr;
   void* d_perm = nullptr;
   void* d_offsets = nullptr;
-  void* d_counter = nullptr;
+  void* d_counter = nullptr;   // one 64-byte slot per tile launch
   void* d_scratch = nullptr;
   uint64_t scratch_bytes = 0;
   std::vector<SplitGroup> split;
   bool graph = false;
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
-  int variant[SPX_BATCH_BUCKETS] = {0};
+  Choice ch[SPX_BATCH_BUCKETS];
   uint32_t items_per_bucket[SPX_BATCH_BUCKETS] = {0};
   uint32_t chans_per_bucket[SPX_BATCH_BUCKETS] = {0};
+  uint32_t chunk_per_bucket[SPX_BATCH_BUCKETS] = {0};
   uint32_t n_items = 0;
 };
 
@@ -53,19 +76,47 @@ int log2u(uint32_t n) {
   return l;
 }
 
+uint32_t bucket_len(int bucket) { return 1u << (SPX_BATCH_LOG_MIN + bucket); }
+bool is_split(int bucket) { return bucket >= SPX_BATCH_FIRST_LARGE; }
+
 // Length of the transform whose radix order the variant selects: the whole
 // channel for LDS tiles, the column transform (L/256) for four-step channels.
 uint32_t tuned_len(int bucket) {
-  const uint32_t L = 1u << (SPX_BATCH_LOG_MIN + bucket);
-  return bucket >= SPX_BATCH_FIRST_LARGE ? L / 256 : L;
+  const uint32_t L = bucket_len(bucket);
+  return is_split(bucket) ? L / 256 : L;
 }
 
 // the split path of the long lengths has one radix order (radix 16 first)
-bool has_variants(int bucket) { return bucket < SPX_BATCH_FIRST_LARGE && (log2u(tuned_len(bucket)) % 4) != 0; }
+bool has_variants(int bucket) { return !is_split(bucket) && (log2u(tuned_len(bucket)) % 4) != 0; }
+
+bool tile_fits(int bucket, int tile) { return spx_batch_tile_ok(tile) && (uint32_t)tile >= bucket_len(bucket); }
+
+// A choice as the bucket can take it (anything else falls back to the defaults).
+Choice sanitize(int bucket, Choice x) {
+  Choice r;
+  if (is_split(bucket)) {
+    r.groups = (x.groups >= 1 && x.groups <= 64) ? x.groups : 1;
+  } else {
+    r.order = (has_variants(bucket) && x.order == 1) ? 1 : 0;
+    r.tile = tile_fits(bucket, x.tile) ? x.tile : 16384;
+  }
+  return r;
+}
+
+// integer value of "key": in s[from, to), or -1
+int json_int(const std::string& s, size_t from, size_t to, const char* key) {
+  const std::string k = std::string("\"") + key + "\":";
+  const size_t p = s.find(k, from);
+  if (p == std::string::npos || p >= to) return -1;
+  int v = -1;
+  return std::sscanf(s.c_str() + p + k.size(), " %d", &v) == 1 ? v : -1;
+}
 
 // Cache format (written by spx_batch_autotune):
-// {"device": "...", "cu": N, "lengths": [{"length": L, "order": V, "us": [t0, t1], "channels": n}, ...]}
-void read_cache(const char* path, int* variant) {
+// {"device": "...", "cu": N, "lengths": [{"length": L, "order": V, "tile": T, "groups": g, "us": best,
+//   "candidates": [...]}, ...]}; the fields of an entry come before its candidate list, and
+// "tile" / "groups" may be absent (defaults).
+void read_cache(const char* path, Choice* ch) {
   if (!path || !*path) return;
   std::ifstream f(path);
   if (!f) return;
@@ -74,12 +125,21 @@ void read_cache(const char* path, int* variant) {
   const std::string s = ss.str();
   size_t pos = 0;
   while ((pos = s.find("\"length\":", pos)) != std::string::npos) {
-    unsigned L = 0;
-    int V = 0;
-    if (std::sscanf(s.c_str() + pos, "\"length\": %u, \"order\": %d", &L, &V) == 2) {
-      const int b = log2u(L) - SPX_BATCH_LOG_MIN;
-      if (b >= 0 && b < SPX_BATCH_BUCKETS && (1u << (SPX_BATCH_LOG_MIN + b)) == L && (V == 0 || V == 1))
-        variant[b] = has_variants(b) ? V : 0;
+    const size_t nxt = s.find("\"length\":", pos + 9);
+    size_t end = s.find("\"candidates\"", pos);
+    end = std::min(end, nxt);
+    if (end == std::string::npos) end = s.size();
+    const int L = json_int(s, pos, end, "length");
+    if (L > 0) {
+      const int b = log2u((uint32_t)L) - SPX_BATCH_LOG_MIN;
+      if (b >= 0 && b < SPX_BATCH_BUCKETS && bucket_len(b) == (uint32_t)L) {
+        Choice x;
+        x.order = json_int(s, pos, end, "order");
+        const int t = json_int(s, pos, end, "tile"), g = json_int(s, pos, end, "groups");
+        if (t > 0) x.tile = t;
+        if (g > 0) x.groups = g;
+        ch[b] = sanitize(b, x);
+      }
     }
     pos += 9;
   }
@@ -113,64 +173,86 @@ void release(spx_batch_plan* p) {
   delete p;
 }
 
-// The plan's launches in stream order: the split-path sequence of every long
-// length (the longest work), then the persistent LDS-tile launch.
+// The plan's launches in stream order: the split-path sequences of every long
+// length (the longest work), then one persistent launch per tile shape.
 hipError_t launch_all(spx_batch_plan* p, hipStream_t st) {
   for (const SplitGroup& g : p->split) {
     hipError_t e = spx_launch_fourstep_ex(&g.fa, p->d_scratch, g.chunk, (const uint32_t*)p->d_perm + g.first,
                                           (const uint64_t*)p->d_offsets, st);
     if (e != hipSuccess) return e;
   }
-  if (p->n_items) return spx_launch_batch(&p->args, p->grid, st);
+  for (const TileLaunch& t : p->tl) {
+    hipError_t e = spx_launch_batch(&t.args, t.tile, t.grid, st);
+    if (e != hipSuccess) return e;
+  }
   return hipSuccess;
 }
 
-// Builds a plan; bucket_only >= 0 restricts it to one length and forces
-// `force_variant` there (the autotuner's measurement plans).
-int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_variant, bool graph, spx_batch_plan** out) {
+// Builds a plan.  The autotuner's measurement plans pass `force` (a choice
+// per bucket, used instead of the cache) and `mask` (bit b: keep the
+// channels of bucket b; ~0u keeps all).
+int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* force, bool graph, spx_batch_plan** out) {
   int rc = check_desc(c, d);
   if (rc) return rc;
   spx_batch_plan* p = new spx_batch_plan();
   p->c = c;
-  read_cache(d->tune_cache, p->variant);
-  if (bucket_only >= 0) p->variant[bucket_only] = has_variants(bucket_only) ? force_variant : 0;
+  if (force) {
+    for (int b = 0; b < SPX_BATCH_BUCKETS; ++b) p->ch[b] = sanitize(b, force[b]);
+  } else {
+    read_cache(d->tune_cache, p->ch);
+  }
 
   // group channels by bucket (ascending channel id inside a bucket)
   std::vector<std::vector<uint32_t>> by(SPX_BATCH_BUCKETS);
   for (uint32_t ch = 0; ch < d->n_chan; ++ch) {
     const int b = log2u(d->lengths[ch]) - SPX_BATCH_LOG_MIN;
-    if (bucket_only >= 0 && b != bucket_only) continue;
+    if (!((mask >> b) & 1u)) continue;
     by[b].push_back(ch);
   }
-  // long lengths: one split-path group each (their channels first in perm);
-  // short lengths: work list of LDS tiles, longest-processing-time first
+  // long lengths: one split-path group each (their channels first in perm),
+  // run in `groups` launch groups; short lengths: one work list of LDS tiles
+  // per tile shape, longest-processing-time first
   std::vector<uint32_t> perm;
-  std::vector<uint32_t> items;  // 4 words per item
+  std::vector<uint32_t> items[3];  // per tile shape, 4 words per item
   for (int b = SPX_BATCH_BUCKETS - 1; b >= 0; --b) {
     const auto& v = by[b];
     p->chans_per_bucket[b] = (uint32_t)v.size();
     if (v.empty()) continue;
-    const uint32_t L = 1u << (SPX_BATCH_LOG_MIN + b);
-    if (b >= SPX_BATCH_FIRST_LARGE) {
+    const uint32_t L = bucket_len(b);
+    if (is_split(b)) {
       SplitGroup g;
       std::memset(&g.fa, 0, sizeof(g.fa));
       g.fa.batch = (uint32_t)v.size();
       g.first = (uint32_t)perm.size();
-      g.chunk = fourstep_chunk(L, g.fa.batch);
+      const uint32_t per_group = (g.fa.batch + (uint32_t)p->ch[b].groups - 1) / (uint32_t)p->ch[b].groups;
+      g.chunk = std::min(fourstep_chunk(L, g.fa.batch), per_group);
       p->split.push_back(g);
       p->items_per_bucket[b] = (g.fa.batch + g.chunk - 1) / g.chunk;
+      p->chunk_per_bucket[b] = g.chunk;
     } else {
-      const uint32_t G = 16384u / L;
-      const uint32_t word = (uint32_t)b | ((uint32_t)p->variant[b] << 8);
+      const int ti = p->ch[b].tile == 16384 ? 0 : p->ch[b].tile == 8192 ? 1 : 2;
+      const uint32_t G = (uint32_t)kTiles[ti] / L;
+      const uint32_t word = (uint32_t)b | ((uint32_t)p->ch[b].order << 8);
       for (size_t i = 0; i < v.size(); i += G) {
         const uint32_t cnt = (uint32_t)std::min<size_t>(G, v.size() - i);
-        items.insert(items.end(), {word, (uint32_t)(perm.size() + i), cnt, 0u});
+        items[ti].insert(items[ti].end(), {word, (uint32_t)(perm.size() + i), cnt, 0u});
         p->items_per_bucket[b]++;
       }
     }
     perm.insert(perm.end(), v.begin(), v.end());
   }
-  p->n_items = (uint32_t)(items.size() / 4);
+  std::vector<uint32_t> all_items;
+  std::vector<size_t> item_first;  // first item of each launch in all_items
+  for (int ti = 0; ti < 3; ++ti) {
+    if (items[ti].empty()) continue;
+    TileLaunch t;
+    t.tile = kTiles[ti];
+    t.n_items = (uint32_t)(items[ti].size() / 4);
+    item_first.push_back(all_items.size() / 4);
+    all_items.insert(all_items.end(), items[ti].begin(), items[ti].end());
+    p->tl.push_back(t);
+  }
+  p->n_items = (uint32_t)(all_items.size() / 4);
   if (p->n_items == 0 && p->split.empty()) {
     release(p);
     return fail(c, SPX_EINVAL, "spx_batch: no channels in the requested length");
@@ -183,25 +265,23 @@ int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_varian
   hipError_t e;
   if ((e = hipSetDevice(c->device)) != hipSuccess) return fail_hip(e, "hipSetDevice");
   const size_t off_bytes = (size_t)d->n_chan * sizeof(uint64_t);
-  if ((e = hipMalloc(&p->d_items, std::max<size_t>(items.size(), 4) * 4)) != hipSuccess)
+  if ((e = hipMalloc(&p->d_items, std::max<size_t>(all_items.size(), 4) * 4)) != hipSuccess)
     return fail_hip(e, "hipMalloc(items)");
   if ((e = hipMalloc(&p->d_perm, perm.size() * 4)) != hipSuccess) return fail_hip(e, "hipMalloc(perm)");
   if ((e = hipMalloc(&p->d_offsets, off_bytes)) != hipSuccess) return fail_hip(e, "hipMalloc(offsets)");
-  if ((e = hipMalloc(&p->d_counter, 256)) != hipSuccess) return fail_hip(e, "hipMalloc(counter)");
-  if (!items.empty() && (e = hipMemcpy(p->d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+  if ((e = hipMalloc(&p->d_counter, 64 * 3)) != hipSuccess) return fail_hip(e, "hipMalloc(counter)");
+  if (!all_items.empty() &&
+      (e = hipMemcpy(p->d_items, all_items.data(), all_items.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
     return fail_hip(e, "upload items");
   if ((e = hipMemcpy(p->d_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
     return fail_hip(e, "upload perm");
   if ((e = hipMemcpy(p->d_offsets, d->offsets, off_bytes, hipMemcpyHostToDevice)) != hipSuccess)
     return fail_hip(e, "upload offsets");
 
-  BatchArgs& a = p->args;
+  BatchArgs a{};
   a.in = d->input;
   a.offsets = (const uint64_t*)p->d_offsets;
   a.perm = (const uint32_t*)p->d_perm;
-  a.items = (const uint4*)p->d_items;
-  a.counter = (uint32_t*)p->d_counter;
-  a.n_items = p->n_items;
   a.dtype = d->dtype;
   a.kcap = d->max_peaks;
   a.refine = d->refine;
@@ -214,12 +294,21 @@ int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_varian
   const void* t = nullptr;
   for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b) {
     if (by[b].empty()) continue;
-    const uint32_t L = 1u << (SPX_BATCH_LOG_MIN + b);
-    if ((rc = get_table(c, TW, tuned_len(b), (uint32_t)p->variant[b], &t))) { release(p); return rc; }
+    const uint32_t L = bucket_len(b);
+    if ((rc = get_table(c, TW, tuned_len(b), (uint32_t)p->ch[b].order, &t))) { release(p); return rc; }
     a.bk[b].tw = (const float2*)t;
     const float* w = nullptr;
     if ((rc = window_ptr(c, d->window, L, &w))) { release(p); return rc; }
     a.bk[b].window = w;
+  }
+  const int cu = c->cu_count > 0 ? c->cu_count : 256;
+  for (size_t i = 0; i < p->tl.size(); ++i) {
+    TileLaunch& tl = p->tl[i];
+    tl.args = a;
+    tl.args.items = (const uint4*)p->d_items + item_first[i];
+    tl.args.n_items = tl.n_items;
+    tl.args.counter = (uint32_t*)((char*)p->d_counter + 64 * i);
+    tl.grid = (int)std::min<uint64_t>((uint64_t)cu * (16384 / tl.tile), tl.n_items);
   }
   uint64_t ws = 0;
   for (SplitGroup& g : p->split) {
@@ -251,8 +340,6 @@ int build(spx_ctx* c, const spx_batch_desc* d, int bucket_only, int force_varian
     ws = std::max<uint64_t>(ws, spx_fs_workspace_bytes(L, g.chunk));
   }
   if ((e = spx_batch_kernel_init()) != hipSuccess) return fail_hip(e, "spx_batch_kernel_init");
-  p->grid = c->cu_count > 0 ? c->cu_count : 256;
-  if ((uint32_t)p->grid > p->n_items) p->grid = (int)p->n_items;
   if (ws) {
     p->scratch_bytes = ws;
     if ((e = hipMalloc(&p->d_scratch, p->scratch_bytes)) != hipSuccess) {
@@ -293,7 +380,7 @@ extern "C" {
 int spx_batch_plan_create(spx_ctx* ctx, const spx_batch_desc* desc, spx_batch_plan** out) {
   if (!ctx || !out) return SPX_EINVAL;
   *out = nullptr;
-  return build(ctx, desc, -1, 0, !(desc && (desc->flags & SPX_BATCH_NO_GRAPH)), out);
+  return build(ctx, desc, ~0u, nullptr, !(desc && (desc->flags & SPX_BATCH_NO_GRAPH)), out);
 }
 
 int spx_batch_execute(spx_batch_plan* plan) {
@@ -310,14 +397,23 @@ int spx_batch_plan_destroy(spx_batch_plan* plan) {
 int spx_batch_plan_info(spx_batch_plan* p, char* buf, size_t len) {
   if (!p || !buf || len == 0) return SPX_EINVAL;
   std::ostringstream o;
-  o << "{\"items\": " << p->n_items << ", \"grid\": " << p->grid << ", \"graph\": " << (p->graph ? "true" : "false")
-    << ", \"scratch_bytes\": " << p->scratch_bytes << ", \"lengths\": [";
+  o << "{\"items\": " << p->n_items << ", \"graph\": " << (p->graph ? "true" : "false")
+    << ", \"scratch_bytes\": " << p->scratch_bytes << ", \"launches\": [";
+  for (size_t i = 0; i < p->tl.size(); ++i)
+    o << (i ? ", " : "") << "{\"tile\": " << p->tl[i].tile << ", \"threads\": " << p->tl[i].tile / 16
+      << ", \"grid\": " << p->tl[i].grid << ", \"items\": " << p->tl[i].n_items << "}";
+  o << "], \"lengths\": [";
   bool first = true;
   for (int b = 0; b < SPX_BATCH_BUCKETS; ++b) {
     if (!p->chans_per_bucket[b]) continue;
-    o << (first ? "" : ", ") << "{\"length\": " << (1u << (SPX_BATCH_LOG_MIN + b)) << ", \"order\": " << p->variant[b]
-      << ", \"channels\": " << p->chans_per_bucket[b] << ", \"items\": " << p->items_per_bucket[b]
-      << ", \"path\": \"" << (b >= SPX_BATCH_FIRST_LARGE ? "split" : "lds-tile") << "\"}";
+    o << (first ? "" : ", ") << "{\"length\": " << bucket_len(b) << ", \"order\": " << p->ch[b].order
+      << ", \"channels\": " << p->chans_per_bucket[b] << ", \"items\": " << p->items_per_bucket[b];
+    if (is_split(b))
+      o << ", \"path\": \"split\", \"groups\": " << p->items_per_bucket[b] << ", \"chunk\": " << p->chunk_per_bucket[b];
+    else
+      o << ", \"path\": \"lds-tile\", \"tile\": " << p->ch[b].tile << ", \"threads\": " << p->ch[b].tile / 16
+        << ", \"channels_per_tile\": " << (uint32_t)p->ch[b].tile / bucket_len(b);
+    o << "}";
     first = false;
   }
   o << "]}";
@@ -335,54 +431,166 @@ int spx_batch_autotune(spx_ctx* c, const spx_batch_desc* d, const char* cache_pa
   for (uint32_t i = 0; i < d->n_chan; ++i) count[log2u(d->lengths[i]) - SPX_BATCH_LOG_MIN]++;
   spx_batch_desc dd = *d;
   dd.tune_cache = nullptr;
-  hipEvent_t e0, e1;
   SPX_HIP(c, hipSetDevice(c->device));
-  SPX_HIP(c, hipEventCreate(&e0));
-  SPX_HIP(c, hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  auto done = [&](int r) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return r;
+  };
+  if (he != hipSuccess) return done(hip_fail(c, he, "hipEventCreate"));
+  // median of the last 5 of 7 executes of a plan over the buckets in `mask`
+  auto time_us = [&](uint32_t mask, const Choice* x, double* us) -> int {
+    spx_batch_plan* p = nullptr;
+    int r = build(c, &dd, mask, x, false, &p);
+    if (r) return r;
+    std::vector<float> t;
+    for (int k = 0; k < 7 && r == SPX_OK; ++k) {
+      if (hipEventRecord(e0, c->stream) != hipSuccess) r = SPX_EHIP;
+      if (!r) r = run(p);
+      if (!r && hipEventRecord(e1, c->stream) != hipSuccess) r = SPX_EHIP;
+      if (!r && hipEventSynchronize(e1) != hipSuccess) r = SPX_EHIP;
+      float ms = 0;
+      if (!r && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) r = SPX_EHIP;
+      if (k >= 2) t.push_back(ms);
+    }
+    release(p);
+    if (r) return fail(c, r, "spx_batch_autotune: timing run failed");
+    std::sort(t.begin(), t.end());
+    *us = 1000.0 * t[t.size() / 2];
+    return SPX_OK;
+  };
+  Choice best[SPX_BATCH_BUCKETS];
+  std::vector<std::vector<std::pair<Choice, double>>> cands(SPX_BATCH_BUCKETS);
+  uint32_t lds_mask = 0;
+  // 1. every length alone: each (radix order, tile shape) of an LDS-tile
+  //    length, 1 / 2 / 4 launch groups (of at least 16 channels) of a split one
+  for (int b = 0; b < SPX_BATCH_BUCKETS; ++b) {
+    if (!count[b]) continue;
+    std::vector<Choice> cs;
+    if (is_split(b)) {
+      for (int g : {1, 2, 4})
+        if (g == 1 || count[b] / (uint32_t)g >= 16) {
+          Choice x;
+          x.groups = g;
+          cs.push_back(x);
+        }
+    } else {
+      lds_mask |= 1u << b;
+      for (int tile : kTiles)
+        for (int v = 0; v < (has_variants(b) ? 2 : 1); ++v)
+          if (tile_fits(b, tile)) {
+            Choice x;
+            x.order = v;
+            x.tile = tile;
+            cs.push_back(x);
+          }
+    }
+    double bu = 0;
+    for (size_t i = 0; i < cs.size(); ++i) {
+      Choice f[SPX_BATCH_BUCKETS];
+      f[b] = cs[i];
+      double us = 0;
+      if ((rc = time_us(1u << b, f, &us))) return done(rc);
+      cands[b].push_back({cs[i], us});
+      if (i == 0 || us < bu) {
+        bu = us;
+        best[b] = cs[i];
+      }
+    }
+  }
+  // 2. the LDS-tile lengths share persistent launches (one per tile shape in
+  //    use), so their shapes are chosen together: the assignments "every
+  //    length on its largest fitting tile up to 16384 / 8192 / 4096" and the
+  //    per-length fastest one, each timed over all LDS-tile lengths at once
+  struct TileSet {
+    Choice ch[SPX_BATCH_BUCKETS];
+    double us = 0;
+  };
+  std::vector<TileSet> sets;
+  if (lds_mask) {
+    auto order_for = [&](int b, int tile) {
+      int o = 0;
+      double bu = -1;
+      for (const auto& pc : cands[b])
+        if (pc.first.tile == tile && (bu < 0 || pc.second < bu)) {
+          bu = pc.second;
+          o = pc.first.order;
+        }
+      return o;
+    };
+    for (int cap : kTiles) {
+      TileSet t;
+      for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b) {
+        if (!count[b]) continue;
+        // the largest fitting tile <= cap, else the smallest fitting one
+        // (kTiles descend)
+        int tile = 0;
+        for (int tt : kTiles)
+          if (tt <= cap && tile_fits(b, tt)) { tile = tt; break; }
+        if (!tile)
+          for (int tt : kTiles)
+            if (tile_fits(b, tt)) tile = tt;
+        t.ch[b].tile = tile;
+        t.ch[b].order = order_for(b, tile);
+      }
+      sets.push_back(t);
+    }
+    TileSet own;
+    for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b)
+      if (count[b]) own.ch[b] = best[b];
+    sets.push_back(own);
+    for (TileSet& t : sets)
+      if ((rc = time_us(lds_mask, t.ch, &t.us))) return done(rc);
+    size_t w = 0;
+    for (size_t i = 1; i < sets.size(); ++i)
+      if (sets[i].us < sets[w].us) w = i;
+    for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b)
+      if (count[b]) best[b] = sets[w].ch[b];
+  }
   std::ostringstream o;
   o << "{\"device\": \"" << c->name << "\", \"cu\": " << c->cu_count << ", \"lengths\": [";
   bool first = true;
+  char line[200];
   for (int b = 0; b < SPX_BATCH_BUCKETS; ++b) {
     if (!count[b]) continue;
-    const int nv = has_variants(b) ? 2 : 1;
-    double us[2] = {0, 0};
-    for (int v = 0; v < nv; ++v) {
-      spx_batch_plan* p = nullptr;
-      if ((rc = build(c, &dd, b, v, false, &p))) return rc;
-      std::vector<float> t;
-      for (int r = 0; r < 7 && rc == SPX_OK; ++r) {
-        if (hipEventRecord(e0, c->stream) != hipSuccess) rc = SPX_EHIP;
-        if (!rc) rc = run(p);
-        if (!rc && hipEventRecord(e1, c->stream) != hipSuccess) rc = SPX_EHIP;
-        if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = SPX_EHIP;
-        float ms = 0;
-        if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = SPX_EHIP;
-        if (r >= 2) t.push_back(ms);
-      }
-      release(p);
-      if (rc) return fail(c, rc, "spx_batch_autotune: timing run failed");
-      std::sort(t.begin(), t.end());
-      us[v] = 1000.0 * t[t.size() / 2];
+    const Choice& x = best[b];
+    double us = 0;
+    for (const auto& pc : cands[b])
+      if (pc.first.order == x.order && pc.first.tile == x.tile && pc.first.groups == x.groups) us = pc.second;
+    std::snprintf(line, sizeof line, "{\"length\": %u, \"order\": %d, \"tile\": %d, \"groups\": %d, \"us\": %.3f, "
+                  "\"channels\": %u, \"candidates\": [", bucket_len(b), x.order, is_split(b) ? 0 : x.tile,
+                  is_split(b) ? x.groups : 1, us, count[b]);
+    o << (first ? "\n  " : ",\n  ") << line;
+    for (size_t i = 0; i < cands[b].size(); ++i) {
+      const Choice& y = cands[b][i].first;
+      std::snprintf(line, sizeof line, "%s{\"order\": %d, \"tile\": %d, \"groups\": %d, \"us\": %.3f}", i ? ", " : "",
+                    y.order, is_split(b) ? 0 : y.tile, is_split(b) ? y.groups : 1, cands[b][i].second);
+      o << line;
     }
-    const int best = (nv == 2 && us[1] < us[0]) ? 1 : 0;
-    char line[256];
-    if (nv == 2)
-      std::snprintf(line, sizeof line, "{\"length\": %u, \"order\": %d, \"us\": [%.3f, %.3f], \"channels\": %u}",
-                    1u << (SPX_BATCH_LOG_MIN + b), best, us[0], us[1], count[b]);
-    else
-      std::snprintf(line, sizeof line, "{\"length\": %u, \"order\": 0, \"us\": [%.3f], \"channels\": %u}",
-                    1u << (SPX_BATCH_LOG_MIN + b), us[0], count[b]);
-    const std::string ls(line);
-    o << (first ? "\n  " : ",\n  ") << ls;
+    o << "]}";
     first = false;
   }
-  o << "\n]}\n";
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  o << "\n], \"tile_sets\": [";
+  for (size_t i = 0; i < sets.size(); ++i) {
+    o << (i ? ",\n  " : "\n  ") << "{\"us\": ";
+    std::snprintf(line, sizeof line, "%.3f", sets[i].us);
+    o << line << ", \"tiles\": {";
+    bool f2 = true;
+    for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b)
+      if (count[b]) {
+        o << (f2 ? "" : ", ") << "\"" << bucket_len(b) << "\": " << sets[i].ch[b].tile;
+        f2 = false;
+      }
+    o << "}}";
+  }
+  o << "]}\n";
   std::ofstream f(cache_path);
-  if (!f) return fail(c, SPX_EINVAL, std::string("spx_batch_autotune: cannot write ") + cache_path);
+  if (!f) return done(fail(c, SPX_EINVAL, std::string("spx_batch_autotune: cannot write ") + cache_path));
   f << o.str();
-  return f.good() ? SPX_OK : fail(c, SPX_EINVAL, "spx_batch_autotune: write failed");
+  return done(f.good() ? SPX_OK : fail(c, SPX_EINVAL, "spx_batch_autotune: write failed"));
 }
 
 }  // extern "C"

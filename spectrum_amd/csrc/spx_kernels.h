@@ -45,9 +45,10 @@ struct FusedArgs {
 };
 
 // Uneven-length batch.  Bucket b holds length L = 2^(10+b): b 0..4 run in
-// one persistent launch over a work list of tiles of 16384/L channels (whole
-// channels in LDS); b 5..7 run on the split four-step path, one launch
-// sequence per length with channel indirection (spx_launch_fourstep_ex).
+// persistent launches (one per tile shape TILE = 16384, 8192 or 4096 points)
+// over work lists of tiles of TILE/L whole channels in LDS; b 5..7 run on the
+// split four-step path, one launch sequence per length (or per chunk of its
+// channels) with channel indirection (spx_launch_fourstep_ex).
 #define SPX_BATCH_BUCKETS 8
 #define SPX_BATCH_LOG_MIN 10
 #define SPX_BATCH_FIRST_LARGE 5
@@ -62,7 +63,7 @@ struct BatchArgs {
   const uint64_t* offsets; // [n_chan] element offset of each channel in `in`
   const uint32_t* perm;    // channel ids in work-list order
   const uint4* items;      // {bucket | variant << 8, first index into perm, channel count, 0}
-  uint32_t* counter;       // work-list cursor (zeroed by spx_launch_batch before the batch kernel)
+  uint32_t* counter;       // work-list cursor of this launch (zeroed by spx_launch_batch before the batch kernel)
   BatchBucket bk[SPX_BATCH_FIRST_LARGE];
   uint32_t n_items, dtype, kcap, refine;
   float fs, nsigma;
@@ -89,7 +90,8 @@ hipError_t spx_launch_mlp_predict(const void* dep, const float* X, uint32_t n, f
 hipError_t spx_launch_mlp_refine(const void* dep, const void* feat, void* peaks, const uint32_t* counts,
                                  uint32_t batch, uint32_t kcap, float bin_hz, hipStream_t st);
 hipError_t spx_launch_mixed(const FusedArgs* a, int compute, int grid, hipStream_t st);
-hipError_t spx_launch_batch(const BatchArgs* b, int grid, hipStream_t st);
+int spx_batch_tile_ok(int tile);
+hipError_t spx_launch_batch(const BatchArgs* b, int tile, int grid, hipStream_t st);
 hipError_t spx_launch_fused(const FusedArgs* a, int grid, hipStream_t st);
 int spx_fused_is_large(uint32_t nfft);
 uint64_t spx_fs_workspace_bytes(uint32_t nfft, uint32_t chunk);

@@ -154,18 +154,68 @@ def test_alternate_radix_order_and_autotune(ctx, tmp_path):
     pick = [int(ids[i]) for _, (_, ids) in sorted(groups.items()) for i in range(min(2, len(ids)))]
     oracle_check(x, off, L, got, pick)
     plan.close()
-    # the autotuner measures both orders per length and writes its cache
+    # the autotuner measures every (order, tile shape) of the LDS-tile lengths
+    # and 1 / 2 / 4 launch groups of the split lengths, and writes its cache
     tuned = tmp_path / "tuned.json"
     ctx.batch_autotune(ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], max_peaks=K), str(tuned))
     data = json.loads(tuned.read_text())
     assert {e["length"] for e in data["lengths"]} == set(int(v) for v in np.unique(L))
+    sets = data["tile_sets"]
+    win = min(sets, key=lambda t: t["us"])
     for e in data["lengths"]:
-        assert e["order"] in (0, 1) and all(t > 0 for t in e["us"])
-        if len(e["us"]) == 2:
-            assert e["us"][e["order"]] == min(e["us"])
+        cands = e["candidates"]
+        assert all(c["us"] > 0 for c in cands)
+        chosen = [c for c in cands if (c["order"], c["tile"], c["groups"]) == (e["order"], e["tile"], e["groups"])]
+        assert len(chosen) == 1 and chosen[0]["us"] == e["us"], (e, cands)
+        Ln = e["length"]
+        if Ln >= 32768:
+            # launch groups of a split length: the fastest alone
+            assert {c["groups"] for c in cands} <= {1, 2, 4} and all(c["order"] == 0 for c in cands)
+            assert e["us"] == min(c["us"] for c in cands)
+        else:
+            two = (Ln.bit_length() - 1) % 4 != 0
+            want = {(o, t) for t in (16384, 8192, 4096) if t >= Ln for o in ((0, 1) if two else (0,))}
+            assert {(c["order"], c["tile"]) for c in cands} == want, (Ln, cands)
+            # tile shapes: those of the fastest set timed over all LDS-tile
+            # lengths together; order: the faster one on that shape
+            assert e["tile"] == win["tiles"][str(Ln)]
+            assert e["us"] == min(c["us"] for c in cands if c["tile"] == e["tile"])
     plan = ctx.batch_plan(ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], max_peaks=K, tune_cache=str(tuned)))
-    assert {b["length"]: b["order"] for b in plan.info()["lengths"]} == {e["length"]: e["order"] for e in data["lengths"]}
+    info = plan.info()
+    for b in info["lengths"]:
+        e = [x for x in data["lengths"] if x["length"] == b["length"]][0]
+        assert b["order"] == e["order"]
+        if b["path"] == "lds-tile":
+            assert b["tile"] == e["tile"] and b["channels_per_tile"] == e["tile"] // b["length"]
     plan.close()
+
+
+@pytest.mark.parametrize("tile", [16384, 8192, 4096])
+def test_tile_shapes_equal_per_length_launches(ctx, tmp_path, tile):
+    """Every tile shape (and both radix orders where a length has two), with the
+    split lengths in two launch groups, gives the per-length launches' results."""
+    n = 96
+    L, off, total, groups, x, d_in = make_batch(ctx, n, 0x5EED0403)
+    ref = sequential(ctx, d_in, groups, n)
+    for order in (0, 1):
+        print(f"[gpu] batch tile={tile} order={order} lengths={sorted(groups)}", flush=True)
+        cache = tmp_path / f"t{tile}_{order}.json"
+        cache.write_text(json.dumps({"lengths": [{"length": 1 << b, "order": order, "tile": tile, "groups": 2}
+                                                 for b in range(10, 18)]}))
+        bufs = outputs(ctx, n)
+        plan = ctx.batch_plan(ctx.batch_desc(d_in, off, L, bufs[0], bufs[1], gmax=bufs[2], stats=bufs[3],
+                                             max_peaks=K, tune_cache=str(cache)))
+        info = plan.info()
+        for b in info["lengths"]:
+            if b["path"] == "lds-tile":
+                assert b["tile"] == (tile if tile >= b["length"] else 16384), info
+            else:
+                assert b["groups"] == 2, info
+        assert {t["tile"] for t in info["launches"]} == {b["tile"] for b in info["lengths"] if b["path"] == "lds-tile"}
+        plan.execute()
+        ctx.sync()
+        assert_same(read(ctx, bufs, n), ref, f"tile {tile} order {order}")
+        plan.close()
 
 
 def test_full_uneven_config_properties(ctx):
