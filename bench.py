@@ -260,7 +260,7 @@ def extra_uneven(ctx, K=64):
             "per_length_launches_ms": t_seq, "speedup": t_seq / t_batch, "channels_per_s": n / (t_batch * 1e-3),
             "achieved_GBps": alg / (t_batch * 1e-3) / 1e9, "autotune_s": tune_s,
             "tuned": [{k: e[k] for k in ("length", "order", "tile", "groups", "us")} for e in tuned["lengths"]],
-            "tile_sets_us": [{"us": t["us"], "tiles": t["tiles"]} for t in tuned.get("tile_sets", [])],
+            "concurrent": bool(tuned.get("concurrent", 0)), "tile_sets": tuned.get("tile_sets", []),
             "plan": info}
 
 

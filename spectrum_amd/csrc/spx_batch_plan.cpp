@@ -62,6 +62,12 @@ struct spx_batch_plan {
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   Choice ch[SPX_BATCH_BUCKETS];
+  // the LDS-tile launches on a branch of their own beside the split
+  // sequences (a side stream forked from and joined back to the context's
+  // stream), instead of after them
+  bool concurrent = false;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   uint32_t items_per_bucket[SPX_BATCH_BUCKETS] = {0};
   uint32_t chans_per_bucket[SPX_BATCH_BUCKETS] = {0};
   uint32_t chunk_per_bucket[SPX_BATCH_BUCKETS] = {0};
@@ -113,16 +119,17 @@ int json_int(const std::string& s, size_t from, size_t to, const char* key) {
 }
 
 // Cache format (written by spx_batch_autotune):
-// {"device": "...", "cu": N, "lengths": [{"length": L, "order": V, "tile": T, "groups": g, "us": best,
+// {"device": "...", "cu": N, "concurrent": 0|1, "lengths": [{"length": L, "order": V, "tile": T, "groups": g, "us": best,
 //   "candidates": [...]}, ...]}; the fields of an entry come before its candidate list, and
 // "tile" / "groups" may be absent (defaults).
-void read_cache(const char* path, Choice* ch) {
+void read_cache(const char* path, Choice* ch, bool* concurrent) {
   if (!path || !*path) return;
   std::ifstream f(path);
   if (!f) return;
   std::stringstream ss;
   ss << f.rdbuf();
   const std::string s = ss.str();
+  *concurrent = json_int(s, 0, s.size(), "concurrent") == 1;
   size_t pos = 0;
   while ((pos = s.find("\"length\":", pos)) != std::string::npos) {
     const size_t nxt = s.find("\"length\":", pos + 9);
@@ -168,38 +175,56 @@ void release(spx_batch_plan* p) {
   if (p->c) (void)hipSetDevice(p->c->device);
   if (p->ge) (void)hipGraphExecDestroy(p->ge);
   if (p->g) (void)hipGraphDestroy(p->g);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  if (p->side) (void)hipStreamDestroy(p->side);
   for (void* q : {p->d_items, p->d_perm, p->d_offsets, p->d_counter, p->d_scratch})
     if (q) (void)hipFree(q);
   delete p;
 }
 
-// The plan's launches in stream order: the split-path sequences of every long
-// length (the longest work), then one persistent launch per tile shape.
+// The plan's launches: the split-path sequences of every long length (the
+// longest work), then one persistent launch per tile shape, in stream order
+// or (concurrent) on the side stream beside the split sequences.
 hipError_t launch_all(spx_batch_plan* p, hipStream_t st) {
+  const bool conc = p->concurrent && p->side && !p->tl.empty() && !p->split.empty();
+  hipStream_t ts = st;
+  hipError_t e;
+  if (conc) {
+    if ((e = hipEventRecord(p->ev_fork, st)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(p->side, p->ev_fork, 0)) != hipSuccess) return e;
+    ts = p->side;
+    for (const TileLaunch& t : p->tl)
+      if ((e = spx_launch_batch(&t.args, t.tile, t.grid, ts)) != hipSuccess) return e;
+  }
   for (const SplitGroup& g : p->split) {
-    hipError_t e = spx_launch_fourstep_ex(&g.fa, p->d_scratch, g.chunk, (const uint32_t*)p->d_perm + g.first,
-                                          (const uint64_t*)p->d_offsets, st);
+    e = spx_launch_fourstep_ex(&g.fa, p->d_scratch, g.chunk, (const uint32_t*)p->d_perm + g.first,
+                               (const uint64_t*)p->d_offsets, st);
     if (e != hipSuccess) return e;
   }
-  for (const TileLaunch& t : p->tl) {
-    hipError_t e = spx_launch_batch(&t.args, t.tile, t.grid, st);
-    if (e != hipSuccess) return e;
+  if (conc) {
+    if ((e = hipEventRecord(p->ev_join, p->side)) != hipSuccess) return e;
+    return hipStreamWaitEvent(st, p->ev_join, 0);
   }
+  for (const TileLaunch& t : p->tl)
+    if ((e = spx_launch_batch(&t.args, t.tile, t.grid, st)) != hipSuccess) return e;
   return hipSuccess;
 }
 
 // Builds a plan.  The autotuner's measurement plans pass `force` (a choice
-// per bucket, used instead of the cache) and `mask` (bit b: keep the
-// channels of bucket b; ~0u keeps all).
-int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* force, bool graph, spx_batch_plan** out) {
+// per bucket and the concurrency, used instead of the cache) and `mask`
+// (bit b: keep the channels of bucket b; ~0u keeps all).
+int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* force, bool force_conc, bool graph,
+          spx_batch_plan** out) {
   int rc = check_desc(c, d);
   if (rc) return rc;
   spx_batch_plan* p = new spx_batch_plan();
   p->c = c;
   if (force) {
     for (int b = 0; b < SPX_BATCH_BUCKETS; ++b) p->ch[b] = sanitize(b, force[b]);
+    p->concurrent = force_conc;
   } else {
-    read_cache(d->tune_cache, p->ch);
+    read_cache(d->tune_cache, p->ch, &p->concurrent);
   }
 
   // group channels by bucket (ascending channel id inside a bucket)
@@ -340,6 +365,15 @@ int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* forc
     ws = std::max<uint64_t>(ws, spx_fs_workspace_bytes(L, g.chunk));
   }
   if ((e = spx_batch_kernel_init()) != hipSuccess) return fail_hip(e, "spx_batch_kernel_init");
+  if (p->concurrent && !p->tl.empty() && !p->split.empty()) {
+    // default priority, like every stream of the library
+    if ((e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking)) != hipSuccess)
+      return fail_hip(e, "hipStreamCreate(side)");
+    if ((e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming)) != hipSuccess)
+      return fail_hip(e, "hipEventCreate(fork)");
+    if ((e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming)) != hipSuccess)
+      return fail_hip(e, "hipEventCreate(join)");
+  }
   if (ws) {
     p->scratch_bytes = ws;
     if ((e = hipMalloc(&p->d_scratch, p->scratch_bytes)) != hipSuccess) {
@@ -380,7 +414,7 @@ extern "C" {
 int spx_batch_plan_create(spx_ctx* ctx, const spx_batch_desc* desc, spx_batch_plan** out) {
   if (!ctx || !out) return SPX_EINVAL;
   *out = nullptr;
-  return build(ctx, desc, ~0u, nullptr, !(desc && (desc->flags & SPX_BATCH_NO_GRAPH)), out);
+  return build(ctx, desc, ~0u, nullptr, false, !(desc && (desc->flags & SPX_BATCH_NO_GRAPH)), out);
 }
 
 int spx_batch_execute(spx_batch_plan* plan) {
@@ -398,6 +432,7 @@ int spx_batch_plan_info(spx_batch_plan* p, char* buf, size_t len) {
   if (!p || !buf || len == 0) return SPX_EINVAL;
   std::ostringstream o;
   o << "{\"items\": " << p->n_items << ", \"graph\": " << (p->graph ? "true" : "false")
+    << ", \"concurrent\": " << (p->side ? "true" : "false")
     << ", \"scratch_bytes\": " << p->scratch_bytes << ", \"launches\": [";
   for (size_t i = 0; i < p->tl.size(); ++i)
     o << (i ? ", " : "") << "{\"tile\": " << p->tl[i].tile << ", \"threads\": " << p->tl[i].tile / 16
@@ -441,20 +476,20 @@ int spx_batch_autotune(spx_ctx* c, const spx_batch_desc* d, const char* cache_pa
     return r;
   };
   if (he != hipSuccess) return done(hip_fail(c, he, "hipEventCreate"));
-  // median of the last 5 of 7 executes of a plan over the buckets in `mask`
-  auto time_us = [&](uint32_t mask, const Choice* x, double* us) -> int {
+  // median of the last 10 of 13 executes of a plan over the buckets in `mask`
+  auto time_us = [&](uint32_t mask, const Choice* x, bool conc, double* us) -> int {
     spx_batch_plan* p = nullptr;
-    int r = build(c, &dd, mask, x, false, &p);
+    int r = build(c, &dd, mask, x, conc, false, &p);
     if (r) return r;
     std::vector<float> t;
-    for (int k = 0; k < 7 && r == SPX_OK; ++k) {
+    for (int k = 0; k < 13 && r == SPX_OK; ++k) {
       if (hipEventRecord(e0, c->stream) != hipSuccess) r = SPX_EHIP;
       if (!r) r = run(p);
       if (!r && hipEventRecord(e1, c->stream) != hipSuccess) r = SPX_EHIP;
       if (!r && hipEventSynchronize(e1) != hipSuccess) r = SPX_EHIP;
       float ms = 0;
       if (!r && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) r = SPX_EHIP;
-      if (k >= 2) t.push_back(ms);
+      if (k >= 3) t.push_back(ms);
     }
     release(p);
     if (r) return fail(c, r, "spx_batch_autotune: timing run failed");
@@ -488,18 +523,20 @@ int spx_batch_autotune(spx_ctx* c, const spx_batch_desc* d, const char* cache_pa
             cs.push_back(x);
           }
     }
+    // the fastest, but the default (the first candidate) unless it is 2 % faster
     double bu = 0;
     for (size_t i = 0; i < cs.size(); ++i) {
       Choice f[SPX_BATCH_BUCKETS];
       f[b] = cs[i];
       double us = 0;
-      if ((rc = time_us(1u << b, f, &us))) return done(rc);
+      if ((rc = time_us(1u << b, f, false, &us))) return done(rc);
       cands[b].push_back({cs[i], us});
       if (i == 0 || us < bu) {
         bu = us;
         best[b] = cs[i];
       }
     }
+    if (!(bu < 0.98 * cands[b][0].second)) best[b] = cands[b][0].first;
   }
   // 2. the LDS-tile lengths share persistent launches (one per tile shape in
   //    use), so their shapes are chosen together: the assignments "every
@@ -511,15 +548,12 @@ int spx_batch_autotune(spx_ctx* c, const spx_batch_desc* d, const char* cache_pa
   };
   std::vector<TileSet> sets;
   if (lds_mask) {
+    // radix order on a tile shape: 1 only where it is 2 % faster than 0
     auto order_for = [&](int b, int tile) {
-      int o = 0;
-      double bu = -1;
+      double t[2] = {-1, -1};
       for (const auto& pc : cands[b])
-        if (pc.first.tile == tile && (bu < 0 || pc.second < bu)) {
-          bu = pc.second;
-          o = pc.first.order;
-        }
-      return o;
+        if (pc.first.tile == tile) t[pc.first.order] = pc.second;
+      return (t[1] > 0 && t[0] > 0 && t[1] < 0.98 * t[0]) ? 1 : 0;
     };
     for (int cap : kTiles) {
       TileSet t;
@@ -543,15 +577,53 @@ int spx_batch_autotune(spx_ctx* c, const spx_batch_desc* d, const char* cache_pa
       if (count[b]) own.ch[b] = best[b];
     sets.push_back(own);
     for (TileSet& t : sets)
-      if ((rc = time_us(lds_mask, t.ch, &t.us))) return done(rc);
+      if ((rc = time_us(lds_mask, t.ch, false, &t.us))) return done(rc);
     size_t w = 0;
     for (size_t i = 1; i < sets.size(); ++i)
       if (sets[i].us < sets[w].us) w = i;
     for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b)
       if (count[b]) best[b] = sets[w].ch[b];
   }
+  // 3. with split lengths present too: the whole batch with the LDS-tile
+  //    launches after the split sequences or beside them, for every tile set
+  //    (beside them, small tiles can share CUs with the split kernels); each
+  //    candidate timed in three alternating rounds (median), and the default
+  //    (first set, in sequence) kept unless another is 2 % faster
+  bool conc = false;
+  std::vector<double> whole[2];
+  const bool any_split = (lds_mask != 0) && [&] {
+    for (int b = SPX_BATCH_FIRST_LARGE; b < SPX_BATCH_BUCKETS; ++b)
+      if (count[b]) return true;
+    return false;
+  }();
+  if (any_split) {
+    const size_t nc = sets.size() * 2;
+    std::vector<std::vector<double>> rounds(nc);
+    for (int round = 0; round < 3; ++round)
+      for (size_t k = 0; k < nc; ++k) {
+        Choice f[SPX_BATCH_BUCKETS];
+        for (int b = 0; b < SPX_BATCH_BUCKETS; ++b) f[b] = is_split(b) ? best[b] : sets[k / 2].ch[b];
+        double us = 0;
+        if ((rc = time_us(~0u, f, (k & 1) == 1, &us))) return done(rc);
+        rounds[k].push_back(us);
+      }
+    std::vector<double> med(nc);
+    for (size_t k = 0; k < nc; ++k) {
+      std::sort(rounds[k].begin(), rounds[k].end());
+      med[k] = rounds[k][1];
+      whole[k & 1].push_back(med[k]);
+    }
+    size_t bk = 0;
+    for (size_t k = 1; k < nc; ++k)
+      if (med[k] < med[bk]) bk = k;
+    if (!(med[bk] < 0.98 * med[0])) bk = 0;
+    conc = (bk & 1) == 1;
+    for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b)
+      if (count[b]) best[b] = sets[bk / 2].ch[b];
+  }
   std::ostringstream o;
-  o << "{\"device\": \"" << c->name << "\", \"cu\": " << c->cu_count << ", \"lengths\": [";
+  o << "{\"device\": \"" << c->name << "\", \"cu\": " << c->cu_count << ", \"concurrent\": " << (conc ? 1 : 0)
+    << ", \"lengths\": [";
   bool first = true;
   char line[200];
   for (int b = 0; b < SPX_BATCH_BUCKETS; ++b) {
@@ -577,7 +649,12 @@ int spx_batch_autotune(spx_ctx* c, const spx_batch_desc* d, const char* cache_pa
   for (size_t i = 0; i < sets.size(); ++i) {
     o << (i ? ",\n  " : "\n  ") << "{\"us\": ";
     std::snprintf(line, sizeof line, "%.3f", sets[i].us);
-    o << line << ", \"tiles\": {";
+    o << line;
+    if (any_split) {
+      std::snprintf(line, sizeof line, ", \"whole_us\": %.3f, \"whole_concurrent_us\": %.3f", whole[0][i], whole[1][i]);
+      o << line;
+    }
+    o << ", \"tiles\": {";
     bool f2 = true;
     for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b)
       if (count[b]) {
