@@ -12,6 +12,7 @@ weights) is what spx_fused_peaks uses with refine = REFINE_LEARNED.
 """
 from __future__ import annotations
 
+import hashlib
 import os
 from dataclasses import dataclass
 
@@ -21,6 +22,49 @@ from . import capi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "data", "refiner_hann_8192.npz")
+
+# Weights file: an .npz (no pickled objects) with the FP32 master parameters
+# `params` in the order the library's kernels read them, the architecture tag
+# `format`, the file-format `version` and the SHA-256 of the parameters'
+# little-endian bytes (`sha256`), plus free-form training metadata.
+WEIGHTS_FORMAT = "spx-refiner mlp 10-64-64-1 relu fp32"
+WEIGHTS_VERSION = 1
+WEIGHTS_NPARAM = 10 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+def params_digest(p: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(p, dtype="<f4").tobytes()).hexdigest()
+
+
+def read_weights(path: str) -> np.ndarray:
+    """Parameters of a weights file, checked: architecture tag and version
+    (when the file has them), parameter count and dtype, finiteness, and the
+    checksum.  Raises ValueError on any mismatch."""
+    with np.load(path, allow_pickle=False) as z:
+        if "params" not in z.files:
+            raise ValueError(f"{path}: no 'params' array")
+        p = z["params"]
+        if "format" in z.files and str(z["format"]) != WEIGHTS_FORMAT:
+            raise ValueError(f"{path}: format {str(z['format'])!r}, expected {WEIGHTS_FORMAT!r}")
+        if "version" in z.files and int(z["version"]) > WEIGHTS_VERSION:
+            raise ValueError(f"{path}: file version {int(z['version'])} is newer than {WEIGHTS_VERSION}")
+        digest = str(z["sha256"]) if "sha256" in z.files else None
+    if p.dtype != np.float32 or p.shape != (WEIGHTS_NPARAM,):
+        raise ValueError(f"{path}: params {p.dtype}{p.shape}, expected float32({WEIGHTS_NPARAM},)")
+    if not np.all(np.isfinite(p)):
+        raise ValueError(f"{path}: non-finite parameters")
+    if digest is not None and digest != params_digest(p):
+        raise ValueError(f"{path}: checksum mismatch (file damaged or edited)")
+    return p
+
+
+def write_weights(path: str, params: np.ndarray, **meta) -> None:
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    if p.shape != (WEIGHTS_NPARAM,):
+        raise ValueError(f"params shape {p.shape}, expected ({WEIGHTS_NPARAM},)")
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=p, format=np.asarray(WEIGHTS_FORMAT), version=np.asarray(WEIGHTS_VERSION),
+             sha256=np.asarray(params_digest(p)), **{k: np.asarray(v) for k, v in meta.items()})
 
 
 @dataclass
@@ -141,12 +185,10 @@ class Refiner:
         return r
 
     def save(self, path: str, **meta):
-        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-        np.savez(path, params=self.get_params(), **{k: np.asarray(v) for k, v in meta.items()})
+        write_weights(path, self.get_params(), **meta)
 
     def load(self, path: str):
-        with np.load(path, allow_pickle=False) as z:
-            self.set_params(z["params"])
+        self.set_params(read_weights(path))
         return self
 
 
