@@ -295,7 +295,9 @@ typedef struct spx_batch_desc {
   float sample_rate;
   float nsigma;
   uint32_t max_peaks;         /* records kept per channel (>= 1) */
-  uint32_t refine;            /* SPX_REFINE_NONE or SPX_REFINE_PARABOLIC */
+  uint32_t refine;            /* SPX_REFINE_NONE, _PARABOLIC or _LEARNED (the refiner set with
+                                 spx_set_refiner when the plan is created; it must stay allocated
+                                 while the plan exists) */
   spx_max_value* peaks_out;   /* device [n_chan][max_peaks] */
   uint32_t* counts_out;       /* device [n_chan][2] = {stored, found} */
   spx_max_value* gmax_out;    /* optional device [n_chan] */

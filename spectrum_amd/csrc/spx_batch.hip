@@ -16,6 +16,7 @@
 // kernel's.  The tile shape and radix order (Plan<L, V>) of each length are
 // chosen by the host autotuner; the order is carried in the item word.
 #include "spx_fused_common.hpp"
+#include "spx_mlp.hpp"
 
 namespace spx {
 
@@ -174,6 +175,10 @@ __device__ __forceinline__ void batch_small_tile(const BatchArgs& b, const uint4
         const float yl = mag_of(lds[LAY::idx(s, (k + L - 1) % L)]);
         const float yr = mag_of(lds[LAY::idx(s, (k + 1) % L)]);
         write_record_hz(out + pos, k, lds[LAY::idx(s, k)], yl, yc, yr, b.refine, bin_hz);
+        if (b.mlp_feat != nullptr)
+          mlp_store_features([&](uint32_t q) { return lds[LAY::idx(s, (int)q)]; }, k, (uint32_t)L,
+                             reinterpret_cast<_Float16*>(b.mlp_feat) +
+                                 ((uint64_t)(first + s - b.feat_base) * b.kcap + (uint32_t)pos) * 16);
         ++pos;
       }
     }

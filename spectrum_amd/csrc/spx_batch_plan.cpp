@@ -55,6 +55,14 @@ struct spx_batch_plan {
   void* d_perm = nullptr;
   void* d_offsets = nullptr;
   void* d_counter = nullptr;   // one 64-byte slot per tile launch
+  // learned refinement of the LDS-tile channels: their feature rows, the
+  // channel lengths (bin widths) and the refiner, run after the tile launches
+  void* d_feat = nullptr;
+  void* d_lens = nullptr;
+  const void* refiner = nullptr;
+  uint32_t first_lds = 0, n_lds = 0;
+  uint32_t kcap = 0;
+  float fs = 1.0f;
   void* d_scratch = nullptr;
   uint64_t scratch_bytes = 0;
   std::vector<SplitGroup> split;
@@ -158,7 +166,9 @@ int check_desc(spx_ctx* c, const spx_batch_desc* d) {
   if (d->n_chan == 0) return fail(c, SPX_EINVAL, "spx_batch: n_chan must be > 0");
   if (d->dtype > SPX_DTYPE_C32BF16) return fail(c, SPX_EINVAL, "spx_batch: bad dtype");
   if (d->window > SPX_WINDOW_BLACKMAN) return fail(c, SPX_EINVAL, "spx_batch: bad window");
-  if (d->refine > SPX_REFINE_PARABOLIC) return fail(c, SPX_ENOTSUP, "spx_batch: refine must be NONE or PARABOLIC");
+  if (d->refine > SPX_REFINE_LEARNED) return fail(c, SPX_EINVAL, "spx_batch: bad refine");
+  if (d->refine == SPX_REFINE_LEARNED && !c->refiner)
+    return fail(c, SPX_EINVAL, "spx_batch: learned refinement requested but no refiner is set (spx_set_refiner)");
   if (d->max_peaks == 0) return fail(c, SPX_EINVAL, "spx_batch: max_peaks must be >= 1");
   if (!(d->nsigma >= 0.0f)) return fail(c, SPX_EINVAL, "spx_batch: nsigma must be >= 0");
   for (uint32_t i = 0; i < d->n_chan; ++i) {
@@ -178,7 +188,7 @@ void release(spx_batch_plan* p) {
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->ev_join) (void)hipEventDestroy(p->ev_join);
   if (p->side) (void)hipStreamDestroy(p->side);
-  for (void* q : {p->d_items, p->d_perm, p->d_offsets, p->d_counter, p->d_scratch})
+  for (void* q : {p->d_items, p->d_perm, p->d_offsets, p->d_counter, p->d_scratch, p->d_feat, p->d_lens})
     if (q) (void)hipFree(q);
   delete p;
 }
@@ -190,12 +200,23 @@ hipError_t launch_all(spx_batch_plan* p, hipStream_t st) {
   const bool conc = p->concurrent && p->side && !p->tl.empty() && !p->split.empty();
   hipStream_t ts = st;
   hipError_t e;
+  // the tile launches, then (learned refinement) the refiner over their channels
+  auto tiles = [&](hipStream_t s) -> hipError_t {
+    for (const TileLaunch& t : p->tl) {
+      hipError_t r = spx_launch_batch(&t.args, t.tile, t.grid, s);
+      if (r != hipSuccess) return r;
+    }
+    if (p->d_feat)
+      return spx_launch_mlp_refine(p->refiner, p->d_feat, p->tl[0].args.peaks, p->tl[0].args.counts, p->n_lds,
+                                   p->kcap, 0.0f, s, (const uint32_t*)p->d_perm + p->first_lds,
+                                   (const uint32_t*)p->d_lens, p->fs);
+    return hipSuccess;
+  };
   if (conc) {
     if ((e = hipEventRecord(p->ev_fork, st)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(p->side, p->ev_fork, 0)) != hipSuccess) return e;
     ts = p->side;
-    for (const TileLaunch& t : p->tl)
-      if ((e = spx_launch_batch(&t.args, t.tile, t.grid, ts)) != hipSuccess) return e;
+    if ((e = tiles(ts)) != hipSuccess) return e;
   }
   for (const SplitGroup& g : p->split) {
     e = spx_launch_fourstep_ex(&g.fa, p->d_scratch, g.chunk, (const uint32_t*)p->d_perm + g.first,
@@ -206,9 +227,7 @@ hipError_t launch_all(spx_batch_plan* p, hipStream_t st) {
     if ((e = hipEventRecord(p->ev_join, p->side)) != hipSuccess) return e;
     return hipStreamWaitEvent(st, p->ev_join, 0);
   }
-  for (const TileLaunch& t : p->tl)
-    if ((e = spx_launch_batch(&t.args, t.tile, t.grid, st)) != hipSuccess) return e;
-  return hipSuccess;
+  return tiles(st);
 }
 
 // Builds a plan.  The autotuner's measurement plans pass `force` (a choice
@@ -220,6 +239,7 @@ int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* forc
   if (rc) return rc;
   spx_batch_plan* p = new spx_batch_plan();
   p->c = c;
+  const bool learn = d->refine == SPX_REFINE_LEARNED;
   if (force) {
     for (int b = 0; b < SPX_BATCH_BUCKETS; ++b) p->ch[b] = sanitize(b, force[b]);
     p->concurrent = force_conc;
@@ -250,7 +270,7 @@ int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* forc
       g.fa.batch = (uint32_t)v.size();
       g.first = (uint32_t)perm.size();
       const uint32_t per_group = (g.fa.batch + (uint32_t)p->ch[b].groups - 1) / (uint32_t)p->ch[b].groups;
-      g.chunk = std::min(fourstep_chunk(L, g.fa.batch), per_group);
+      g.chunk = std::min(fourstep_chunk(L, g.fa.batch, learn ? 1 : 0, d->max_peaks), per_group);
       p->split.push_back(g);
       p->items_per_bucket[b] = (g.fa.batch + g.chunk - 1) / g.chunk;
       p->chunk_per_bucket[b] = g.chunk;
@@ -266,6 +286,8 @@ int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* forc
     }
     perm.insert(perm.end(), v.begin(), v.end());
   }
+  for (const SplitGroup& g : p->split) p->first_lds += g.fa.batch;
+  p->n_lds = (uint32_t)perm.size() - p->first_lds;
   std::vector<uint32_t> all_items;
   std::vector<size_t> item_first;  // first item of each launch in all_items
   for (int ti = 0; ti < 3; ++ti) {
@@ -316,6 +338,18 @@ int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* forc
   a.counts = d->counts_out;
   a.gmax = d->gmax_out;
   a.stats = d->stats_out;
+  if (learn && p->n_lds > 0) {
+    p->refiner = c->refiner;
+    p->kcap = d->max_peaks;
+    p->fs = d->sample_rate;
+    if ((e = hipMalloc(&p->d_feat, (size_t)p->n_lds * d->max_peaks * 32)) != hipSuccess)
+      return fail_hip(e, "hipMalloc(features)");
+    if ((e = hipMalloc(&p->d_lens, (size_t)d->n_chan * 4)) != hipSuccess) return fail_hip(e, "hipMalloc(lengths)");
+    if ((e = hipMemcpy(p->d_lens, d->lengths, (size_t)d->n_chan * 4, hipMemcpyHostToDevice)) != hipSuccess)
+      return fail_hip(e, "upload lengths");
+    a.mlp_feat = p->d_feat;
+    a.feat_base = p->first_lds;
+  }
   const void* t = nullptr;
   for (int b = 0; b < SPX_BATCH_FIRST_LARGE; ++b) {
     if (by[b].empty()) continue;
@@ -352,6 +386,7 @@ int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* forc
     fa.s0 = 0;
     fa.s1 = L;
     fa.refine = d->refine;
+    fa.mlp = learn ? c->refiner : nullptr;
     fa.peaks = d->peaks_out;
     fa.counts = d->counts_out;
     fa.gmax = d->gmax_out;
@@ -362,7 +397,7 @@ int build(spx_ctx* c, const spx_batch_desc* d, uint32_t mask, const Choice* forc
     fa.tw_b = (const float2*)t;
     if ((rc = get_table(c, TW4, L, 0, &t))) { release(p); return rc; }
     fa.tw_4 = (const float2*)t;
-    ws = std::max<uint64_t>(ws, spx_fs_workspace_bytes(L, g.chunk));
+    ws = std::max<uint64_t>(ws, spx_fs_workspace_bytes_ex(L, g.chunk, learn ? 1 : 0, d->max_peaks));
   }
   if ((e = spx_batch_kernel_init()) != hipSuccess) return fail_hip(e, "spx_batch_kernel_init");
   if (p->concurrent && !p->tl.empty() && !p->split.empty()) {

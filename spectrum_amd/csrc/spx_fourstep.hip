@@ -763,8 +763,14 @@ static hipError_t launch_fs(const FusedArgs& a, const FsWork& w0, uint32_t chunk
       hipLaunchKernelGGL((k_fs_finish<N2, TC, true>), dim3(w.nch), dim3(C::TF), 0, st, a, w);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
-      e = spx_launch_mlp_refine(a.mlp, a.mlp_feat, a.peaks + (uint64_t)c0 * a.kcap, a.counts + 2 * (uint64_t)c0,
-                                w.nch, a.kcap, a.fs / (float)a.nfft, st);
+      // feature rows are chunk-local; with channel indirection the records
+      // are the caller's channels chan[c0 + i]
+      if (w.chan != nullptr)
+        e = spx_launch_mlp_refine(a.mlp, a.mlp_feat, a.peaks, a.counts, w.nch, a.kcap, a.fs / (float)a.nfft, st,
+                                  w.chan + c0);
+      else
+        e = spx_launch_mlp_refine(a.mlp, a.mlp_feat, a.peaks + (uint64_t)c0 * a.kcap,
+                                  a.counts + 2 * (uint64_t)c0, w.nch, a.kcap, a.fs / (float)a.nfft, st);
       if (e != hipSuccess) return e;
     } else {
       hipLaunchKernelGGL((k_fs_rows<N2, TC>), dim3(w.nch * C::NT2), dim3(C::TR), 0, st, a, w);

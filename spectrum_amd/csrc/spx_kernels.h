@@ -71,6 +71,10 @@ struct BatchArgs {
   uint32_t* counts;        // [n_chan][2]
   spx_max_value* gmax;     // [n_chan] or null
   float* stats;            // [n_chan][4] or null
+  // learned refinement (refine 2): feature rows of the stored peaks of the
+  // channel at perm index i at mlp_feat + ((i - feat_base) * kcap + pos) * 16 halves
+  void* mlp_feat;
+  uint32_t feat_base;
 };
 
 extern "C" {
@@ -87,8 +91,10 @@ hipError_t spx_launch_mlp_adam(float* params, const float* grad, float* m, float
                                float eps, int step, hipStream_t st);
 hipError_t spx_launch_mlp_export(const float* params, void* dep, hipStream_t st);
 hipError_t spx_launch_mlp_predict(const void* dep, const float* X, uint32_t n, float* out, hipStream_t st);
+// feature row i refines channel chan ? chan[i] : i; its bin width fs / lens[channel] when lens is given
 hipError_t spx_launch_mlp_refine(const void* dep, const void* feat, void* peaks, const uint32_t* counts,
-                                 uint32_t batch, uint32_t kcap, float bin_hz, hipStream_t st);
+                                 uint32_t batch, uint32_t kcap, float bin_hz, hipStream_t st,
+                                 const uint32_t* chan = nullptr, const uint32_t* lens = nullptr, float fs = 0.0f);
 hipError_t spx_launch_mixed(const FusedArgs* a, int compute, int grid, hipStream_t st);
 int spx_batch_tile_ok(int tile);
 hipError_t spx_launch_batch(const BatchArgs* b, int tile, int grid, hipStream_t st);

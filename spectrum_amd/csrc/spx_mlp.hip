@@ -260,20 +260,25 @@ __global__ void __launch_bounds__(256) k_mlp_predict(const unsigned char* dep, c
 // fused kernel wrote (mlp_store_features).  A prediction inside [-0.5, 0.5]
 // replaces the record's parabolic offset and refined frequency; anything
 // else (outside, not finite) keeps them.
+// Feature rows [batch][kcap][16]; row ch refines the records of channel
+// och = chan ? chan[ch] : ch (peaks[och][kcap], counts[och]), whose bin width
+// is fs / lens[och] when lens is given, else bin_hz.
 __global__ void __launch_bounds__(256) k_mlp_refine(const unsigned char* dep, const _Float16* feat,
                                                     MaxValue* peaks, const uint32_t* counts, uint32_t batch,
-                                                    uint32_t kcap, float bin_hz) {
+                                                    uint32_t kcap, float bin_hz, const uint32_t* chan,
+                                                    const uint32_t* lens, float fs) {
   __shared__ __align__(16) _Float16 fbuf[4][16 * 16];
   __shared__ __align__(16) _Float16 hbuf[4][16 * 64];
   __shared__ float res[4][16];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
   const uint32_t ch = blockIdx.x * 4 + wv;
   if (ch >= batch) return;  // whole waves; the kernel has no workgroup barrier
-  const uint32_t n = min(gp(counts)[2 * ch], kcap);
+  const uint64_t och = chan != nullptr ? (uint64_t)gp(chan)[ch] : (uint64_t)ch;
+  const uint32_t n = min(gp(counts)[2 * och], kcap);
   if (n == 0) return;
+  const float bw = lens != nullptr ? fs / (float)gp(lens)[och] : bin_hz;
   const MlpRegs w = mlp_load(dep);
-  mlp_refine_rows(w, feat + (uint64_t)ch * kcap * 16, peaks + (uint64_t)ch * kcap, n, bin_hz, fbuf[wv], hbuf[wv],
-                  res[wv]);
+  mlp_refine_rows(w, feat + (uint64_t)ch * kcap * 16, peaks + och * kcap, n, bw, fbuf[wv], hbuf[wv], res[wv]);
 }
 
 }  // namespace spx
@@ -281,10 +286,11 @@ __global__ void __launch_bounds__(256) k_mlp_refine(const unsigned char* dep, co
 extern "C" {
 
 hipError_t spx_launch_mlp_refine(const void* dep, const void* feat, void* peaks, const uint32_t* counts,
-                                 uint32_t batch, uint32_t kcap, float bin_hz, hipStream_t st) {
+                                 uint32_t batch, uint32_t kcap, float bin_hz, hipStream_t st, const uint32_t* chan,
+                                 const uint32_t* lens, float fs) {
   if (batch == 0 || kcap == 0) return hipSuccess;
   hipLaunchKernelGGL(spx::k_mlp_refine, dim3((batch + 3) / 4), dim3(256), 0, st, (const unsigned char*)dep,
-                     (const _Float16*)feat, (spx::MaxValue*)peaks, counts, batch, kcap, bin_hz);
+                     (const _Float16*)feat, (spx::MaxValue*)peaks, counts, batch, kcap, bin_hz, chan, lens, fs);
   return hipGetLastError();
 }
 
