@@ -120,7 +120,14 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
         const float2 xr = lds[LAY::idx(0, (k0 + E) % L)];
 #pragma unroll
         for (int e = 0; e < E; ++e) m[e] = mag_of(xv[e]);
-        if (full) {
+        if (full && a.gmax == nullptr) {
+          // no global-maximum record asked for: sums only
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            fs1 += m[e];
+            fs2 = fmaf(m[e], m[e], fs2);
+          }
+        } else if (full) {
 #pragma unroll
           for (int e = 0; e < E; ++e) {
             fs1 += m[e];
@@ -161,7 +168,8 @@ __global__ void __launch_bounds__(SmallCfg<L>::T) __attribute__((amdgpu_waves_pe
       s2 = fs2;
       // one channel per workgroup: `red` is written once per field, so
       // neither reduction needs its closing barrier
-      block_stats<T, false>(s1, s2, bm, bk, red);
+      if (a.gmax != nullptr) block_stats<T, false>(s1, s2, bm, bk, red);
+      else block_sums<T, false>(s1, s2, red);
       const double cnt = (double)(a.s1 - a.s0);
       float mu, sd;
       const float thr = threshold_of(a, s1, s2, cnt, mu, sd);

@@ -58,15 +58,27 @@ __device__ __forceinline__ float2 ld_raw(const void* chan_base, uint32_t i) {
 template <int DT, int R, bool PAD, bool STREAM = false>
 __device__ __forceinline__ void load_run_t(const FusedArgs& a, uint64_t ch, uint32_t n0, uint32_t stride, float2* v) {
   constexpr int ES = DT == SPX_DTYPE_C64 ? 8 : 4;
-  const void* cb = reinterpret_cast<const char*>(a.in) + ch * (uint64_t)a.n_point * ES;
+  const char* cb = reinterpret_cast<const char*>(a.in) + ch * (uint64_t)a.n_point * ES;
   const uint32_t last = a.n_point - 1;
   float2 raw[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) raw[r] = ld_raw<DT, STREAM>(cb, PAD ? min(n0 + r * stride, last) : n0 + r * stride);
   float wv[R];
-  if (a.window != nullptr) {
+  if constexpr (!PAD) {
+    // load r of every thread at the uniform base cb + r * stride * ES: one
+    // offset VGPR (n0) for all R loads, the r steps in scalar registers
 #pragma unroll
-    for (int r = 0; r < R; ++r) wv[r] = ld_at<float>(a.window, 4u * (PAD ? min(n0 + r * stride, last) : n0 + r * stride));
+    for (int r = 0; r < R; ++r) raw[r] = ld_raw<DT, STREAM>(cb + (uint64_t)r * stride * ES, n0);
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) raw[r] = ld_raw<DT, STREAM>(cb, min(n0 + r * stride, last));
+  }
+  if (a.window != nullptr) {
+    if constexpr (!PAD) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) wv[r] = ld_at<float>(reinterpret_cast<const char*>(a.window) + 4u * r * stride, 4u * n0);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) wv[r] = ld_at<float>(a.window, 4u * min(n0 + r * stride, last));
+    }
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) wv[r] = 1.0f;
@@ -337,6 +349,36 @@ __device__ __forceinline__ void block_stats(double& s1, double& s2, float& bm, u
   s2 = rowred::lane_of(a2, NW - 1);
   bm = rowred::lane_of(m, NW - 1);
   bk = rowred::lane_of(k, NW - 1);
+  if constexpr (REUSE) __syncthreads();
+}
+
+// block_stats without the arg-max: block-wide sums of two doubles, valid in
+// all threads on return.
+template <int T, bool REUSE = true>
+__device__ __forceinline__ void block_sums(double& s1, double& s2, BlockRed& red) {
+  constexpr int NW = (T + 63) / 64;
+  static_assert(NW <= 16, "at most 16 waves");
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if (lane == 0) {
+    red.s1[w] = s1;
+    red.s2[w] = s2;
+  }
+  __syncthreads();
+  const bool in = lane < NW;
+  double a1 = in ? red.s1[lane] : 0.0, a2 = in ? red.s2[lane] : 0.0;
+  auto step = [&](auto ctrl) {
+    constexpr int C = decltype(ctrl)::value;
+    a1 += rowred::shr<C>(0.0, a1);
+    a2 += rowred::shr<C>(0.0, a2);
+  };
+  if constexpr (NW > 1) step(std::integral_constant<int, dpp::kShr1>{});
+  if constexpr (NW > 2) step(std::integral_constant<int, dpp::kShr2>{});
+  if constexpr (NW > 4) step(std::integral_constant<int, dpp::kShr4>{});
+  if constexpr (NW > 8) step(std::integral_constant<int, dpp::kShr8>{});
+  s1 = rowred::lane_of(a1, NW - 1);
+  s2 = rowred::lane_of(a2, NW - 1);
   if constexpr (REUSE) __syncthreads();
 }
 

@@ -168,7 +168,16 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
     for (int e = 0; e < E; ++e) m[e] = mag_of(make_float2(xre[adr(k0) + e], xim[adr(k0) + e]));
     const float ml = mag_of(xat((k0 + N - 1) % N));
     const float mr = mag_of(xat((k0 + E) % N));
-    if (full) {
+    if (full && a.gmax == nullptr) {
+      // no global-maximum record asked for: sums and maxima only
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float yl = e == 0 ? ml : m[e - 1], yr = e == E - 1 ? mr : m[e + 1];
+        fs1 += m[e];
+        fs2 = fmaf(m[e], m[e], fs2);
+        lmax |= (m[e] > yl && m[e] > yr && m[e] > lb) ? 1u << e : 0u;
+      }
+    } else if (full) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const float yl = e == 0 ? ml : m[e - 1], yr = e == E - 1 ? mr : m[e + 1];
@@ -197,7 +206,8 @@ __device__ __forceinline__ void mixed_peaks(const FusedArgs& a, uint32_t ch, con
   double s1 = fs1, s2 = fs2;
   // the scan's first barrier follows every read of the sums, and the caller's
   // barrier after the peaks every read of the scan totals
-  block_stats<T, false>(s1, s2, bm, bk, red);
+  if (a.gmax != nullptr) block_stats<T, false>(s1, s2, bm, bk, red);
+  else block_sums<T, false>(s1, s2, red);
   const double cnt = (double)(a.s1 - a.s0);
   float mu, sd;
   const float thr = threshold_of(a, s1, s2, cnt, mu, sd);
