@@ -108,8 +108,9 @@ __device__ __forceinline__ double fs_energy(const FsWork& w, uint32_t ci) {
 // the LDS exchange done in halves through a float tile of 5104 entries.
 // Element e = 16 a + b of column s is at 320 a + 16 b + s + 16 (a & 3) (rows
 // of 320 floats, so the skew never reaches the next row): the pass-0 writes
-// (a = jj, b = r) and the pass-1 reads (a = r, b = jj) of a wave both fall on
-// 64 distinct banks.
+// (a = jj, b = r) and the pass-1 reads (a = r, b = jj) of each half-wave both
+// fall on 32 distinct banks (4-byte accesses go to 32 banks, a half-wave per
+// cycle).
 template <bool TWL>
 __device__ __forceinline__ void fft256_cols16_halfx(float* lf, float2 (&v)[16], const float2* tw, int tid) {
   using PL = Plan<256, 0>;

@@ -45,7 +45,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 namespace mx {
 constexpr int N = 16384;
 constexpr int T = 512;             // 8 waves
-constexpr int PSTR = 36;           // stage-2 operand row stride in complex (144 B: conflict-free stage-1 stores)
+// stage-2 operand row stride in half2 words (144 B: 16-byte aligned rows for
+// the operand reads; the stage-1 stores, one word per lane, then meet 4-way
+// bank conflicts, which measured cheaper than a conflict-free layout read
+// one word at a time)
+constexpr int PSTR = 36;
 constexpr int PING_BYTES = 512 * PSTR * 4;     // [k1][m2] rows of 32 (+4) half2 {re, im}
 // [j1][k1] rows of 32 (+4) half2, row r shifted by 16 (r >> 6) words
 constexpr int PONG_BYTES = (512 * PSTR + 112) * 4;
