@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
 #pragma unroll
     for (int r = 0; r < PPT; ++r) cxt[LAY2::idx(s_row, jj_row + 16 * r)] = v[r];
     const int er = s_row < 2 ? s_row : (s_row >= RB - 2 ? s_row - (RB - 4) : -1);
-    if (er >= 0) {
+    if (er >= 0 && w.edgex != nullptr) {
       float2* ex = w.edgex + (((uint64_t)ci * NT2 + t) * 4 + er) * N1;
 #pragma unroll
       for (int r = 0; r < PPT; ++r) gp(ex)[jj_row + 16 * r] = v[r];
@@ -296,6 +296,16 @@ __global__ void __launch_bounds__(256) k_fs_rows(FusedArgs a, FsWork w) {
 
 // ---------------------------------------------------------------------------
 
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+
+// A refiner feature pair still to be filled: the neighbour of a stored
+// record (feature row `row`, pair `slot`) that lies in another row block.
+struct FsNbrReq {
+  uint32_t row, k, slot;
+  float cr, ci;
+};
+constexpr int FS_NBR_CAP = 128;  // two per record, records 0 .. 63
+
 template <int N2, int TC>
 struct FsFinishShared {
   static constexpr int NW = FsCfg<N2, TC>::N / 32;  // survivor bitmap words
@@ -303,7 +313,46 @@ struct FsFinishShared {
   uint16_t pre[NW];
   int base[FsCfg<N2, TC>::TF];
   BlockRed red;
+  FsNbrReq req[FS_NBR_CAP];
+  uint32_t nreq;
 };
+
+// X at bins kb[0..G) of the channel whose four-step intermediate is z
+// (Z[k2][n1], X[k2 + N2 k1] = sum_n1 Z[k2][n1] W_256^(n1 k1)), the 256 terms
+// of each shared by the wave's 64 lanes (all loads issued first); the
+// results in every lane.  The twiddles come from the hardware sine and
+// cosine (in revolutions): a refiner feature is rounded to fp16, so their
+// ~1e-6 error does not matter there.
+template <int N2, int G>
+__device__ __forceinline__ void fs_bins_from_z(const float2* z, const uint32_t (&kb)[G], float2 (&out)[G]) {
+  const int l = threadIdx.x & 63;
+  float2 v[G][4];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const float2* row = z + (uint64_t)(kb[g] & (N2 - 1)) * 256;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[g][q] = gp(row)[(uint32_t)l + 64u * q];
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const uint32_t k1 = kb[g] / N2;
+    float re = 0.f, im = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t n1 = (uint32_t)l + 64u * q;
+      const float t = (float)((n1 * k1) & 255u) * (1.0f / 256.0f);  // revolutions
+      const float c = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
+      // v * (c - i sn)
+      re = fmaf(v[g][q].x, c, fmaf(v[g][q].y, sn, re));
+      im = fmaf(v[g][q].y, c, fmaf(-v[g][q].x, sn, im));
+    }
+    auto add = [](float a, float b) { return a + b; };
+    re = dpp::reduce63(re, add);
+    im = dpp::reduce63(im, add);
+    out[g] = make_float2(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, re), 63)),
+                         __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, im), 63)));
+  }
+}
 
 // The finish of one channel, one wave (TF = 64 threads) per channel: sums and
 // best bin of the row blocks (one block per lane), threshold, then the first
@@ -333,6 +382,9 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
   __shared__ FsFinishShared<N2, TC> sh;
   const int lane = threadIdx.x;
   const uint32_t ci = blockIdx.x;
+  if constexpr (LEARN) {
+    if (lane == 0) sh.nreq = 0u;
+  }
   const uint64_t och = w.chan != nullptr ? (uint64_t)gp(w.chan)[w.ch0 + ci] : (uint64_t)w.ch0 + ci;
   const Cand* cand = w.cand + (uint64_t)ci * C::CAND_CAP;
   const float* edge = w.edge + (uint64_t)ci * NT2 * 2 * C::N1;
@@ -399,12 +451,56 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
       for (int j = 0; j < 4; ++j) {
         const int d = j < 2 ? j - 2 : j - 1;
         float2 v = gp(cx4)[j];
-        if (__builtin_isnan(v.x)) v = fs_outer_x<N2, C::RB>(ex, q.k, d);
+        if (__builtin_isnan(v.x)) {
+          if (w.edgex != nullptr) {
+            v = fs_outer_x<N2, C::RB>(ex, q.k, d);
+          } else {
+            // in another row block: its feature pair is filled after the
+            // records, from Z (the pair is written as zero meanwhile)
+            const uint32_t s = atomicAdd(&sh.nreq, 1u);
+            if (s < (uint32_t)FS_NBR_CAP)
+              sh.req[s] = FsNbrReq{rank, (q.k + (uint32_t)C::N + (uint32_t)d) % (uint32_t)C::N, (uint32_t)(d + 2),
+                                   q.re, q.im};
+            v = make_float2(0.f, 0.f);
+          }
+        }
         x[d + 2] = v;
       }
       const uint32_t k0 = q.k;
       mlp_store_features([&](uint32_t qq) { return x[(qq + (uint32_t)C::N + 2u - k0) % (uint32_t)C::N]; }, q.k,
                          (uint32_t)C::N, mlp_feat_row(a, ci, (int)rank));
+    }
+  };
+  // the feature pairs of neighbours in other row blocks, one bin at a time
+  // by the whole wave (at most two per record; records 0 .. 63 without edge
+  // rows, the host's choice whenever max_peaks <= 64)
+  auto fill_requests = [&]() {
+    if constexpr (LEARN) {
+      if (w.edgex != nullptr) return;
+      __builtin_amdgcn_s_waitcnt(0);  // the wave's LDS appends are done
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t n = min(sh.nreq, (uint32_t)FS_NBR_CAP);
+      const float2* z = w.Z + (uint64_t)ci * C::N;
+      constexpr int G = 4;  // bins per round, their loads in flight together
+      for (uint32_t i0 = 0; i0 < n; i0 += G) {
+        uint32_t kb[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) kb[g] = sh.req[min(i0 + g, n - 1)].k;
+        float2 xv[G];
+        fs_bins_from_z<N2, G>(z, kb, xv);
+        if (lane < G && i0 + (uint32_t)lane < n) {
+          const FsNbrReq r = sh.req[i0 + lane];
+          float2 x = xv[0];
+#pragma unroll
+          for (int g = 1; g < G; ++g) x = lane == g ? xv[g] : x;
+          // x conj(c) / |c|^2, as mlp_features computes it
+          const float den = fmaf(r.cr, r.cr, r.ci * r.ci);
+          const float inv = den > 0.f ? 1.0f / den : 0.f;
+          const f16x2_t h = {(_Float16)(fmaf(x.x, r.cr, x.y * r.ci) * inv),
+                             (_Float16)(fmaf(x.y, r.cr, -x.x * r.ci) * inv)};
+          *gp(reinterpret_cast<f16x2_t*>(mlp_feat_row(a, ci, (int)r.row) + 2 * r.slot)) = h;
+        }
+      }
     }
   };
   if (nc <= 64u) {
@@ -424,6 +520,7 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
       write_record(a.peaks + och * a.kcap + rank, c.k, make_float2(c.re, c.im), c.ml, c.mc, c.mr, a);
       features((uint32_t)lane, (uint32_t)rank, c);
     }
+    fill_requests();
     if (lane == 0) {
       if (a.counts) {
         gp(a.counts)[2 * och + 0] = (uint32_t)min(total, kcap);
@@ -442,6 +539,7 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
       resolve(q);
       return q;
     }, nc, thr, mu, sd, P, och, sh.bits, sh.pre, sh.base, sh.red, features);
+    fill_requests();
   }
 }
 
@@ -794,7 +892,8 @@ static uint64_t fs_channel_bytes(bool learn, uint32_t kcap) {
   auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
   uint64_t b = r((uint64_t)C::N * 8) + r(C::CAND_CAP * sizeof(Cand)) + 256 + r((uint64_t)C::NT1 * 8) +
                r((uint64_t)C::NT2 * sizeof(FsRowPart)) + r((uint64_t)C::NT2 * 2 * 256 * 4);
-  if (learn) b += r(C::CAND_CAP * 4 * 8) + r((uint64_t)C::NT2 * 4 * 256 * 8) + (uint64_t)kcap * 32;
+  if (learn)
+    b += r(C::CAND_CAP * 4 * 8) + (kcap > 64 ? r((uint64_t)C::NT2 * 4 * 256 * 8) : 0) + (uint64_t)kcap * 32;
   return b;
 }
 
@@ -816,7 +915,10 @@ static hipError_t launch_fs_n(const FusedArgs& a, void* ws, uint32_t chunk, cons
   FusedArgs al = a;
   if (fs_learn(a)) {
     w.candx = reinterpret_cast<float2*>(p);      p += r((uint64_t)chunk * C::CAND_CAP * 4 * 8);
-    w.edgex = reinterpret_cast<float2*>(p);      p += r((uint64_t)chunk * C::NT2 * 4 * 256 * 8);
+    if (a.kcap > 64) {
+      // more than 64 records: the outer neighbours come from the edge rows
+      w.edgex = reinterpret_cast<float2*>(p);    p += r((uint64_t)chunk * C::NT2 * 4 * 256 * 8);
+    }
     al.mlp_feat = p;  // chunk feature rows: [chunk][kcap][16 halves]
   }
   w.ch0 = 0;

@@ -22,9 +22,11 @@ struct FsWork {
   FsRowPart* part;    // [nch][NT2]
   float* edge;        // [nch][NT2][2][256]  |X| of block rows 0 and RB-1
   // learned refinement only: X at bins k-2, k-1, k+1, k+2 of every candidate
-  // (NaN: in another row block) and X of block rows 0, 1, RB-2, RB-1
+  // (NaN: in another row block) and, when records past the 64th are kept,
+  // X of block rows 0, 1, RB-2, RB-1 (otherwise a neighbour in another block
+  // is recomputed from Z in the finish)
   float2* candx;      // [nch][CAND_CAP][4]
-  float2* edgex;      // [nch][NT2][4][256]
+  float2* edgex;      // [nch][NT2][4][256] or null
   uint32_t ch0, nch;  // chunk: channels ch0 .. ch0+nch-1 of the launch
   // optional indirection (uneven batches): channel i of the launch is channel
   // chan[i] of the caller, whose input starts inoff[chan[i]] elements into a.in
