@@ -306,14 +306,14 @@ struct FsNbrReq {
 };
 constexpr int FS_NBR_CAP = 128;  // two per record, records 0 .. 63
 
-template <int N2, int TC>
+template <int N2, int TC, bool LEARN>
 struct FsFinishShared {
   static constexpr int NW = FsCfg<N2, TC>::N / 32;  // survivor bitmap words
   uint32_t bits[NW];
   uint16_t pre[NW];
   int base[FsCfg<N2, TC>::TF];
   BlockRed red;
-  FsNbrReq req[FS_NBR_CAP];
+  FsNbrReq req[LEARN ? FS_NBR_CAP : 1];  // learned refinement only
   uint32_t nreq;
 };
 
@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
   using C = FsCfg<N2, TC>;
   constexpr int NT1 = C::NT1, NT2 = C::NT2, T = C::TF;
   static_assert(T == 64 && NT1 <= 64 && NT2 <= 64 && C::CAND_CAP >= 64, "one value per lane");
-  __shared__ FsFinishShared<N2, TC> sh;
+  __shared__ FsFinishShared<N2, TC, LEARN> sh;
   const int lane = threadIdx.x;
   const uint32_t ci = blockIdx.x;
   if constexpr (LEARN) {
@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(64) k_fs_finish(FusedArgs a, FsWork w) {
       }
     }
   } else {
-    select_first_k<T, FsFinishShared<N2, TC>::NW>(a, [at, resolve](uint32_t i) {
+    select_first_k<T, FsFinishShared<N2, TC, LEARN>::NW>(a, [at, resolve](uint32_t i) {
       Cand q = at(i);
       resolve(q);
       return q;
