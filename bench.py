@@ -15,8 +15,10 @@ pocketfft CpuFFT, compiled from the reference sources into oracle/_ref) plus
 the oracle's C peak rules, timed on a bounded sample on this host (one
 thread, plus a multi-core variant).  Extra objects on the same line (skipped
 with --no-extras):
+  small_batch  64 x 4096, three tones (the CPU-runnable shape): time per
+             call and the reference CPU path on the same 64 channels;
   uneven     4096 channels of lengths 2^10..2^17 in one hipGraph launch vs
-             per-length launches;
+             per-length launches (autotuned plan and the default one);
   mixed_mlp  2048 x 16384: the FP16-MFMA + learned-refinement form against
              the plain FP32 form (complex64, parabolic), with its spectral
              error against FP64;
@@ -439,6 +441,37 @@ def extra_refiner_split(ctx, K=64, reps=20):
     return out
 
 
+def extra_small_batch(ctx, K=32, reps=100):
+    """64 x 4096 complex64, three tones at 10 dB (the CPU-runnable shape):
+    one fused launch per call, latency-bound; beside it the reference CPU path
+    (pocketfft float via the reference CpuFFT, one thread, the same peak rules)
+    on all 64 channels, 1 warm-up run, median of 5."""
+    from spectrum_amd import capi
+    from oracle import oracle as orc
+
+    B, N = 64, 4096
+    x = capi.siggen(capi.GEN_THREE_TONES, 0x5EED0001, B, N)
+    d = ctx.to_device(x)
+    pk, ct, gm = ctx.alloc(B * K * 32), ctx.alloc(B * 8), ctx.alloc(B * 32)
+    ms = _timer_ms(ctx, lambda: ctx.fused_peaks(d, B, N, N, pk, ct, max_peaks=K, gmax=gm), reps)
+    out = {"workload": f"{B} x {N} complex64, 3 tones at 10 dB, Hann, mean+6std, K={K}, global max",
+           "ms": ms, "channels_per_s": B / (ms * 1e-3),
+           "achieved_GBps": B * (8 * N + K * 32 + 8 + 32) / (ms * 1e-3) / 1e9}
+    w = orc.window(1, N).astype(np.float32)
+    kind = "reference" if orc.have_reference() else "port"
+
+    def cpu():
+        for b in range(B):
+            xw = (x[b] * w).astype(np.complex64)[None]
+            X = orc.ref_cpufft(xw)[0] if kind == "reference" else orc.spectrum64(xw, N, 0)[0].astype(np.complex64)
+            orc.fused_peaks(X, 1.0, nsigma=6.0, kcap=K)
+
+    med, ts = _median_runs(cpu, 5)
+    out["cpu_baseline"] = {"ms": med * 1e3, "channels_per_s": B / med, "cores": 1, "kind": kind}
+    out["gpu_vs_cpu"] = med * 1e3 / ms
+    return out
+
+
 def extra_top_peaks(ctx, reps=20):
     """The reference ONE_PEAK / TWO_PEAKS search of SpectrumProcessorROCm at
     1024 x 65536 complex64 (search range nfft/4): inside the transform
@@ -481,7 +514,8 @@ def main():
     ap.add_argument("--kpeaks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extra", choices=["host_stream", "uneven", "mixed_mlp", "refiner", "refiner_split", "top_peaks"],
+    ap.add_argument("--extra", choices=["host_stream", "uneven", "mixed_mlp", "refiner", "refiner_split", "top_peaks",
+                                        "small_batch"],
                     help="run only this extra workload (no headline, no CPU baseline) and print its JSON; "
                          "for profiling one workload")
     ap.add_argument("--reps", type=int, default=30, help="timed repetitions of each --extra form")
@@ -505,7 +539,7 @@ def main():
             res = extra_mixed_mlp(ctx, reps=args.reps)
         else:
             res = {"uneven": extra_uneven, "refiner": extra_refiner, "refiner_split": extra_refiner_split,
-                   "top_peaks": extra_top_peaks}[args.extra](ctx)
+                   "top_peaks": extra_top_peaks, "small_batch": extra_small_batch}[args.extra](ctx)
         print(json.dumps({args.extra: res}), flush=True)
         return
     info = ctx.info()
@@ -585,8 +619,9 @@ def main():
         except Exception as e:
             out["host_stream"] = {"error": repr(e)}
         del d_in, x
-        for name, fn in (("uneven", extra_uneven), ("mixed_mlp", extra_mixed_mlp), ("refiner", extra_refiner),
-                         ("refiner_split", extra_refiner_split), ("top_peaks", extra_top_peaks)):
+        for name, fn in (("small_batch", extra_small_batch), ("uneven", extra_uneven), ("mixed_mlp", extra_mixed_mlp),
+                         ("refiner", extra_refiner), ("refiner_split", extra_refiner_split),
+                         ("top_peaks", extra_top_peaks)):
             print(f"[bench] {name} ...", file=sys.stderr, flush=True)
             try:
                 out[name] = fn(ctx)
