@@ -174,6 +174,11 @@ int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out) {
   }
   else if (kind == TW4) host = make_fourstep(a);
   else if (kind == MIX) host = make_mixed_tables(a);
+  else if (kind == LG) {
+    host.resize(spx_large_tables(a, nullptr));
+    if (host.empty()) return fail(c, SPX_EINVAL, "no large-length plan for this nfft");
+    spx_large_tables(a, host.data());
+  }
   else host = make_window(a, b);
   void* d = nullptr;
   SPX_HIP(c, hipSetDevice(c->device));
@@ -271,16 +276,22 @@ int run_fused(spx_ctx* c, FusedArgs& a) {
   if (a.batch == 0) return SPX_OK;
   int grid = 0, rc;
   if (spx_large_is(a.nfft)) {
-    // 2^19 .. 2^24: global-memory Stockham passes, peak search on the stored spectrum
-    const void* t;
-    if ((rc = get_table(c, TW, a.nfft, 0, &t))) return rc;
+    // 2^19 .. 2^24: three Stockham passes through global memory (two of them
+    // L-point transforms in LDS tiles), peak search on the stored spectrum
+    int r0, l1, l2;
+    if (spx_large_plan(a.nfft, &r0, &l1, &l2)) return fail(c, SPX_EINVAL, "no large-length plan for this nfft");
+    const void *t, *t1, *t2;
+    if ((rc = get_table(c, LG, a.nfft, 0, &t))) return rc;
+    if ((rc = get_table(c, TW, 1u << l1, 0, &t1))) return rc;
+    if ((rc = get_table(c, TW, 1u << l2, 0, &t2))) return rc;
     const uint64_t per = spx_large_workspace_bytes(a.nfft, 1);
     const uint64_t budget = c->ws_fixed ? c->ws_fixed_bytes : (2ull << 30);
     uint32_t chunk = (uint32_t)std::min<uint64_t>(a.batch, std::max<uint64_t>(c->ws_fixed ? 0 : 1, budget / per));
     if (chunk == 0) return ensure_scratch(c, per);  // the fixed workspace is too small: reports the shortfall
     if ((rc = ensure_scratch(c, spx_large_workspace_bytes(a.nfft, chunk)))) return rc;
     SPX_HIP(c, hipSetDevice(c->device));
-    SPX_HIP(c, spx_launch_large(&a, (const float2*)t, c->ws_fixed ? c->ws_fixed : c->scratch, chunk, c->stream));
+    SPX_HIP(c, spx_launch_large(&a, (const float2*)t, (const float2*)t1, (const float2*)t2,
+                                c->ws_fixed ? c->ws_fixed : c->scratch, chunk, c->stream));
     return SPX_OK;
   }
   if (spx_fused_is_large(a.nfft)) {

@@ -40,15 +40,16 @@ struct spx_ctx {
 
 namespace spx_host {
 
-enum TableKind { TW = 0, TW4 = 1, WIN = 2, MIX = 3 };
+enum TableKind { TW = 0, TW4 = 1, WIN = 2, MIX = 3, LG = 4 };
 
 int fail(spx_ctx* c, int code, const std::string& msg);
 int hip_fail(spx_ctx* c, hipError_t e, const char* what);
 // SPX_ENOMEM after a failed device or pinned allocation
 int oom(spx_ctx* c, const char* what);
-// Device table of kind TW (a = length, b = radix order), TW4 (a = length) or
-// WIN (a = window type, b = length) or MIX (a = compute type), generated in FP64
-// once per context.
+// Device table of kind TW (a = length, b = radix order), TW4 (a = length),
+// WIN (a = window type, b = length), MIX (a = compute type) or LG (a = length
+// 2^19 .. 2^24: the Stockham tables of its LDS passes), generated in FP64 once
+// per context.
 int get_table(spx_ctx* c, int kind, uint32_t a, uint32_t b, const void** out);
 int window_ptr(spx_ctx* c, uint32_t window, uint32_t n_point, const float** w);
 // Releases the host-streaming buffers, streams and events.

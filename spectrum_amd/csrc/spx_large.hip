@@ -1,22 +1,32 @@
 // Transforms and peak searches of lengths N = 2^19 .. 2^24 (beyond the
 // split four-step path, whose row and column lengths stop at 256 x 1024).
 //
-// The transform is a Stockham FFT over global memory: ceil(log2 N / 4)
-// passes, radix 16 except a leftover radix (2, 4 or 8) in the last pass, one
-// butterfly per thread, the window, zero padding and (16-bit storage) the
-// per-channel load scale applied as the first pass reads the input, and the
-// dense output kinds (spectrum, |X|, |X|^2 scaled,
-// {|X|, arg}) written by the last pass.  Between passes the data go through a
-// ping-pong workspace, so a transform moves 16 B per sample per pass: these
-// lengths are served for completeness (the reference plans any power of two
-// with hipFFT), not at the rate of the fused lengths.
+// The transform is a Stockham FFT over global memory in three passes: a
+// register pass of radix R0 = 2^r0 (one butterfly per thread; it applies the
+// window, zero padding and, for 16-bit storage, the per-channel load scale
+// as it reads the input), then two passes of radix L1 and L2 (256, 512 or
+// 1024 points, N = R0 L1 L2) whose butterflies are whole L-point transforms
+// in an LDS tile: a workgroup takes 16 consecutive butterflies j (8 in the
+// 1024-point passes; the columns of the tile, so every load and store moves
+// runs of 16 or 8 consecutive complex values), multiplies element e of butterfly j by the Stockham
+// twiddle W_(NS L)^(e (j mod NS)), transforms the 16 columns with the
+// LDS-tile FFT of the split path's column kernel and stores element q at
+// (j div NS) NS L + (j mod NS) + q NS.  The last pass writes the dense output
+// kinds (spectrum, |X|, |X|^2 scaled, {|X|, arg}).  Between passes the data
+// go through a ping-pong workspace: 16 B per sample per pass, three passes.
+// A thread holds the elements e = jj + r L/16 (r < 16) of one column, so the
+// twiddle splits as W_(NS L)^(jj k) W_(16 NS)^(r k) (k = j mod NS): the first
+// factor from a [jj][k] table, the second from the powers 1, 2, 3, 4, 8 and
+// 12 of W_(16 NS)^k (an [i][k] table) and their products; both tables are
+// laid out so that the columns of a workgroup read consecutive entries.
 //
 // The peak searches (spx_fused_peaks at these lengths) run on the stored
-// spectrum with the same rules and records as the fused kernels, in four
-// launches that each spread a channel over many workgroups except the two
-// per-channel steps that touch a few values:
-//   k_ls_stats   blocks of 16384 bins: sums of |X| and |X|^2 over the search
-//                range and the largest |X| (lowest bin on ties)
+// spectrum with the same rules and records as the fused kernels: the
+// transform's last pass also writes per-workgroup statistics, then three
+// launches, the candidate search spread over many workgroups per channel and
+// the two per-channel steps touching a few values:
+//   (last pass)  per workgroup: sums of |X| and |X|^2 over the search range
+//                and the largest |X| (lowest bin on ties)
 //   k_ls_thresh  one block per channel: statistics, threshold, the
 //                global-maximum record and the stats output
 //   k_ls_cand    blocks of 16384 bins: strict local maxima (circular
@@ -26,26 +36,32 @@
 //                the first K bins overall as records, in ascending order
 #include "spx_fused_common.hpp"
 
+#include <cmath>
+
 namespace spx {
 
 struct GsArgs {
-  FusedArgs a;             // input, window, n_point, dense outputs (first / last pass)
-  const float2* src;       // [nch][N] (middle and last passes)
-  float2* dst;             // [nch][N] (first and middle passes)
-  const float2* tw;        // this pass's table [k][r] = W_(NS*R)^(r*k), or null when NS = 1
-  uint32_t N, NS;
+  FusedArgs a;             // input, window, n_point
+  float2* dst;             // [nch][N]
+  uint32_t N;
   uint64_t ch0;            // first caller channel of the chunk
 };
 
-// FIRST: read the caller's input (window, zero padding); LAST: write the dense outputs.
-template <int R, bool FIRST, bool LAST>
-__global__ void __launch_bounds__(256) k_gs_pass(GsArgs g) {
+// First pass, radix R (sub-transform length 1, no twiddles): butterfly j
+// reads the caller's input x[j + r N/R] (window, zero padding, load scale)
+// and its outputs go to y[j R + q].  The 256 butterflies of a workgroup
+// cover one contiguous run of 256 R outputs, stored through LDS so that
+// consecutive lanes write consecutive values.
+template <int R>
+__global__ void __launch_bounds__(256) k_gs_first(GsArgs g) {
+  constexpr int T = 256, E = T * R;
+  __shared__ float2 xs[E + E / 16];  // element i at i + i/16
   const uint32_t N = g.N, NB = N / R;
-  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= NB) return;
+  const int tid = threadIdx.x;
+  const uint32_t j = blockIdx.x * T + (uint32_t)tid;  // NB is a multiple of T
   const uint64_t c = blockIdx.y;
   float2 v[R];
-  if constexpr (FIRST) {
+  {
     const FusedArgs& a = g.a;
     const uint64_t ch = g.ch0 + c;
     const uint32_t es = a.dtype == SPX_DTYPE_C64 ? 8 : 4;
@@ -56,89 +72,166 @@ __global__ void __launch_bounds__(256) k_gs_pass(GsArgs g) {
       const uint32_t n = j + (uint32_t)r * NB;
       const uint32_t ni = min(n, a.n_point - 1);
       const float w = a.window != nullptr ? gp(a.window)[ni] : 1.0f;
-      const float2 x = a.dtype == SPX_DTYPE_C64 ? ld_raw<SPX_DTYPE_C64>(in, ni)
-                       : a.dtype == SPX_DTYPE_C32F16 ? ld_raw<SPX_DTYPE_C32F16>(in, ni)
-                                                     : ld_raw<SPX_DTYPE_C32BF16>(in, ni);
+      const float2 x = a.dtype == SPX_DTYPE_C64 ? ld_raw<SPX_DTYPE_C64, true>(in, ni)
+                       : a.dtype == SPX_DTYPE_C32F16 ? ld_raw<SPX_DTYPE_C32F16, true>(in, ni)
+                                                     : ld_raw<SPX_DTYPE_C32BF16, true>(in, ni);
       v[r] = c_scale(x, n < a.n_point ? w * sc : 0.0f);
     }
-  } else {
-    const float2* src = g.src + c * (uint64_t)N;
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = gp(src)[j + (uint32_t)r * NB];
-  }
-  const uint32_t NS = g.NS;
-  const uint32_t k = j % NS;
-  if (NS > 1) {
-    float2 w[R];
-    load_tw_row<R, false>(g.tw + (uint64_t)k * R, w);
-#pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = c_mul(v[r], w[r]);
   }
   Dft<R>::run(v);
-  const uint32_t d0 = (j / NS) * NS * R + k;
-  if constexpr (LAST) {
-    const uint64_t base = (g.ch0 + c) * (uint64_t)N;
+  // y[j R + q] = v[q]: local element tid R + q
 #pragma unroll
-    for (int r = 0; r < R; ++r) store_dense(g.a, base + d0 + (uint32_t)r * NS, v[r]);
-  } else {
-    float2* dst = g.dst + c * (uint64_t)N;
+  for (int q = 0; q < R; ++q) {
+    const int i = tid * R + q;
+    xs[i + (i >> 4)] = v[q];
+  }
+  __syncthreads();
+  float2* dst = g.dst + c * (uint64_t)N + (uint64_t)blockIdx.x * E;
 #pragma unroll
-    for (int r = 0; r < R; ++r) gp(dst)[d0 + (uint32_t)r * NS] = v[r];
+  for (int m = 0; m < R; ++m) {
+    const int i = tid + m * T;
+    gp(dst)[i] = xs[i + (i >> 4)];
   }
 }
 
 template <int R>
-static void launch_gs(const GsArgs& g, bool first, bool last, uint32_t nch, hipStream_t st) {
-  const dim3 grid((g.N / R + 255) / 256, nch);
-  if (first) hipLaunchKernelGGL((k_gs_pass<R, true, false>), grid, dim3(256), 0, st, g);
-  else if (last) hipLaunchKernelGGL((k_gs_pass<R, false, true>), grid, dim3(256), 0, st, g);
-  else hipLaunchKernelGGL((k_gs_pass<R, false, false>), grid, dim3(256), 0, st, g);
+static void launch_gs_first(const GsArgs& g, uint32_t nch, hipStream_t st) {
+  const dim3 grid(g.N / R / 256, nch);
+  hipLaunchKernelGGL((k_gs_first<R>), grid, dim3(256), 0, st, g);
 }
 
 // ---------------------------------------------------------------------------
-// peak search on a stored spectrum
+// Stockham pass of radix L through an LDS tile
 // ---------------------------------------------------------------------------
+// Partial statistics of a block of bins for the threshold search.
 struct LsPart {
   double s1, s2;  // sum |X|, sum |X|^2 over the block's in-range bins
   float m;        // largest |X|, -1 if none
   uint32_t k;     // its bin (lowest on ties)
 };
 
-constexpr int LS_T = 256;
-constexpr int LS_BPT = 64;                    // bins per thread in k_ls_stats
-constexpr int LS_BLK = LS_T * LS_BPT;         // bins per k_ls_stats block
+struct GlArgs {
+  FusedArgs a;             // the dense outputs (LAST)
+  const float2* src;       // [nch][N]
+  float2* dst;             // [nch][N] (not LAST)
+  const float2* tw_tile;   // plan (L, 0) tables of the L-point transform
+  const float2* ta;        // [jj][k] = W_(NS L)^(jj k), jj < L/16, k < NS
+  const float2* tb;        // [i][k] = W_(16 NS)^(m_i k), m = 1, 2, 3, 4, 8, 12
+  uint32_t N, NS;          // NS: product of the earlier radices, a power of two >= 2
+  uint64_t ch0;            // first caller channel of the chunk
+  LsPart* part;            // LAST, threshold search: [nch][gridDim.x] statistics of the workgroups' bins
+};
 
-__global__ void __launch_bounds__(LS_T) k_ls_stats(FusedArgs a, const float2* spec, LsPart* part, uint32_t nblk) {
-  __shared__ BlockRed red;
+// CB columns (consecutive butterflies) per workgroup of T = CB L/16
+// threads, 16 elements per thread.
+template <int L, int CB, bool LAST>
+__global__ void __launch_bounds__(CB * L / 16) k_gl_pass(GlArgs g) {
+  constexpr int T = CB * L / 16;
+  using LAY = LayInter<L, CB>;
+  using PL = Plan<L, 0>;
+  constexpr int RL = PL::RL;
+  static_assert(PL::R0 == 16, "the tile's first pass takes the 16 elements a thread loads");
+  __shared__ __align__(16) float2 tile[LAY::SIZE];
+  __shared__ __align__(16) float2 twl[PL::TWSIZE_LDS];
+  const int tid = threadIdx.x;
+  copy_tw_to_lds<L, 0, T>(g.tw_tile, twl, tid);
+  const uint32_t N = g.N, NS = g.NS, M = N / L;
   const uint64_t c = blockIdx.y;
-  const uint32_t N = a.nfft;
-  const float2* X = spec + c * (uint64_t)N;
-  const uint32_t k0 = blockIdx.x * LS_BLK;
-  float fs1 = 0.f, fs2 = 0.f, bm = -1.f;
-  uint32_t bk = 0xffffffffu;
-  for (int i = 0; i < LS_BPT; ++i) {
-    const uint32_t k = k0 + (uint32_t)i * LS_T + threadIdx.x;  // coalesced; ascending per thread
-    if (k < N && in_range(k, a)) {
-      const float m = mag_of(gp(X)[k]);
-      fs1 += m;
-      fs2 = fmaf(m, m, fs2);
-      if (m > bm) {
-        bm = m;
-        bk = k;
+  // column s = butterfly j; the thread's elements e = jj + r L/16 (the
+  // tile's first-pass inputs for thread tid)
+  const int s = tid % CB, jj = tid / CB;
+  const uint32_t j = blockIdx.x * (uint32_t)CB + (uint32_t)s;
+  const uint32_t k = j & (NS - 1);
+  const float2* src = g.src + c * (uint64_t)N;
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = ld_f2<true>(src + j + (uint32_t)(jj + r * (L / 16)) * M);
+  {
+    // W_(NS L)^(e k) = W_(NS L)^(jj k) U^r, U = W_(16 NS)^k; U^r from
+    // U, U^2, U^3, U^4, U^8, U^12 (at most one product)
+    const float2 wj = gp(g.ta)[(uint32_t)jj * NS + k];
+    float2 u[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) u[i] = gp(g.tb)[(uint32_t)i * NS + k];
+    v[0] = c_mul(v[0], wj);
+#pragma unroll
+    for (int r = 1; r < 16; ++r) {
+      const int lo = r & 3, hi = r >> 2;
+      const float2 ur = hi == 0 ? u[lo - 1] : (lo == 0 ? u[2 + hi] : c_mul(u[lo - 1], u[2 + hi]));
+      v[r] = c_mul(v[r], c_mul(wj, ur));
+    }
+  }
+  fft_tile<LAY, L, CB, T, 16, 0, true>(tile, v, twl, tid);
+  // v[b RL + r] = X[s][jj + b L/16 + r L/RL]  (butterfly tid + b T of the
+  // last tile pass: column s, index jj + b T/CB)
+  const uint32_t base = (j & ~(NS - 1)) * (uint32_t)L + k;
+#pragma unroll
+  for (int b = 0; b < 16 / RL; ++b)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const uint32_t q = (uint32_t)(jj + b * (L / 16) + r * (L / RL));
+      if constexpr (LAST) store_dense(g.a, (g.ch0 + c) * (uint64_t)N + base + q * NS, v[b * RL + r]);
+      else gp(g.dst)[c * (uint64_t)N + base + q * NS] = v[b * RL + r];
+    }
+  if constexpr (LAST) {
+    if (g.part != nullptr) {  // uniform
+      // sums of |X|, |X|^2 and the largest |X| (lowest bin on ties) over the
+      // workgroup's in-range bins (in the last pass NS = N/L and base = j:
+      // the bins j + q N/L of its CB consecutive j)
+      float fs1 = 0.f, fs2 = 0.f, bm = -1.f;
+      uint32_t bk = 0xffffffffu;
+#pragma unroll
+      for (int b = 0; b < 16 / RL; ++b)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const uint32_t bin = base + (uint32_t)(jj + b * (L / 16) + r * (L / RL)) * NS;
+          if (in_range(bin, g.a)) {
+            const float m = mag_of(v[b * RL + r]);
+            fs1 += m;
+            fs2 = fmaf(m, m, fs2);
+            if (m > bm || (m == bm && bin < bk)) {
+              bm = m;
+              bk = bin;
+            }
+          }
+        }
+      __shared__ BlockRed red;
+      double s1 = fs1, s2 = fs2;
+      block_stats<T, false>(s1, s2, bm, bk, red);
+      if (tid == 0) {
+        LsPart p;
+        p.s1 = s1;
+        p.s2 = s2;
+        p.m = bm;
+        p.k = bk;
+        gp(g.part)[c * gridDim.x + blockIdx.x] = p;
       }
     }
   }
-  double s1 = fs1, s2 = fs2;
-  block_stats<LS_T, false>(s1, s2, bm, bk, red);
-  if (threadIdx.x == 0) {
-    LsPart p;
-    p.s1 = s1;
-    p.s2 = s2;
-    p.m = bm;
-    p.k = bk;
-    gp(part)[c * nblk + blockIdx.x] = p;
-  }
 }
+
+template <int L, int CB>
+static void launch_gl(const GlArgs& g, bool last, uint32_t nch, hipStream_t st) {
+  const dim3 grid(g.N / L / CB, nch);
+  if (last) hipLaunchKernelGGL((k_gl_pass<L, CB, true>), grid, dim3(CB * L / 16), 0, st, g);
+  else hipLaunchKernelGGL((k_gl_pass<L, CB, false>), grid, dim3(CB * L / 16), 0, st, g);
+}
+
+// columns per workgroup of the 2^lg-point LDS pass: 16 (128-byte runs) up
+// to 512 points; 8 at 1024, so that two workgroups share a CU (80 KiB of LDS
+// each instead of 143 KiB)
+static inline uint32_t gl_cols(int) { return 16u; }
+static void launch_gl_any(int lg, const GlArgs& g, bool last, uint32_t nch, hipStream_t st) {
+  if (lg == 8) launch_gl<256, 16>(g, last, nch, st);
+  else if (lg == 9) launch_gl<512, 16>(g, last, nch, st);
+  else launch_gl<1024, 16>(g, last, nch, st);
+}
+
+// ---------------------------------------------------------------------------
+// peak search on a stored spectrum
+// ---------------------------------------------------------------------------
+constexpr int LS_T = 256;
+constexpr int LS_BLK = 16384;                 // bins per k_ls_cand block
 
 // |X| of bin k of a channel's stored spectrum (k taken modulo N)
 __device__ __forceinline__ float ls_mag(const float2* X, uint32_t k, uint32_t N) { return mag_of(gp(X)[k & (N - 1)]); }
@@ -197,13 +290,22 @@ __global__ void __launch_bounds__(LS_T) k_ls_cand(FusedArgs a, const float2* spe
   const float thr = gp(chs)[c].thr;
   const uint64_t slot = c * nblk + blockIdx.x;
   uint32_t* out = idx + slot * LS_CAP;
-  constexpr int E = 16;
+  constexpr int E = 16, TB = LS_T * E;
+  // |X| of the tile's bins t0 - 1 .. t0 + TB (circular), read with
+  // consecutive lanes on consecutive bins; value i at i + i/16
+  __shared__ float ms[TB + 2 + (TB + 2) / 16];
   uint32_t total = 0;
-  for (uint32_t tile = 0; tile < (uint32_t)LS_BLK; tile += LS_T * E) {
-    const uint32_t k0 = blockIdx.x * LS_BLK + tile + (uint32_t)tid * E;  // N is a multiple of LS_BLK
+  for (uint32_t tile = 0; tile < (uint32_t)LS_BLK; tile += TB) {
+    const uint32_t t0 = blockIdx.x * LS_BLK + tile;  // N is a multiple of LS_BLK
+    for (int i = tid; i < TB + 2; i += LS_T) ms[i + (i >> 4)] = ls_mag(X, t0 + N - 1 + (uint32_t)i, N);
+    __syncthreads();
+    const uint32_t k0 = t0 + (uint32_t)tid * E;
     float m[E + 2];
 #pragma unroll
-    for (int e = 0; e < E + 2; ++e) m[e] = ls_mag(X, k0 + N - 1 + (uint32_t)e, N);
+    for (int e = 0; e < E + 2; ++e) {
+      const int i = tid * E + e;
+      m[e] = ms[i + (i >> 4)];
+    }
     uint32_t mask = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -211,6 +313,7 @@ __global__ void __launch_bounds__(LS_T) k_ls_cand(FusedArgs a, const float2* spe
       mask |= (ok ? 1u : 0u) << e;
     }
     int tot;
+    // the scan's barriers also end every read of ms before the next tile
     uint32_t rank = total + (uint32_t)block_excl_scan<LS_T>(__popc(mask), tot, red);
     while (mask) {
       const int e = __builtin_ctz(mask);
@@ -275,68 +378,125 @@ extern "C" {
 int spx_large_is(uint32_t nfft) { return nfft >= (1u << 19) && nfft <= (1u << 24) && (nfft & (nfft - 1)) == 0; }
 
 // Workspace of the large-length path for `chunk` channels: two ping-pong
-// spectra, plus the peak-search partials.
+// spectra, plus the peak-search partials (statistics of the last pass's
+// workgroups, at most N/4096 per channel; candidate lists of the blocks of
+// LS_BLK bins).
 uint64_t spx_large_workspace_bytes(uint32_t nfft, uint32_t chunk) {
   using namespace spx;
   const uint64_t nblk = (nfft + LS_BLK - 1) / LS_BLK;
-  return 2ull * chunk * nfft * 8 + (uint64_t)chunk * nblk * sizeof(LsPart) + (uint64_t)chunk * sizeof(LsChan) +
+  return 2ull * chunk * nfft * 8 + (uint64_t)chunk * (nfft / 4096) * sizeof(LsPart) + (uint64_t)chunk * sizeof(LsChan) +
          (uint64_t)chunk * nblk * 4 * (1 + LS_CAP) + 1024;
+}
+
+// Radix plan of N = 2^lg: the register pass 2^r0, then the LDS passes 2^l1
+// and 2^l2 (r0 + l1 + l2 = lg; a 1024-point pass only where 512 does not
+// reach: one workgroup per CU instead of two).
+int spx_large_plan(uint32_t nfft, int* r0, int* l1, int* l2) {
+  if (!spx_large_is(nfft)) return -1;
+  int lg = 0;
+  while ((1u << lg) < nfft) ++lg;
+  static const int tab[6][3] = {{3, 8, 8}, {4, 8, 8}, {4, 9, 8}, {4, 9, 9}, {4, 10, 9}, {4, 10, 10}};
+  *r0 = tab[lg - 19][0];
+  *l1 = tab[lg - 19][1];
+  *l2 = tab[lg - 19][2];
+  return 0;
+}
+
+// Stockham twiddles of the two LDS passes, in float pairs (cos, -sin),
+// generated in FP64 and rounded once: per pass (NS = product of the earlier
+// radices, L its radix) the table A[jj][k] = W_(NS L)^(jj k) (jj < L/16,
+// k < NS), then B[i][k] = W_(16 NS)^(m_i k) with m = 1, 2, 3, 4, 8, 12.
+// Returns the number of floats; fills `out` when it is not null.
+uint64_t spx_large_tables(uint32_t nfft, float* out) {
+  int r0, l1, l2;
+  if (spx_large_plan(nfft, &r0, &l1, &l2)) return 0;
+  uint64_t n = 0;
+  auto put = [&](uint64_t e, uint64_t m) {
+    if (out) {
+      const double a = 2.0 * M_PI * ((double)(e % m) / (double)m);
+      out[2 * n] = (float)std::cos(a);
+      out[2 * n + 1] = (float)-std::sin(a);
+    }
+    ++n;
+  };
+  const int lgs[2] = {l1, l2};
+  uint64_t ns = 1ull << r0;
+  for (int p = 0; p < 2; ++p) {
+    const uint64_t L = 1ull << lgs[p];
+    for (uint64_t jj = 0; jj < L / 16; ++jj)
+      for (uint64_t k = 0; k < ns; ++k) put(jj * k, ns * L);
+    static const int pw[6] = {1, 2, 3, 4, 8, 12};
+    for (int i = 0; i < 6; ++i)
+      for (uint64_t k = 0; k < ns; ++k) put((uint64_t)pw[i] * k, 16 * ns);
+    ns *= L;
+  }
+  return 2 * n;
 }
 
 // Transform (and, for a->peak_mode != 0, the peak search) of a->batch
 // channels of length a->nfft in chunks of `chunk` channels.  tw: the
-// concatenated per-pass tables of length nfft (radix 16 first, the leftover
-// radix last), as make_twiddles builds them.
-hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, void* ws, uint32_t chunk, hipStream_t st) {
+// Stockham tables of spx_large_tables; tw1, tw2: the plan (L, 0) tables of
+// the two LDS passes' L-point transforms, as make_twiddles builds them.
+hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, const float2* tw1, const float2* tw2, void* ws,
+                            uint32_t chunk, hipStream_t st) {
   using namespace spx;
   const uint32_t N = a0->nfft;
-  if (!spx_large_is(N) || chunk == 0) return hipErrorInvalidValue;
-  int lg = 0;
-  while ((1u << lg) < N) ++lg;
-  const int np = (lg + 3) / 4;
-  const int rx = (lg % 4) ? (1 << (lg % 4)) : 16;
+  int r0, l1, l2;
+  if (spx_large_plan(N, &r0, &l1, &l2) || chunk == 0) return hipErrorInvalidValue;
+  const uint32_t R0 = 1u << r0, L1 = 1u << l1, L2 = 1u << l2;
   const uint64_t nblk = (N + LS_BLK - 1) / LS_BLK;
+  const uint32_t cols2 = gl_cols(l2), nparts = N / ((1u << l2) * cols2);  // last-pass workgroups per channel
   float2* buf[2] = {reinterpret_cast<float2*>(ws), reinterpret_cast<float2*>(ws) + (uint64_t)chunk * N};
   unsigned char* tail = reinterpret_cast<unsigned char*>(ws) + 2ull * chunk * N * 8;
   LsPart* part = reinterpret_cast<LsPart*>(tail);
-  tail += (uint64_t)chunk * nblk * sizeof(LsPart);
+  tail += ((uint64_t)chunk * (N / 4096) * sizeof(LsPart) + 255) & ~255ull;
   LsChan* chs = reinterpret_cast<LsChan*>(tail);
   tail += ((uint64_t)chunk * sizeof(LsChan) + 255) & ~255ull;
   uint32_t* cnt = reinterpret_cast<uint32_t*>(tail);
   tail += ((uint64_t)chunk * nblk * 4 + 255) & ~255ull;
   uint32_t* idx = reinterpret_cast<uint32_t*>(tail);
   const bool peaks = a0->peak_mode != 0;
+  // table offsets (float2 units): A1 | B1 | A2 | B2
+  const uint64_t a1 = 0, b1 = a1 + (uint64_t)R0 * (L1 / 16), a2 = b1 + 6ull * R0,
+                 b2 = a2 + (uint64_t)R0 * L1 * (L2 / 16);
   for (uint32_t c0 = 0; c0 < a0->batch; c0 += chunk) {
     const uint32_t nch = min(chunk, a0->batch - c0);
+    // pass 0 -> buf[0] -> pass 1 -> buf[1] -> pass 2 -> the dense outputs,
+    // or for the peak search the chunk's complex spectrum in buf[0]
     GsArgs g;
     g.a = *a0;
     g.ch0 = c0;
     g.N = N;
-    // pass p reads buf[(p+1) % 2] and writes buf[p % 2]; for the peak search
-    // the last pass writes the chunk's complex spectrum into the buffer it
-    // does not read
-    float2* spec = buf[(np - 1) % 2];
+    g.dst = buf[0];
+    switch (R0) {
+      case 8: launch_gs_first<8>(g, nch, st); break;
+      default: launch_gs_first<16>(g, nch, st); break;
+    }
+    GlArgs h;
+    h.a = *a0;
+    h.ch0 = c0;
+    h.N = N;
+    h.src = buf[0];
+    h.dst = buf[1];
+    h.NS = R0;
+    h.tw_tile = tw1;
+    h.ta = tw + a1;
+    h.tb = tw + b1;
+    h.part = nullptr;
+    launch_gl_any(l1, h, false, nch, st);
+    float2* spec = buf[0];
     if (peaks) {
-      g.a.spec_out = spec - (uint64_t)c0 * N;
-      g.a.mag_out = nullptr;
+      h.a.spec_out = spec - (uint64_t)c0 * N;
+      h.a.mag_out = nullptr;
     }
-    uint64_t ns = 1, off = 0;
-    for (int p = 0; p < np; ++p) {
-      const int R = (p == np - 1) ? rx : 16;
-      g.NS = (uint32_t)ns;
-      g.tw = p == 0 ? nullptr : tw + off;
-      g.src = buf[(p + 1) % 2];
-      g.dst = buf[p % 2];
-      const bool first = p == 0, last = p == np - 1;
-      switch (R) {
-        case 2: launch_gs<2>(g, first, last, nch, st); break;
-        case 4: launch_gs<4>(g, first, last, nch, st); break;
-        case 8: launch_gs<8>(g, first, last, nch, st); break;
-        default: launch_gs<16>(g, first, last, nch, st); break;
-      }
-      if (p > 0) off += ns * (uint64_t)R;
-      ns *= (uint64_t)R;
-    }
+    h.src = buf[1];
+    h.dst = nullptr;
+    h.NS = R0 * L1;
+    h.tw_tile = tw2;
+    h.ta = tw + a2;
+    h.tb = tw + b2;
+    h.part = peaks ? part : nullptr;
+    launch_gl_any(l2, h, true, nch, st);
     if (peaks) {
       FusedArgs b = *a0;
       b.batch = nch;
@@ -344,8 +504,7 @@ hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, void* ws, uin
       b.counts = a0->counts + 2ull * c0;
       b.gmax = a0->gmax ? a0->gmax + c0 : nullptr;
       b.stats = a0->stats ? a0->stats + 4ull * c0 : nullptr;
-      hipLaunchKernelGGL(k_ls_stats, dim3((uint32_t)nblk, nch), dim3(LS_T), 0, st, b, spec, part, (uint32_t)nblk);
-      hipLaunchKernelGGL(k_ls_thresh, dim3(nch), dim3(LS_T), 0, st, b, spec, part, (uint32_t)nblk, chs);
+      hipLaunchKernelGGL(k_ls_thresh, dim3(nch), dim3(LS_T), 0, st, b, spec, part, nparts, chs);
       hipLaunchKernelGGL(k_ls_cand, dim3((uint32_t)nblk, nch), dim3(LS_T), 0, st, b, spec, chs, (uint32_t)nblk, cnt,
                          idx);
       hipLaunchKernelGGL(k_ls_gather, dim3(nch), dim3(LS_T), 0, st, b, spec, cnt, idx, (uint32_t)nblk);
