@@ -334,6 +334,17 @@ int run_fused(spx_ctx* c, FusedArgs& a) {
   return learned_refine(c, a);
 }
 
+// Cosine coefficients of the symmetric windows, w(n) = c0 - c1 cos(2 pi
+// n/(n-1)) + c2 cos(4 pi n/(n-1)) (the formulas of make_window), for kernels
+// that evaluate the window instead of reading its table.
+void window_coefs(uint32_t window, FusedArgs& a) {
+  a.wkind = window;
+  a.wc0 = a.wc1 = a.wc2 = 0.0f;
+  if (window == SPX_WINDOW_HANN) { a.wc0 = 0.5f; a.wc1 = 0.5f; }
+  else if (window == SPX_WINDOW_HAMMING) { a.wc0 = 0.54f; a.wc1 = 0.46f; }
+  else if (window == SPX_WINDOW_BLACKMAN) { a.wc0 = 0.42f; a.wc1 = 0.5f; a.wc2 = 0.08f; }
+}
+
 int window_ptr(spx_ctx* c, uint32_t window, uint32_t n_point, const float** w) {
   *w = nullptr;
   if (window == SPX_WINDOW_NONE) return SPX_OK;
@@ -357,6 +368,7 @@ int dense_fft(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_t n_
   a.nfft = nfft;
   a.dtype = SPX_DTYPE_C64;
   if ((rc = window_ptr(c, window, n_point, &a.window))) return rc;
+  window_coefs(window, a);
   if (kind == 0) a.spec_out = (float2*)out;
   else {
     a.mag_out = (float*)out;
@@ -656,6 +668,7 @@ int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a) {
   a.dtype = d->dtype;
   a.fs = d->sample_rate;
   if ((rc = window_ptr(c, d->window, d->n_point, &a.window))) return rc;
+  window_coefs(d->window, a);
   a.peak_mode = d->peak_mode;
   a.nsigma = d->nsigma;
   a.kcap = d->max_peaks;
@@ -758,15 +771,10 @@ int spx_fused_peaks(spx_ctx* c, const spx_fused_desc* d) {
   return run_fused(c, a);
 }
 
-// Window coefficients (w = c0 - c1 cos(2 pi n/(n-1)) + c2 cos(4 pi n/(n-1)),
-// the formulas of make_window), tables, persistent grid, launch.
+// Window coefficients, tables, persistent grid, launch.
 static int run_mixed(spx_ctx* c, FusedArgs& a, uint32_t window, uint32_t compute) {
   a.window = nullptr;
-  a.wkind = window;
-  a.wc0 = a.wc1 = a.wc2 = 0.0f;
-  if (window == SPX_WINDOW_HANN) { a.wc0 = 0.5f; a.wc1 = 0.5f; }
-  else if (window == SPX_WINDOW_HAMMING) { a.wc0 = 0.54f; a.wc1 = 0.46f; }
-  else if (window == SPX_WINDOW_BLACKMAN) { a.wc0 = 0.42f; a.wc1 = 0.5f; a.wc2 = 0.08f; }
+  window_coefs(window, a);
   const void* t;
   int rc = get_table(c, MIX, compute, 0, &t);
   if (rc) return rc;

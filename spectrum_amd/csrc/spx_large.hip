@@ -6,9 +6,9 @@
 // window, zero padding and, for 16-bit storage, the per-channel load scale
 // as it reads the input), then two passes of radix L1 and L2 (256, 512 or
 // 1024 points, N = R0 L1 L2) whose butterflies are whole L-point transforms
-// in an LDS tile: a workgroup takes 16 consecutive butterflies j (8 in the
-// 1024-point passes; the columns of the tile, so every load and store moves
-// runs of 16 or 8 consecutive complex values), multiplies element e of butterfly j by the Stockham
+// in an LDS tile: a workgroup takes 16 or 32 consecutive butterflies j (the
+// columns of the tile, so every load and store moves runs of 16 or 32
+// consecutive complex values), multiplies element e of butterfly j by the Stockham
 // twiddle W_(NS L)^(e (j mod NS)), transforms the 16 columns with the
 // LDS-tile FFT of the split path's column kernel and stores element q at
 // (j div NS) NS L + (j mod NS) + q NS.  The last pass writes the dense output
@@ -52,32 +52,46 @@ struct GsArgs {
 // and its outputs go to y[j R + q].  The 256 butterflies of a workgroup
 // cover one contiguous run of 256 R outputs, stored through LDS so that
 // consecutive lanes write consecutive values.
-template <int R>
+template <int R, int DT>
 __global__ void __launch_bounds__(256) k_gs_first(GsArgs g) {
   constexpr int T = 256, E = T * R;
+  constexpr uint32_t ES = DT == SPX_DTYPE_C64 ? 8 : 4;
   __shared__ float2 xs[E + E / 16];  // element i at i + i/16
   const uint32_t N = g.N, NB = N / R;
   const int tid = threadIdx.x;
   const uint32_t j = blockIdx.x * T + (uint32_t)tid;  // NB is a multiple of T
   const uint64_t c = blockIdx.y;
+  const FusedArgs& a = g.a;
+  const uint64_t ch = g.ch0 + c;
+  const void* in = reinterpret_cast<const char*>(a.in) + ch * (uint64_t)a.n_point * ES;
+  const uint32_t last = a.n_point - 1;
+  // every load is issued before any value is used (no branch between them:
+  // a per-sample branch made each load wait for the previous one)
   float2 v[R];
-  {
-    const FusedArgs& a = g.a;
-    const uint64_t ch = g.ch0 + c;
-    const uint32_t es = a.dtype == SPX_DTYPE_C64 ? 8 : 4;
-    const void* in = reinterpret_cast<const char*>(a.in) + ch * (uint64_t)a.n_point * es;
-    const float sc = a.in_scale != nullptr ? gp(a.in_scale)[ch] : 1.0f;
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = ld_raw<DT, true>(in, min(j + (uint32_t)r * NB, last));
+  const float sc = a.in_scale != nullptr ? gp(a.in_scale)[ch] : 1.0f;
+  // the window from its cosine coefficients (hardware cosine, input in
+  // revolutions; measured within 3e-7 of the FP64 window up to 2^24
+  // points) rather than from a table that would compete with the signal
+  // for the L2 at these lengths
+  float w[R];
+  if (a.wkind != 0 && a.n_point > 1) {  // uniform
+    const float winv = 1.0f / (float)(a.n_point - 1);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const uint32_t n = j + (uint32_t)r * NB;
-      const uint32_t ni = min(n, a.n_point - 1);
-      const float w = a.window != nullptr ? gp(a.window)[ni] : 1.0f;
-      const float2 x = a.dtype == SPX_DTYPE_C64 ? ld_raw<SPX_DTYPE_C64, true>(in, ni)
-                       : a.dtype == SPX_DTYPE_C32F16 ? ld_raw<SPX_DTYPE_C32F16, true>(in, ni)
-                                                     : ld_raw<SPX_DTYPE_C32BF16, true>(in, ni);
-      v[r] = c_scale(x, n < a.n_point ? w * sc : 0.0f);
+      const float cx = __builtin_amdgcn_cosf((float)min(j + (uint32_t)r * NB, last) * winv);
+      w[r] = fmaf(-a.wc1, cx, a.wc0) + a.wc2 * fmaf(2.0f * cx, cx, -1.0f);
     }
+  } else if (a.window != nullptr && a.wkind == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) w[r] = gp(a.window)[min(j + (uint32_t)r * NB, last)];
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) w[r] = 1.0f;
   }
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = c_scale(v[r], j + (uint32_t)r * NB < a.n_point ? w[r] * sc : 0.0f);
   Dft<R>::run(v);
   // y[j R + q] = v[q]: local element tid R + q
 #pragma unroll
@@ -97,7 +111,10 @@ __global__ void __launch_bounds__(256) k_gs_first(GsArgs g) {
 template <int R>
 static void launch_gs_first(const GsArgs& g, uint32_t nch, hipStream_t st) {
   const dim3 grid(g.N / R / 256, nch);
-  hipLaunchKernelGGL((k_gs_first<R>), grid, dim3(256), 0, st, g);
+  if (g.a.dtype == SPX_DTYPE_C64) hipLaunchKernelGGL((k_gs_first<R, SPX_DTYPE_C64>), grid, dim3(256), 0, st, g);
+  else if (g.a.dtype == SPX_DTYPE_C32F16)
+    hipLaunchKernelGGL((k_gs_first<R, SPX_DTYPE_C32F16>), grid, dim3(256), 0, st, g);
+  else hipLaunchKernelGGL((k_gs_first<R, SPX_DTYPE_C32BF16>), grid, dim3(256), 0, st, g);
 }
 
 // ---------------------------------------------------------------------------
@@ -170,7 +187,7 @@ __global__ void __launch_bounds__(CB * L / 16) k_gl_pass(GlArgs g) {
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const uint32_t q = (uint32_t)(jj + b * (L / 16) + r * (L / RL));
-      if constexpr (LAST) store_dense(g.a, (g.ch0 + c) * (uint64_t)N + base + q * NS, v[b * RL + r]);
+      if constexpr (LAST) store_dense<true>(g.a, (g.ch0 + c) * (uint64_t)N + base + q * NS, v[b * RL + r]);
       else gp(g.dst)[c * (uint64_t)N + base + q * NS] = v[b * RL + r];
     }
   if constexpr (LAST) {
@@ -217,12 +234,16 @@ static void launch_gl(const GlArgs& g, bool last, uint32_t nch, hipStream_t st) 
   else hipLaunchKernelGGL((k_gl_pass<L, CB, false>), grid, dim3(CB * L / 16), 0, st, g);
 }
 
-// columns per workgroup of the 2^lg-point LDS pass: 16 (128-byte runs) up
-// to 512 points; 8 at 1024, so that two workgroups share a CU (80 KiB of LDS
-// each instead of 143 KiB)
-static inline uint32_t gl_cols(int) { return 16u; }
+// columns per workgroup of the 2^lg-point LDS pass, measured (512 MB of
+// input, complex spectrum): 256 points 32 columns (256-byte runs, two
+// workgroups of 66 KiB per CU; 2^20: 0.595 -> 0.559 ms against 16 columns,
+// 0.609 ms with 64), 512 points 16 (32 columns: one workgroup per CU,
+// slower), 1024 points 16 (one workgroup per CU; 8 columns, two per CU with
+// 64-byte runs, were slower: the two passes at 2^24 took 356 and 419 us
+// against 270 and 302 us)
+static inline uint32_t gl_cols(int lg) { return lg == 8 ? 32u : 16u; }
 static void launch_gl_any(int lg, const GlArgs& g, bool last, uint32_t nch, hipStream_t st) {
-  if (lg == 8) launch_gl<256, 16>(g, last, nch, st);
+  if (lg == 8) launch_gl<256, 32>(g, last, nch, st);
   else if (lg == 9) launch_gl<512, 16>(g, last, nch, st);
   else launch_gl<1024, 16>(g, last, nch, st);
 }
