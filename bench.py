@@ -26,7 +26,9 @@ with --no-extras):
              (time per launch, frequency RMSE against the true tones);
   refiner_split  the same at the headline shape, 1024 x 65536 (split path);
   top_peaks  the reference ONE/TWO_PEAKS search inside the transform vs
-             transform-then-search, 1024 x 65536.
+             transform-then-search, 1024 x 65536;
+  large      64 x 2^20 (past the split path): peak search and complex
+             spectrum, time and fraction of 8 TB/s on algorithmic bytes.
 """
 import argparse
 import ctypes as C
@@ -472,6 +474,30 @@ def extra_small_batch(ctx, K=32, reps=100):
     return out
 
 
+def extra_large(ctx, K=64, reps=20):
+    """Lengths past the split path: 64 x 2^20 complex64 (512 MB), Hann,
+    mean+6std, K=64, and the complex spectrum of the same batch.  Three
+    Stockham passes through HBM (a radix-16 register pass, two 256-point
+    LDS-tile passes), the threshold statistics inside the last pass, then the
+    candidate search on the stored spectrum; algorithmic bytes are the input
+    (plus the spectrum for the dense output)."""
+    from spectrum_amd import capi
+
+    B, N = 64, 1 << 20
+    x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED0314, B, N)
+    d = ctx.to_device(x)
+    out = ctx.alloc(B * N * 8)
+    pk, ct = ctx.alloc(B * K * 32), ctx.alloc(B * 8)
+    t_p = _timer_ms(ctx, lambda: ctx.fused_peaks(d, B, N, N, pk, ct, max_peaks=K), reps, 5)
+    t_c = _timer_ms(ctx, lambda: ctx.fft_c2c(d, out, B, N, N, 1), reps, 5)
+    alg_p = B * (8 * N + K * 32 + 8)
+    alg_c = B * 16 * N
+    return {"workload": f"{B} x {N} complex64, Hann, mean+6std, K={K}; and its complex spectrum",
+            "peaks_ms": t_p, "peaks_channels_per_s": B / (t_p * 1e-3),
+            "peaks_achieved_GBps": alg_p / (t_p * 1e-3) / 1e9, "peaks_frac": alg_p / (t_p * 1e-3) / 8e12,
+            "c2c_ms": t_c, "c2c_achieved_GBps": alg_c / (t_c * 1e-3) / 1e9, "c2c_frac": alg_c / (t_c * 1e-3) / 8e12}
+
+
 def extra_top_peaks(ctx, reps=20):
     """The reference ONE_PEAK / TWO_PEAKS search of SpectrumProcessorROCm at
     1024 x 65536 complex64 (search range nfft/4): inside the transform
@@ -515,7 +541,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--extra", choices=["host_stream", "uneven", "mixed_mlp", "refiner", "refiner_split", "top_peaks",
-                                        "small_batch"],
+                                        "small_batch", "large"],
                     help="run only this extra workload (no headline, no CPU baseline) and print its JSON; "
                          "for profiling one workload")
     ap.add_argument("--reps", type=int, default=30, help="timed repetitions of each --extra form")
@@ -539,7 +565,7 @@ def main():
             res = extra_mixed_mlp(ctx, reps=args.reps)
         else:
             res = {"uneven": extra_uneven, "refiner": extra_refiner, "refiner_split": extra_refiner_split,
-                   "top_peaks": extra_top_peaks, "small_batch": extra_small_batch}[args.extra](ctx)
+                   "top_peaks": extra_top_peaks, "small_batch": extra_small_batch, "large": extra_large}[args.extra](ctx)
         print(json.dumps({args.extra: res}), flush=True)
         return
     info = ctx.info()
@@ -621,7 +647,7 @@ def main():
         del d_in, x
         for name, fn in (("small_batch", extra_small_batch), ("uneven", extra_uneven), ("mixed_mlp", extra_mixed_mlp),
                          ("refiner", extra_refiner), ("refiner_split", extra_refiner_split),
-                         ("top_peaks", extra_top_peaks)):
+                         ("top_peaks", extra_top_peaks), ("large", extra_large)):
             print(f"[bench] {name} ...", file=sys.stderr, flush=True)
             try:
                 out[name] = fn(ctx)

@@ -15,9 +15,9 @@
  * Layouts: input [batch][n_point] complex64 (beam-major), spectra
  * [batch][nfft], records spx_max_value[batch][cap] (32 bytes each).
  * nfft must be a power of two, 2 .. 2^24, nfft >= n_point.  Lengths up to
- * 2^18 run on the fused kernels; 2^19 .. 2^24 run as global-memory
- * Stockham passes with the peak search on the stored spectrum (same rules
- * and records).
+ * 2^18 run on the fused kernels; 2^19 .. 2^24 run as three Stockham
+ * passes through global memory with the peak search on the stored spectrum
+ * (same rules and records).
  */
 #ifndef SPECTRUM_MI355X_H
 #define SPECTRUM_MI355X_H

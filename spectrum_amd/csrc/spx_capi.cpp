@@ -368,7 +368,6 @@ int dense_fft(spx_ctx* c, const void* in, void* out, uint32_t batch, uint32_t n_
   a.nfft = nfft;
   a.dtype = SPX_DTYPE_C64;
   if ((rc = window_ptr(c, window, n_point, &a.window))) return rc;
-  window_coefs(window, a);
   if (kind == 0) a.spec_out = (float2*)out;
   else {
     a.mag_out = (float*)out;
@@ -668,7 +667,6 @@ int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a) {
   a.dtype = d->dtype;
   a.fs = d->sample_rate;
   if ((rc = window_ptr(c, d->window, d->n_point, &a.window))) return rc;
-  window_coefs(d->window, a);
   a.peak_mode = d->peak_mode;
   a.nsigma = d->nsigma;
   a.kcap = d->max_peaks;

@@ -71,19 +71,12 @@ __global__ void __launch_bounds__(256) k_gs_first(GsArgs g) {
 #pragma unroll
   for (int r = 0; r < R; ++r) v[r] = ld_raw<DT, true>(in, min(j + (uint32_t)r * NB, last));
   const float sc = a.in_scale != nullptr ? gp(a.in_scale)[ch] : 1.0f;
-  // the window from its cosine coefficients (hardware cosine, input in
-  // revolutions; measured within 3e-7 of the FP64 window up to 2^24
-  // points) rather than from a table that would compete with the signal
-  // for the L2 at these lengths
+  // the window from its FP64-generated table, read after the signal loads
+  // are in flight (evaluating it from its cosine coefficients with the
+  // hardware cosine measured 1-4 % faster from 2^22 up, but 2-6e-7 off the
+  // FP64 window instead of the table's rounding of 3e-8)
   float w[R];
-  if (a.wkind != 0 && a.n_point > 1) {  // uniform
-    const float winv = 1.0f / (float)(a.n_point - 1);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const float cx = __builtin_amdgcn_cosf((float)min(j + (uint32_t)r * NB, last) * winv);
-      w[r] = fmaf(-a.wc1, cx, a.wc0) + a.wc2 * fmaf(2.0f * cx, cx, -1.0f);
-    }
-  } else if (a.window != nullptr && a.wkind == 0) {
+  if (a.window != nullptr) {  // uniform
 #pragma unroll
     for (int r = 0; r < R; ++r) w[r] = gp(a.window)[min(j + (uint32_t)r * NB, last)];
   } else {

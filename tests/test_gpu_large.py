@@ -1,4 +1,4 @@
-"""Lengths 2^19 .. 2^22 (global-memory Stockham passes, peak search on the
+"""Lengths 2^19 .. 2^24 (three Stockham passes through HBM, peak search on the
 stored spectrum) against the FP64 oracle.
 
 The reference plans any power-of-two nFFT = nextPow2(n_point) * repeat_count
@@ -24,9 +24,13 @@ def _tol(nfft):
 
 
 @pytest.mark.parametrize("nfft,n_point,win", [(1 << 19, (1 << 19) - 77, 1), (1 << 20, 1 << 20, 0),
-                                              (1 << 21, 1500000, 2)])
+                                              (1 << 21, 1500000, 2), (1 << 23, (1 << 23) - 5, 3),
+                                              (1 << 24, 1 << 24, 1)])
 def test_large_dense_vs_fp64(ctx, nfft, n_point, win):
-    B = 2
+    """Every radix plan: 2^19 = 8 x 256 x 256, 2^20 = 16 x 256 x 256, 2^21 =
+    16 x 512 x 256, 2^23 = 16 x 1024 x 512, 2^24 = 16 x 1024 x 1024 (the
+    window evaluated in the first pass from its cosine coefficients)."""
+    B = 2 if nfft <= (1 << 21) else 1
     print(f"[case] large dense nfft={nfft} n_point={n_point} win={win}", flush=True)
     rng = np.random.default_rng(nfft + n_point)
     x = (rng.standard_normal((B, n_point)) + 1j * rng.standard_normal((B, n_point))).astype(np.complex64)
@@ -52,7 +56,7 @@ def test_large_dense_vs_fp64(ctx, nfft, n_point, win):
     assert np.max(np.abs(dphi[big])) < 1e-4
 
 
-@pytest.mark.parametrize("nfft,B", [(1 << 19, 4), (1 << 20, 3), (1 << 22, 2)])
+@pytest.mark.parametrize("nfft,B", [(1 << 19, 4), (1 << 20, 3), (1 << 22, 2), (1 << 24, 1)])
 def test_large_threshold_search(ctx, nfft, B):
     """mean + 6 std threshold search, K = 64, Hann, multi-tone input."""
     K = 64
@@ -108,20 +112,21 @@ def test_large_noise_band_and_cap(ctx):
         assert np.all(P2[b, : C2[b, 0]]["freq_offset"] == 0)
 
 
-def test_large_half_storage(ctx):
-    """fp16 storage with a per-channel load scale: the oracle runs on the
-    decoded values times the scale."""
-    nfft, B, K = 1 << 19, 2, 32
-    print(f"[case] large fp16 nfft={nfft}", flush=True)
+@pytest.mark.parametrize("dtype,nfft", [(capi.DTYPE_C32F16, 1 << 19), (capi.DTYPE_C32BF16, 1 << 20)])
+def test_large_half_storage(ctx, dtype, nfft):
+    """fp16 / bf16 storage with a per-channel load scale: the oracle runs on
+    the decoded values times the scale."""
+    B, K = 2, 32
+    print(f"[case] large half storage dtype={dtype} nfft={nfft}", flush=True)
     x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED0310, B, nfft)
-    u = capi.to_half_storage(x * 0.125, capi.DTYPE_C32F16)
+    u = capi.to_half_storage(x * 0.125, dtype)
     scale = np.array([2.0, 0.5], np.float32)
     d_in, d_sc = ctx.to_device(u), ctx.to_device(scale)
     pk, ct, g, st = ctx.alloc(B * K * 32), ctx.alloc(B * 8), ctx.alloc(B * 32), ctx.alloc(B * 16)
-    ctx.fused_peaks(d_in, B, nfft, nfft, pk, ct, max_peaks=K, dtype=capi.DTYPE_C32F16, in_scale=d_sc, gmax=g,
+    ctx.fused_peaks(d_in, B, nfft, nfft, pk, ct, max_peaks=K, dtype=dtype, in_scale=d_sc, gmax=g,
                     stats=st, window=1, fs=1.0)
     ctx.sync()
-    xr = (capi.from_half_storage(u, capi.DTYPE_C32F16) * scale[:, None]).astype(np.complex64)
+    xr = (capi.from_half_storage(u, dtype) * scale[:, None]).astype(np.complex64)
     check_channels(xr, nfft, pk.download(capi.MAXVALUE_DTYPE, (B, K)), ct.download(np.uint32, (B, 2)),
                    g.download(capi.MAXVALUE_DTYPE, (B,)), st.download(np.float32, (B, 4)), K=K, chunk=1)
 
