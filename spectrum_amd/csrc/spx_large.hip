@@ -311,7 +311,22 @@ __global__ void __launch_bounds__(LS_T) k_ls_cand(FusedArgs a, const float2* spe
   uint32_t total = 0;
   for (uint32_t tile = 0; tile < (uint32_t)LS_BLK; tile += TB) {
     const uint32_t t0 = blockIdx.x * LS_BLK + tile;  // N is a multiple of LS_BLK
-    for (int i = tid; i < TB + 2; i += LS_T) ms[i + (i >> 4)] = ls_mag(X, t0 + N - 1 + (uint32_t)i, N);
+    {
+      // all E loads in flight before the first use (non-temporal: the
+      // spectrum is read once here), then the two values past the tile
+      float2 xv[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) xv[e] = ld_f2<true>(X + ((t0 + N - 1 + (uint32_t)(tid + e * LS_T)) & (N - 1)));
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = tid + e * LS_T;
+        ms[i + (i >> 4)] = mag_of(xv[e]);
+      }
+      if (tid < 2) {
+        const int i = TB + tid;
+        ms[i + (i >> 4)] = ls_mag(X, t0 + N - 1 + (uint32_t)i, N);
+      }
+    }
     __syncthreads();
     const uint32_t k0 = t0 + (uint32_t)tid * E;
     float m[E + 2];
