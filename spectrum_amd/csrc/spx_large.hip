@@ -128,7 +128,7 @@ struct GlArgs {
   const float2* ta;        // [jj][k] = W_(NS L)^(jj k), jj < L/16, k < NS
   const float2* tb;        // [i][k] = W_(16 NS)^(m_i k), m = 1, 2, 3, 4, 8, 12
   uint32_t N, NS;          // NS: product of the earlier radices, a power of two >= 2
-  uint64_t ch0;            // first caller channel of the chunk
+  uint64_t ch0;            // LAST: channel index of the chunk's first channel in the dense outputs
   LsPart* part;            // LAST, threshold search: [nch][gridDim.x] statistics of the workgroups' bins
 };
 
@@ -514,9 +514,10 @@ hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, const float2*
     h.part = nullptr;
     launch_gl_any(l1, h, false, nch, st);
     float2* spec = buf[0];
-    if (peaks) {
-      h.a.spec_out = spec - (uint64_t)c0 * N;
+    if (peaks) {  // the chunk's spectrum, indexed from its first channel
+      h.a.spec_out = spec;
       h.a.mag_out = nullptr;
+      h.ch0 = 0;
     }
     h.src = buf[1];
     h.dst = nullptr;

@@ -154,3 +154,29 @@ def test_large_through_facades():
     assert f.get_params()["nFFT"] == 1 << 19
     for r, f0 in zip(res, (12345.6, 67890.1)):
         assert abs(r["freq_hz"] - f0) < fs / (1 << 19)
+
+
+def test_large_launch_groups(ctx):
+    """A batch larger than one launch group of the 2 GiB workspace (2^19:
+    227 channels per group): channels on both sides of the group boundary
+    against the FP64 oracle, for the complex spectrum and the threshold
+    search."""
+    nfft, B, K = 1 << 19, 300, 32
+    print(f"[case] large launch groups nfft={nfft} B={B}", flush=True)
+    x = capi.siggen(capi.GEN_MULTI_TONE, 0x5EED0320, B, nfft)
+    d_in = ctx.to_device(x)
+    spec = ctx.alloc(B * nfft * 8)
+    pk, ct, g, st = ctx.alloc(B * K * 32), ctx.alloc(B * 8), ctx.alloc(B * 32), ctx.alloc(B * 16)
+    ctx.fft_c2c(d_in, spec, B, nfft, nfft, 1)
+    ctx.fused_peaks(d_in, B, nfft, nfft, pk, ct, max_peaks=K, gmax=g, stats=st, window=1, fs=1.0)
+    ctx.sync()
+    idx = np.array([0, 226, 227, 228, 299])
+    S = spec.download(np.complex64, (B, nfft))[idx]
+    ref = orc.spectrum64(x[idx], nfft, 1)
+    rel = np.linalg.norm(S - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.all(rel <= _tol(nfft)), rel
+    P = pk.download(capi.MAXVALUE_DTYPE, (B, K))[idx]
+    C = ct.download(np.uint32, (B, 2))[idx]
+    G = g.download(capi.MAXVALUE_DTYPE, (B,))[idx]
+    T = st.download(np.float32, (B, 4))[idx]
+    check_channels(x[idx], nfft, P, C, G, T, K=K, chunk=1)
