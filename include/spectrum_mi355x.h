@@ -219,11 +219,12 @@ int spx_host_free(spx_ctx* ctx, void* ptr);
  * GPU (own forward/backward and Adam kernels, FP32 master weights); the
  * deployed block holds FP16 weights for MFMA inference with FP32
  * accumulation.  With spx_set_refiner, spx_fused_peaks accepts
- * refine = SPX_REFINE_LEARNED (threshold search, nfft 1024..262144): the
- * fused kernel stores the parabolic records and the features of each stored
- * peak (on the split path, nfft >= 32768, the row kernel keeps each
+ * refine = SPX_REFINE_LEARNED (threshold search, nfft >= 1024): the fused
+ * kernel stores the parabolic records and the features of each stored peak
+ * (on the split path, nfft 32768 .. 262144, the row kernel keeps each
  * candidate's complex neighbourhood and the finish kernel writes the
- * features), and a second MFMA kernel replaces the offset wherever the
+ * features; from 2^19 the gather kernel reads them from the stored chunk
+ * spectrum), and a second MFMA kernel replaces the offset wherever the
  * network's prediction lies in [-0.5, 0.5]. */
 int spx_mlp_param_count(void);    /* 4929 FP32 master parameters */
 int spx_mlp_deployed_size(void);  /* bytes of the deployed block */

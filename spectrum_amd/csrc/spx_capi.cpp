@@ -284,11 +284,12 @@ int run_fused(spx_ctx* c, FusedArgs& a) {
     if ((rc = get_table(c, LG, a.nfft, 0, &t))) return rc;
     if ((rc = get_table(c, TW, 1u << l1, 0, &t1))) return rc;
     if ((rc = get_table(c, TW, 1u << l2, 0, &t2))) return rc;
-    const uint64_t per = spx_large_workspace_bytes(a.nfft, 1);
+    const uint32_t lk = (a.refine == 2 && a.mlp != nullptr && a.peak_mode == 1) ? a.kcap : 0u;
+    const uint64_t per = spx_large_workspace_bytes(a.nfft, 1, lk);
     const uint64_t budget = c->ws_fixed ? c->ws_fixed_bytes : (2ull << 30);
     uint32_t chunk = (uint32_t)std::min<uint64_t>(a.batch, std::max<uint64_t>(c->ws_fixed ? 0 : 1, budget / per));
     if (chunk == 0) return ensure_scratch(c, per);  // the fixed workspace is too small: reports the shortfall
-    if ((rc = ensure_scratch(c, spx_large_workspace_bytes(a.nfft, chunk)))) return rc;
+    if ((rc = ensure_scratch(c, spx_large_workspace_bytes(a.nfft, chunk, lk)))) return rc;
     SPX_HIP(c, hipSetDevice(c->device));
     SPX_HIP(c, spx_launch_large(&a, (const float2*)t, (const float2*)t1, (const float2*)t2,
                                 c->ws_fixed ? c->ws_fixed : c->scratch, chunk, c->stream));
@@ -646,8 +647,8 @@ int fused_args(spx_ctx* c, const spx_fused_desc* d, FusedArgs& a) {
   if (d->refine > SPX_REFINE_LEARNED) return fail(c, SPX_EINVAL, "unknown refine mode");
   if (d->refine == SPX_REFINE_LEARNED) {
     if (!c->refiner) return fail(c, SPX_EINVAL, "learned refinement requested but no refiner is set (spx_set_refiner)");
-    if (d->peak_mode != SPX_PEAKS_THRESHOLD || d->nfft < 1024 || d->nfft > (1u << 18))
-      return fail(c, SPX_ENOTSUP, "learned refinement: threshold search and nfft 1024..262144 only");
+    if (d->peak_mode != SPX_PEAKS_THRESHOLD || d->nfft < 1024)
+      return fail(c, SPX_ENOTSUP, "learned refinement: threshold search and nfft >= 1024 only");
   }
   uint32_t s0 = d->search_start, s1 = d->search_end;
   if (d->peak_mode == SPX_PEAKS_ALL_MAXIMA) {

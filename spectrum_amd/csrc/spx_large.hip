@@ -35,6 +35,7 @@
 //   k_ls_gather  one block per channel: offsets of the blocks' lists, then
 //                the first K bins overall as records, in ascending order
 #include "spx_fused_common.hpp"
+#include "spx_mlp.hpp"
 
 #include <cmath>
 
@@ -393,6 +394,10 @@ __global__ void __launch_bounds__(LS_T) k_ls_gather(FusedArgs a, const float2* s
       if (b + step < nblk && off[b + step] <= s) b += step;
     const uint32_t k = gp(idx)[(c * nblk + b) * LS_CAP + (s - off[b])];
     write_record(out + s, k, gp(X)[k], ls_mag(X, k + N - 1, N), ls_mag(X, k, N), ls_mag(X, k + 1, N), a);
+    // learned refinement: the record's complex neighbourhood from the stored
+    // spectrum, for k_mlp_refine after this launch
+    if (a.mlp_feat != nullptr)
+      mlp_store_features([&](uint32_t i) { return gp(X)[i]; }, k, N, mlp_feat_row(a, c, (int)s));
   }
   if (tid == 0 && a.counts) {
     gp(a.counts)[2 * c + 0] = nout;
@@ -409,12 +414,13 @@ int spx_large_is(uint32_t nfft) { return nfft >= (1u << 19) && nfft <= (1u << 24
 // Workspace of the large-length path for `chunk` channels: two ping-pong
 // spectra, plus the peak-search partials (statistics of the last pass's
 // workgroups, at most N/4096 per channel; candidate lists of the blocks of
-// LS_BLK bins).
-uint64_t spx_large_workspace_bytes(uint32_t nfft, uint32_t chunk) {
+// LS_BLK bins) and, for learned refinement, 32 bytes of features per record
+// slot (learn_kcap slots per channel, 0 without).
+uint64_t spx_large_workspace_bytes(uint32_t nfft, uint32_t chunk, uint32_t learn_kcap) {
   using namespace spx;
   const uint64_t nblk = (nfft + LS_BLK - 1) / LS_BLK;
   return 2ull * chunk * nfft * 8 + (uint64_t)chunk * (nfft / 4096) * sizeof(LsPart) + (uint64_t)chunk * sizeof(LsChan) +
-         (uint64_t)chunk * nblk * 4 * (1 + LS_CAP) + 1024;
+         (uint64_t)chunk * nblk * 4 * (1 + LS_CAP) + (uint64_t)chunk * learn_kcap * 32 + 1024;
 }
 
 // Radix plan of N = 2^lg: the register pass 2^r0, then the LDS passes 2^l1
@@ -484,7 +490,12 @@ hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, const float2*
   uint32_t* cnt = reinterpret_cast<uint32_t*>(tail);
   tail += ((uint64_t)chunk * nblk * 4 + 255) & ~255ull;
   uint32_t* idx = reinterpret_cast<uint32_t*>(tail);
+  tail += ((uint64_t)chunk * nblk * 4 * LS_CAP + 255) & ~255ull;
   const bool peaks = a0->peak_mode != 0;
+  // learned refinement (threshold search): features of the chunk's records
+  const bool learn = peaks && a0->peak_mode == 1 && a0->refine == 2 && a0->mlp != nullptr && a0->kcap > 0 &&
+                     a0->peaks != nullptr;
+  void* feat = learn ? reinterpret_cast<void*>(tail) : nullptr;
   // table offsets (float2 units): A1 | B1 | A2 | B2
   const uint64_t a1 = 0, b1 = a1 + (uint64_t)R0 * (L1 / 16), a2 = b1 + 6ull * R0,
                  b2 = a2 + (uint64_t)R0 * L1 * (L2 / 16);
@@ -534,10 +545,16 @@ hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, const float2*
       b.counts = a0->counts + 2ull * c0;
       b.gmax = a0->gmax ? a0->gmax + c0 : nullptr;
       b.stats = a0->stats ? a0->stats + 4ull * c0 : nullptr;
+      b.mlp_feat = feat;
       hipLaunchKernelGGL(k_ls_thresh, dim3(nch), dim3(LS_T), 0, st, b, spec, part, nparts, chs);
       hipLaunchKernelGGL(k_ls_cand, dim3((uint32_t)nblk, nch), dim3(LS_T), 0, st, b, spec, chs, (uint32_t)nblk, cnt,
                          idx);
       hipLaunchKernelGGL(k_ls_gather, dim3(nch), dim3(LS_T), 0, st, b, spec, cnt, idx, (uint32_t)nblk);
+      if (learn) {
+        const hipError_t e = spx_launch_mlp_refine(a0->mlp, feat, b.peaks, b.counts, nch, a0->kcap,
+                                                   a0->fs / (float)N, st);
+        if (e != hipSuccess) return e;
+      }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

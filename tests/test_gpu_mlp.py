@@ -163,11 +163,12 @@ def test_fused_learned_many_peaks_matches_predict(ctx):
     assert np.allclose(L["refined_frequency"], (L["index"] + got) / N, rtol=1e-5, atol=1e-7)
 
 
-@pytest.mark.parametrize("N", [1 << 15, 1 << 16, 1 << 17, 1 << 18])
+@pytest.mark.parametrize("N", [1 << 15, 1 << 16, 1 << 17, 1 << 18, 1 << 19, 1 << 20])
 def test_split_learned_matches_predict(ctx, N):
     """Learned refinement on the split long-channel path (features from the
     row kernel's candidate neighbourhoods and complex block-edge rows, the
-    refiner once per chunk).  Records equal the parabolic ones except the
+    refiner once per chunk) and, from 2^19, on the Stockham path (features
+    from the stored chunk spectrum in the gather kernel).  Records equal the parabolic ones except the
     offset, which is the network's prediction on the features of the
     spectrum (the split dense transform, same arithmetic) wherever it lies in
     [-0.5, 0.5].  Two regimes: tones on row-block edges (k2 = k mod N2 in
