@@ -278,12 +278,16 @@ int run_fused(spx_ctx* c, FusedArgs& a) {
   if (spx_large_is(a.nfft)) {
     // 2^19 .. 2^24: three Stockham passes through global memory (two of them
     // L-point transforms in LDS tiles), peak search on the stored spectrum
-    int r0, l1, l2;
-    if (spx_large_plan(a.nfft, &r0, &l1, &l2)) return fail(c, SPX_EINVAL, "no large-length plan for this nfft");
-    const void *t, *t1, *t2;
+    int r0, lgs[3], np;
+    if (spx_large_plan(a.nfft, &r0, lgs, &np)) return fail(c, SPX_EINVAL, "no large-length plan for this nfft");
+    const void* t;
+    const float2* twt[3] = {nullptr, nullptr, nullptr};
     if ((rc = get_table(c, LG, a.nfft, 0, &t))) return rc;
-    if ((rc = get_table(c, TW, 1u << l1, 0, &t1))) return rc;
-    if ((rc = get_table(c, TW, 1u << l2, 0, &t2))) return rc;
+    for (int p = 0; p < np; ++p) {
+      const void* tp;
+      if ((rc = get_table(c, TW, 1u << lgs[p], 0, &tp))) return rc;
+      twt[p] = (const float2*)tp;
+    }
     const uint32_t lk = (a.refine == 2 && a.mlp != nullptr && a.peak_mode == 1) ? a.kcap : 0u;
     const uint64_t per = spx_large_workspace_bytes(a.nfft, 1, lk);
     const uint64_t budget = c->ws_fixed ? c->ws_fixed_bytes : (2ull << 30);
@@ -291,8 +295,8 @@ int run_fused(spx_ctx* c, FusedArgs& a) {
     if (chunk == 0) return ensure_scratch(c, per);  // the fixed workspace is too small: reports the shortfall
     if ((rc = ensure_scratch(c, spx_large_workspace_bytes(a.nfft, chunk, lk)))) return rc;
     SPX_HIP(c, hipSetDevice(c->device));
-    SPX_HIP(c, spx_launch_large(&a, (const float2*)t, (const float2*)t1, (const float2*)t2,
-                                c->ws_fixed ? c->ws_fixed : c->scratch, chunk, c->stream));
+    SPX_HIP(c, spx_launch_large(&a, (const float2*)t, twt, c->ws_fixed ? c->ws_fixed : c->scratch, chunk,
+                                c->stream));
     return SPX_OK;
   }
   if (spx_fused_is_large(a.nfft)) {

@@ -109,10 +109,10 @@ uint64_t spx_fs_dense_workspace_bytes(uint32_t nfft, uint32_t chunk);
 hipError_t spx_launch_fourstep_dense(const FusedArgs* a, void* ws, uint32_t chunk, hipStream_t st);
 int spx_large_is(uint32_t nfft);
 uint64_t spx_large_workspace_bytes(uint32_t nfft, uint32_t chunk, uint32_t learn_kcap);
-int spx_large_plan(uint32_t nfft, int* r0, int* l1, int* l2);
+int spx_large_plan(uint32_t nfft, int* r0, int* lgs, int* np);
 uint64_t spx_large_tables(uint32_t nfft, float* out);
-hipError_t spx_launch_large(const FusedArgs* a, const float2* tw, const float2* tw1, const float2* tw2, void* ws,
-                            uint32_t chunk, hipStream_t st);
+hipError_t spx_launch_large(const FusedArgs* a, const float2* tw, const float2* const* twt, void* ws, uint32_t chunk,
+                            hipStream_t st);
 int spx_twiddle_table_len(uint32_t L, int V);
 hipError_t spx_launch_mag_phase(const void* in, void* out, uint64_t total, hipStream_t st);
 hipError_t spx_launch_magnitude(const void* in, float* out, uint64_t total, float inv_n, int squared,

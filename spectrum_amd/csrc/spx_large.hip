@@ -423,28 +423,33 @@ uint64_t spx_large_workspace_bytes(uint32_t nfft, uint32_t chunk, uint32_t learn
          (uint64_t)chunk * nblk * 4 * (1 + LS_CAP) + (uint64_t)chunk * learn_kcap * 32 + 1024;
 }
 
-// Radix plan of N = 2^lg: the register pass 2^r0, then the LDS passes 2^l1
-// and 2^l2 (r0 + l1 + l2 = lg; a 1024-point pass only where 512 does not
-// reach: one workgroup per CU instead of two).
-int spx_large_plan(uint32_t nfft, int* r0, int* l1, int* l2) {
+// Radix plan of N = 2^lg: the register pass 2^r0, then np LDS passes of
+// 2^lgs[i] points (r0 + sum lgs = lg).  Three passes at every length: four
+// passes at 2^23 and 2^24 (16 x 256 x 128 x 16, 16 x 256 x 256 x 16, the
+// last a 16-point pass over 256 columns) instead of the one-workgroup-per-CU
+// 1024-point passes measured slower (512 MiB: complex spectrum 0.846 and
+// 0.864 ms against 0.744 and 0.803 ms).
+int spx_large_plan(uint32_t nfft, int* r0, int* lgs, int* np) {
   if (!spx_large_is(nfft)) return -1;
   int lg = 0;
   while ((1u << lg) < nfft) ++lg;
-  static const int tab[6][3] = {{3, 8, 8}, {4, 8, 8}, {4, 9, 8}, {4, 9, 9}, {4, 10, 9}, {4, 10, 10}};
-  *r0 = tab[lg - 19][0];
-  *l1 = tab[lg - 19][1];
-  *l2 = tab[lg - 19][2];
+  static const int tab[6][5] = {{3, 2, 8, 8, 0}, {4, 2, 8, 8, 0}, {4, 2, 9, 8, 0},
+                                {4, 2, 9, 9, 0}, {4, 2, 10, 9, 0}, {4, 2, 10, 10, 0}};
+  const int* t = tab[lg - 19];
+  *r0 = t[0];
+  *np = t[1];
+  for (int i = 0; i < 3; ++i) lgs[i] = t[2 + i];
   return 0;
 }
 
-// Stockham twiddles of the two LDS passes, in float pairs (cos, -sin),
-// generated in FP64 and rounded once: per pass (NS = product of the earlier
-// radices, L its radix) the table A[jj][k] = W_(NS L)^(jj k) (jj < L/16,
+// Stockham twiddles of the LDS passes, in float pairs (cos, -sin), generated
+// in FP64 and rounded once: per pass (NS = product of the earlier radices, L
+// its radix) the table A[jj][k] = W_(NS L)^(jj k) (jj < max(L/16, 1),
 // k < NS), then B[i][k] = W_(16 NS)^(m_i k) with m = 1, 2, 3, 4, 8, 12.
 // Returns the number of floats; fills `out` when it is not null.
 uint64_t spx_large_tables(uint32_t nfft, float* out) {
-  int r0, l1, l2;
-  if (spx_large_plan(nfft, &r0, &l1, &l2)) return 0;
+  int r0, lgs[3], np;
+  if (spx_large_plan(nfft, &r0, lgs, &np)) return 0;
   uint64_t n = 0;
   auto put = [&](uint64_t e, uint64_t m) {
     if (out) {
@@ -454,11 +459,10 @@ uint64_t spx_large_tables(uint32_t nfft, float* out) {
     }
     ++n;
   };
-  const int lgs[2] = {l1, l2};
   uint64_t ns = 1ull << r0;
-  for (int p = 0; p < 2; ++p) {
+  for (int p = 0; p < np; ++p) {
     const uint64_t L = 1ull << lgs[p];
-    for (uint64_t jj = 0; jj < L / 16; ++jj)
+    for (uint64_t jj = 0; jj < (L >= 16 ? L / 16 : 1); ++jj)
       for (uint64_t k = 0; k < ns; ++k) put(jj * k, ns * L);
     static const int pw[6] = {1, 2, 3, 4, 8, 12};
     for (int i = 0; i < 6; ++i)
@@ -470,17 +474,18 @@ uint64_t spx_large_tables(uint32_t nfft, float* out) {
 
 // Transform (and, for a->peak_mode != 0, the peak search) of a->batch
 // channels of length a->nfft in chunks of `chunk` channels.  tw: the
-// Stockham tables of spx_large_tables; tw1, tw2: the plan (L, 0) tables of
-// the two LDS passes' L-point transforms, as make_twiddles builds them.
-hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, const float2* tw1, const float2* tw2, void* ws,
-                            uint32_t chunk, hipStream_t st) {
+// Stockham tables of spx_large_tables; twt[i]: the plan (L, 0) table of LDS
+// pass i's L-point transform, as make_twiddles builds it.
+hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, const float2* const* twt, void* ws, uint32_t chunk,
+                            hipStream_t st) {
   using namespace spx;
   const uint32_t N = a0->nfft;
-  int r0, l1, l2;
-  if (spx_large_plan(N, &r0, &l1, &l2) || chunk == 0) return hipErrorInvalidValue;
-  const uint32_t R0 = 1u << r0, L1 = 1u << l1, L2 = 1u << l2;
+  int r0, lgs[3], np;
+  if (spx_large_plan(N, &r0, lgs, &np) || chunk == 0) return hipErrorInvalidValue;
+  const uint32_t R0 = 1u << r0;
   const uint64_t nblk = (N + LS_BLK - 1) / LS_BLK;
-  const uint32_t cols2 = gl_cols(l2), nparts = N / ((1u << l2) * cols2);  // last-pass workgroups per channel
+  const int lgl = lgs[np - 1];
+  const uint32_t nparts = N / ((1u << lgl) * gl_cols(lgl));  // last-pass workgroups per channel
   float2* buf[2] = {reinterpret_cast<float2*>(ws), reinterpret_cast<float2*>(ws) + (uint64_t)chunk * N};
   unsigned char* tail = reinterpret_cast<unsigned char*>(ws) + 2ull * chunk * N * 8;
   LsPart* part = reinterpret_cast<LsPart*>(tail);
@@ -496,13 +501,21 @@ hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, const float2*
   const bool learn = peaks && a0->peak_mode == 1 && a0->refine == 2 && a0->mlp != nullptr && a0->kcap > 0 &&
                      a0->peaks != nullptr;
   void* feat = learn ? reinterpret_cast<void*>(tail) : nullptr;
-  // table offsets (float2 units): A1 | B1 | A2 | B2
-  const uint64_t a1 = 0, b1 = a1 + (uint64_t)R0 * (L1 / 16), a2 = b1 + 6ull * R0,
-                 b2 = a2 + (uint64_t)R0 * L1 * (L2 / 16);
+  // table offsets (float2 units) of each LDS pass: A | B
+  uint64_t ta[3], tb[3], off = 0, ns = R0;
+  for (int p = 0; p < np; ++p) {
+    const uint64_t L = 1ull << lgs[p];
+    ta[p] = off;
+    off += ns * (L >= 16 ? L / 16 : 1);
+    tb[p] = off;
+    off += 6 * ns;
+    ns *= L;
+  }
   for (uint32_t c0 = 0; c0 < a0->batch; c0 += chunk) {
     const uint32_t nch = min(chunk, a0->batch - c0);
-    // pass 0 -> buf[0] -> pass 1 -> buf[1] -> pass 2 -> the dense outputs,
-    // or for the peak search the chunk's complex spectrum in buf[0]
+    // pass 0 -> buf[0]; LDS pass p (1-based) reads buf[(p-1) % 2] and writes
+    // buf[p % 2], the last one the dense outputs or, for the peak search,
+    // the chunk's complex spectrum into the buffer it does not read
     GsArgs g;
     g.a = *a0;
     g.ch0 = c0;
@@ -512,32 +525,29 @@ hipError_t spx_launch_large(const FusedArgs* a0, const float2* tw, const float2*
       case 8: launch_gs_first<8>(g, nch, st); break;
       default: launch_gs_first<16>(g, nch, st); break;
     }
-    GlArgs h;
-    h.a = *a0;
-    h.ch0 = c0;
-    h.N = N;
-    h.src = buf[0];
-    h.dst = buf[1];
-    h.NS = R0;
-    h.tw_tile = tw1;
-    h.ta = tw + a1;
-    h.tb = tw + b1;
-    h.part = nullptr;
-    launch_gl_any(l1, h, false, nch, st);
-    float2* spec = buf[0];
-    if (peaks) {  // the chunk's spectrum, indexed from its first channel
-      h.a.spec_out = spec;
-      h.a.mag_out = nullptr;
-      h.ch0 = 0;
+    float2* spec = buf[np % 2];
+    uint32_t nsp = R0;
+    for (int p = 0; p < np; ++p) {
+      const bool last = p == np - 1;
+      GlArgs h;
+      h.a = *a0;
+      h.ch0 = c0;
+      h.N = N;
+      h.src = buf[p % 2];
+      h.dst = last ? nullptr : buf[(p + 1) % 2];
+      h.NS = nsp;
+      h.tw_tile = twt[p];
+      h.ta = tw + ta[p];
+      h.tb = tw + tb[p];
+      h.part = (last && peaks) ? part : nullptr;
+      if (last && peaks) {  // the chunk's spectrum, indexed from its first channel
+        h.a.spec_out = spec;
+        h.a.mag_out = nullptr;
+        h.ch0 = 0;
+      }
+      launch_gl_any(lgs[p], h, last, nch, st);
+      nsp <<= lgs[p];
     }
-    h.src = buf[1];
-    h.dst = nullptr;
-    h.NS = R0 * L1;
-    h.tw_tile = tw2;
-    h.ta = tw + a2;
-    h.tb = tw + b2;
-    h.part = peaks ? part : nullptr;
-    launch_gl_any(l2, h, true, nch, st);
     if (peaks) {
       FusedArgs b = *a0;
       b.batch = nch;
